@@ -1,0 +1,143 @@
+/*
+ * vhci.h -- C ABI of libvhci, the MI355X heat-bath VCI engine.
+ *
+ * The reference program (berkelbach-group/VHCI) has no plugin or FFI layer:
+ * its hot path is four C++ functions over process globals, called from
+ * Perform_HCI (src/HB.cpp:176-265) and main (src/VHCI.cpp:39-137).  Each
+ * entry point below replaces one of them (or the globals they read) and is
+ * called by the `vhci` driver in the same order.
+ *
+ * Conventions
+ *   - Quanta are int32 rows, N x M row-major (BasisSet, VCI_headers.h:149).
+ *   - Eigenvector blocks are row-major N x ncol (column j = state j).
+ *   - Every function returns VHCI_OK (0) or a negative VHCI_E* code;
+ *     vhci_last_error(ctx) / vhci_problem_error() give the message, which
+ *     for the reference's own failure cases is the reference's text.
+ *   - A context owns its device state on its own HIP stream.  Calls on one
+ *     context must not overlap; different contexts are independent.
+ *   - Precision forms: VHCI_FP64 stores H and runs every product in FP64;
+ *     VHCI_MIXED stores H as FP32 hi + FP32 lo, runs the bulk of the
+ *     eigensolver and PT2 numerators in FP32, and the final eigen-refinement,
+ *     PT2 denominators and sums in FP64.
+ */
+#ifndef VHCI_H
+#define VHCI_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VHCI_OK 0
+#define VHCI_E_ARG -1      /* bad argument or call order */
+#define VHCI_E_INPUT -2    /* input file / checkpoint error */
+#define VHCI_E_DEVICE -3   /* HIP / device error */
+#define VHCI_E_CONVERGE -4 /* eigensolver did not converge */
+#define VHCI_E_LIMIT -5    /* size limit (modes, quanta, states) */
+
+#define VHCI_FP64 0
+#define VHCI_MIXED 1
+
+typedef struct vhci_problem vhci_problem;
+typedef struct vhci_ctx vhci_ctx;
+
+/* ---- problem (replaces ReadCIInput and its globals) ------------------- */
+
+/* Parse an input file: src/Input_Reader.cpp:170-492 (header, modes, force
+ * constants, ScaleFC :494-515, order split :273-291, initial basis
+ * :296-364, AnharmHB :366-460).  restart != 0 builds only the ground state,
+ * as the reference does when a checkpoint exists. */
+int vhci_problem_load(const char* path, int restart, vhci_problem** out);
+/* Same, from an in-memory copy of the file text. */
+int vhci_problem_parse(const char* text, int restart, vhci_problem** out);
+void vhci_problem_free(vhci_problem* p);
+const char* vhci_problem_error(void);
+
+/* Integer settings: 0 M, 1 NEig, 2 Ntot, 3 PT2 flag, 4 N0 (initial basis
+ * size), 5 highest FC order, 6 number of force constants, 7 heat-bath list
+ * length, 8 number of connection patterns, 9 element FCs.
+ * Real settings: 0 eps1 (HCI_Eps), 1 eps2 (PT2_Eps). */
+int64_t vhci_problem_int(const vhci_problem* p, int what);
+double vhci_problem_real(const vhci_problem* p, int what);
+int vhci_problem_freqs(const vhci_problem* p, double* freqs);
+int vhci_problem_initial_basis(const vhci_problem* p, int32_t* q);
+/* Force-constant lists as the reference holds them: which = 0 AnharmFC,
+ * 1 Cubic, 2 Quartic, 3 Quintic, 4 Sextic, 5 AnharmHB (construction order),
+ * 6 AnharmHB sorted by |f| (HB.cpp:17-22, stable).  sm/mp receive up to 6
+ * entries (ShortModes, ModePowers). */
+int vhci_problem_fc_count(const vhci_problem* p, int which);
+int vhci_problem_fc(const vhci_problem* p, int which, int idx, double* fc, int* order, int* nsm, int* sm,
+                    int* mp);
+/* Ladder path sum <qe| (a+a^dag)^k |qs> in the reference's accumulation
+ * order (CreationLO/AnnihilationLO/AnharmPot, src/Ham.cpp:16-104). */
+double vhci_ladder_sum(int k, int qs, int qe);
+
+/* ---- device context ---------------------------------------------------- */
+
+int vhci_ctx_create(const vhci_problem* p, int form, int device, vhci_ctx** out);
+void vhci_ctx_destroy(vhci_ctx* ctx);
+const char* vhci_last_error(const vhci_ctx* ctx);
+
+/* BasisSet (VCI_headers.h:149): replace / read the variational basis. */
+int vhci_set_basis(vhci_ctx* ctx, const int32_t* q, int64_t n);
+int64_t vhci_basis_size(const vhci_ctx* ctx);
+int vhci_get_basis(vhci_ctx* ctx, int32_t* q);
+
+/* MakeHamSparse (src/Ham.cpp:328-344): build H on the device for the
+ * current basis.  stats (nullable, 4 entries): n, stored off-diagonal
+ * nonzeros, structural nonzeros of the reference's triplet list (-1 if not
+ * computed), build time in microseconds. */
+int vhci_build_h(vhci_ctx* ctx, int64_t* stats);
+/* Copy H to the host: rowptr[n+1], col/val[nnz] (off-diagonal entries in
+ * stored order), diag[n].  Pass rowptr=NULL to query nnz via *nnz. */
+int vhci_get_h(vhci_ctx* ctx, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val, double* diag);
+
+/* SparseDiagonalize (src/Ham.cpp:347-370): the nev lowest eigenpairs,
+ * converged to ||H x - theta x|| < tol * max(eps^(2/3), |theta|).  E[nev]
+ * ascending; C (nullable) row-major n x nev.  The Ritz block stays on the
+ * device and warm-starts the next call after the basis grows. */
+int vhci_eigsolve(vhci_ctx* ctx, int nev, double tol, int maxit, double* E, double* C);
+/* DenseDiagonalize (src/Ham.cpp:372-379) for the first HCI iteration
+ * (src/HB.cpp:180-194): all n eigenpairs; C (nullable) n x n. */
+int vhci_dense_diag(vhci_ctx* ctx, double* E, double* C);
+/* Resident eigenvectors: column count, download, upload (checkpoint). */
+int vhci_ritz_cols(const vhci_ctx* ctx);
+int vhci_get_vectors(vhci_ctx* ctx, double* C, int ncol);
+int vhci_set_vectors(vhci_ctx* ctx, const double* C, int ncol);
+
+/* The AddStatesHB loop of Perform_HCI (src/HB.cpp:204-223): Cmax over the
+ * first nopt columns (or the given cmax[n]), then every basis state's
+ * heat-bath candidates with |Cmax_n f| >= eps.  New states are kept on the
+ * device in first-discovery order; *nnew receives their count. */
+int vhci_hb_select(vhci_ctx* ctx, const double* cmax, int nopt, double eps, int64_t* nnew);
+int vhci_get_new_states(vhci_ctx* ctx, int32_t* q);
+/* BasisSet.push_back of the new states (src/HB.cpp:229-231). */
+int vhci_append_new_states(vhci_ctx* ctx);
+
+/* DoPT2 (src/PT2.cpp:16-154): external space by heat bath with eps2 from
+ * the resident vectors, then dE[n] = sum_a (sum_i H_ai C_in)^2 / (E_n - E_a)
+ * for n < nopt.  *ns2 (nullable) receives the external-space size. */
+int vhci_pt2(vhci_ctx* ctx, const double* E, int nopt, double eps2, double* dE, int64_t* ns2);
+
+/* Time b-vector SpMMs with the resident H (reps launches, HIP events on
+ * the context stream).  out[0] = ms per launch, out[1] = algorithmic bytes
+ * per launch. fp32 != 0 selects the mixed form's FP32 product. */
+int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out);
+
+/* ---- whole solve (Perform_HCI + DoPT2 of one input) --------------------
+ * Runs the reference's control flow (src/VHCI.cpp:39-114) on a device
+ * context and returns the final energies: E[0..nE) (after PT2 if enabled),
+ * *nE, the final variational basis size *nvar and *ns2.  timings (nullable,
+ * 8 entries, ms): total, H build, eigensolve, heat bath, PT2 select,
+ * PT2 sum, iterations, SpMM vectors. */
+int vhci_solve(const vhci_problem* p, int form, int device, double tol, double* E, int* nE, int64_t* nvar,
+               int64_t* ns2, double* timings);
+
+/* The drop-in program: same flags, input, freqs.dat and checkpoint formats
+ * and log lines as the reference's `vhci -n -i -o -c`. argv as in main. */
+int vhci_main(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,237 @@
+"""GPU parity: each stage of the device path against the CPU oracle on the
+same inputs, and the drop-in program against the reference's fixtures."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, absolute, manifest, read_freqs
+
+import vhci_amd
+from vhci_amd import synth
+
+pyoracle = pytest.importorskip("oracle.pyoracle")
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def c1_iter1(tmp_path_factory):
+    """Synthetic workload c1 after one heat-bath iteration (oracle)."""
+    d = tmp_path_factory.mktemp("c1")
+    path = str(d / "c1.inp")
+    synth.config_input(1, path)
+    o = pyoracle.OProblem(path)
+    q0 = o.initial_basis()
+    H = o.build_h_dense(q0)
+    w, Z = pyoracle.eigh(H)
+    nopt = min(o.neig, q0.shape[0])
+    cm = pyoracle.cmax(Z, nopt)
+    new = o.hb_select(q0, cm, o.eps1)
+    return path, q0, cm, new
+
+
+def _keys(rp, col):
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(rp))
+    return rows * n + col.astype(np.int64), rows
+
+
+def _compare_h(ctx, oprob, q, mixed=False):
+    """Device CSR vs the oracle's all-pairs CSR: diagonal bit for bit; every
+    device entry is an oracle entry with the same value (FP64) or within
+    2^-46 (hi+lo of the mixed form); every nonzero oracle off-diagonal entry
+    is on the device (the oracle also keeps the reference's explicit zeros)."""
+    rp, col, val, diag = ctx.get_h()
+    orp, ocol, oval = oprob.build_h(q)
+    n = q.shape[0]
+    ok, orows = _keys(orp, ocol)
+    isdiag = orows == ocol
+    odiag = np.zeros(n)
+    odiag[orows[isdiag]] = oval[isdiag]
+    assert isdiag.sum() == n
+    assert np.array_equal(diag, odiag)
+    ok, oval = ok[~isdiag], oval[~isdiag]
+    order = np.argsort(ok)
+    ok, oval = ok[order], oval[order]
+    dk, drows = _keys(rp, col)
+    assert not np.any(drows == col)
+    assert len(np.unique(dk)) == len(dk)
+    pos = np.searchsorted(ok, dk)
+    assert np.all(pos < len(ok)) and np.array_equal(ok[pos], dk)
+    if mixed:
+        assert np.all(np.abs(val - oval[pos]) <= 2.0 ** -46 * np.abs(oval[pos]))
+    else:
+        bad = np.nonzero(val != oval[pos])[0]
+        assert len(bad) == 0, (dk[bad[:5]], val[bad[:5]], oval[pos][bad[:5]])
+    nz = ok[oval != 0.0]
+    assert np.all(np.isin(nz, dk))
+    return len(dk)
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_h_build_clo2(form):
+    path = os.path.join(GOLDEN, "clo2_vci.inp")
+    _say("H build ClO2 VCI N=3276 form", form)
+    p = vhci_amd.Problem(path)
+    ctx = vhci_amd.Context(p, form)
+    st = ctx.build_h()
+    assert st["n"] == 3276
+    nnz = _compare_h(ctx, pyoracle.OProblem(path), p.initial_basis(), mixed=form == vhci_amd.MIXED)
+    assert nnz == st["nnz_offdiag"]
+
+
+def test_h_build_c1_grown(c1_iter1):
+    path, q0, cm, new = c1_iter1
+    q = np.vstack([q0, new])
+    _say("H build c1 after one HB iteration N=", q.shape[0])
+    p = vhci_amd.Problem(path)
+    ctx = vhci_amd.Context(p)
+    ctx.set_basis(q)
+    ctx.build_h()
+    _compare_h(ctx, pyoracle.OProblem(path), q)
+
+
+def test_hb_select_matches_oracle_order(c1_iter1):
+    """Same Cmax in, same new states out, in the same (first-discovery) order."""
+    path, q0, cm, new = c1_iter1
+    _say("HB select c1 initial basis")
+    p = vhci_amd.Problem(path)
+    ctx = vhci_amd.Context(p)
+    nn = ctx.hb_select(p.eps1, cmax=cm)
+    assert nn == new.shape[0]
+    assert np.array_equal(ctx.new_states(nn), new)
+    # ClO2 with random Cmax and a small threshold
+    pc = os.path.join(GOLDEN, "clo2_vhci.inp")
+    o = pyoracle.OProblem(pc)
+    q = o.initial_basis()
+    rng = np.random.default_rng(11)
+    c = rng.standard_normal(q.shape[0]) * np.exp(-rng.uniform(0, 10, q.shape[0]))
+    ref = o.hb_select(q, c, 0.003)
+    ctx2 = vhci_amd.Context(vhci_amd.Problem(pc))
+    n2 = ctx2.hb_select(0.003, cmax=c)
+    assert np.array_equal(ctx2.new_states(n2), ref)
+    # eps = 0 keeps every connected state
+    ref0 = o.hb_select(q, c, 0.0)
+    n3 = ctx2.hb_select(0.0, cmax=c)
+    assert np.array_equal(ctx2.new_states(n3), ref0)
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_eigsolve_c0(tmp_path, form):
+    """Workload c0 (N = 1820, 50 states): Davidson vs the oracle's dense
+    eigensolver on the oracle's H; residuals checked with the oracle H."""
+    path = str(tmp_path / "c0.inp")
+    synth.config_input(0, path)
+    _say("eigsolve c0 form", form)
+    p = vhci_amd.Problem(path)
+    o = pyoracle.OProblem(path)
+    q = p.initial_basis()
+    ctx = vhci_amd.Context(p, form)
+    ctx.build_h()
+    E, C = ctx.eigsolve(50, tol=1e-12)
+    Hd = o.build_h_dense(q)
+    w, _ = pyoracle.eigh(Hd)
+    assert np.max(np.abs(E - w[:50])) < 1e-8
+    R = Hd @ C - C * E
+    assert np.max(np.linalg.norm(R, axis=0)) < 1e-6
+    assert np.allclose(C.T @ C, np.eye(50), atol=1e-10)
+
+
+def test_dense_diag_matches_oracle():
+    path = os.path.join(GOLDEN, "clo2_vhci.inp")
+    p = vhci_amd.Problem(path)
+    o = pyoracle.OProblem(path)
+    ctx = vhci_amd.Context(p)
+    ctx.build_h()
+    E, C = ctx.dense_diag()
+    w, Z = pyoracle.eigh(o.build_h_dense(p.initial_basis()))
+    assert np.max(np.abs(E - w)) < 1e-9
+    ov = np.abs(np.sum(C[:, :20] * Z[:, :20], axis=0))
+    assert np.allclose(ov, 1.0, atol=1e-9)
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_pt2_matches_oracle(form):
+    """ClO2 PT2 (eps2 = 0, all connected states): same external space size
+    and corrections as the oracle for the same vectors and energies."""
+    path = os.path.join(GOLDEN, "clo2_pt2.inp")
+    o = pyoracle.OProblem(path)
+    q = o.initial_basis()
+    H = o.build_h_dense(q)
+    w, Z = pyoracle.eigh(H)
+    nopt = 20
+    dE_ref, ns2_ref = o.pt2(q, Z[:, :nopt], w[:nopt], nopt, 0.0)
+    p = vhci_amd.Problem(path)
+    ctx = vhci_amd.Context(p, form)
+    ctx.set_vectors(Z[:, :nopt])
+    dE, ns2 = ctx.pt2(w, nopt, 0.0)
+    assert ns2 == ns2_ref
+    tol = 1e-9 if form == vhci_amd.FP64 else 1e-4
+    assert np.max(np.abs(dE - dE_ref)) < tol
+    assert dE[0] < 0  # the ground state lies below every external state
+
+
+def _run_cli(tmp_path, name, env=None):
+    out = str(tmp_path / (name + ".freqs"))
+    e = dict(os.environ)
+    if env:
+        e.update(env)
+    r = subprocess.run([vhci_amd.CLI_PATH, "-n", "4", "-i", os.path.join(GOLDEN, name + ".inp"), "-o", out],
+                       capture_output=True, text=True, env=e, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return out, r.stdout
+
+
+@pytest.mark.parametrize("name", ["clo2_vci", "clo2_vhci", "clo2_pt2"])
+def test_cli_clo2_fixtures(tmp_path, name):
+    """The drop-in `vhci` on the ClO2 examples: energies within 1e-8 cm^-1 of
+    the reference's freqs.dat, identical assignment strings, and the same
+    basis sizes / PT2 space in the log."""
+    _say("CLI", name)
+    out, log = _run_cli(tmp_path, name)
+    e, a = read_freqs(out)
+    er, ar = read_freqs(os.path.join(GOLDEN, name + ".freqs"))
+    assert np.max(np.abs(absolute(e) - absolute(er))) < 1e-8
+    assert a == ar
+    pins = manifest()[name]
+    sizes = [int(x) for x in re.findall(r"Basis size after iteration \d+ is: (\d+)", log)]
+    assert sizes == pins["iterations"]
+    m = re.search(r"Perturbative space contains (\d+) states", log)
+    if "pt2_space" in pins:
+        assert int(m.group(1)) == pins["pt2_space"]
+
+
+def test_cli_mixed_matches_fp64(tmp_path):
+    """Mixed form on the ClO2 PT2 example: same basis sizes and PT2 space,
+    every energy within 0.01 cm^-1 (observed far tighter) of the FP64 run."""
+    o64, l64 = _run_cli(tmp_path, "clo2_pt2")
+    e64, _ = read_freqs(o64)
+    os.rename(o64, o64 + ".fp64")
+    o32, l32 = _run_cli(tmp_path, "clo2_pt2", env={"VHCI_FORM": "mixed"})
+    e32, _ = read_freqs(o32)
+    grab = lambda s: re.findall(r"(Basis size after iteration \d+ is: \d+|Perturbative space contains \d+)", s)
+    assert grab(l64) == grab(l32)
+    assert np.max(np.abs(absolute(e64) - absolute(e32))) < 1e-2
+
+
+def test_solve_c0_vs_oracle(tmp_path):
+    """Whole solve of workload c0 on the device (both forms) against the
+    oracle's energies."""
+    path = str(tmp_path / "c0.inp")
+    synth.config_input(0, path)
+    js = str(tmp_path / "c0.json")
+    pyoracle.run(path, str(tmp_path / "c0.freqs"), threads=8, json_out=js)
+    import json
+    ref = np.array(json.load(open(js))["E"])
+    p = vhci_amd.Problem(path)
+    for form in (vhci_amd.FP64, vhci_amd.MIXED):
+        r = vhci_amd.solve(p, form)
+        assert r["nvar"] == 1820
+        assert np.max(np.abs(r["E"] - ref[:len(r["E"])])) < 1e-8

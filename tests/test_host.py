@@ -1,0 +1,109 @@
+"""Host-side logic of libvhci (no GPU calls): the C ABI loads and exports
+every declared symbol; input parsing, force-constant scaling, the heat-bath
+list, the initial basis and the ladder sums are bit-identical to the oracle."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN
+
+import vhci_amd
+from vhci_amd import synth
+
+pyoracle = pytest.importorskip("oracle.pyoracle")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    with open(os.path.join(ROOT, "include", "vhci.h")) as f:
+        txt = f.read()
+    return sorted(set(re.findall(r"\b(vhci_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_header_symbols():
+    L = vhci_amd.lib()
+    names = header_functions()
+    assert len(names) >= 30
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(vhci_amd.EXPORTED)
+
+
+def _inputs(tmp_path):
+    files = [os.path.join(GOLDEN, f) for f in ("clo2_vci.inp", "clo2_vhci.inp", "acn_vhci.inp")]
+    files += [os.path.join(GOLDEN, "fields", f) for f in sorted(os.listdir(os.path.join(GOLDEN, "fields")))]
+    for c in (0, 1, 2):
+        p = str(tmp_path / ("c%d.inp" % c))
+        synth.config_input(c, p)
+        files.append(p)
+    return files
+
+
+def test_parser_matches_oracle(tmp_path):
+    """Every FC list (AnharmFC, the four order lists, AnharmHB in
+    construction and sorted order) equal value for value, bit for bit."""
+    for path in _inputs(tmp_path):
+        p = vhci_amd.Problem(path)
+        o = pyoracle.OProblem(path)
+        assert p.M == o.M and p.n0 == o.n0 and p.fcmax == o.fcmax
+        assert p.eps1 == o.eps1 and p.eps2 == o.eps2
+        for which in range(7):
+            a = p.fc_list(which)
+            b = o.fc_list(which)
+            assert len(a) == len(b), (path, which)
+            for x, y in zip(a, b):
+                assert x[0] == y[0] and x[1] == y[1] and x[2] == y[2], (path, which, x, y)
+        assert np.array_equal(p.initial_basis(), o.initial_basis())
+
+
+def test_restart_builds_ground_state_only():
+    p = vhci_amd.Problem(os.path.join(GOLDEN, "clo2_vhci.inp"), restart=True)
+    assert p.n0 == 1
+    assert np.array_equal(p.initial_basis(), np.zeros((1, 3), dtype=np.int32))
+
+
+def test_ladder_sum_bit_exact():
+    for k in range(7):
+        for qs in range(0, 40):
+            for qe in range(max(0, qs - k), qs + k + 1):
+                assert vhci_amd.ladder_sum(k, qs, qe) == pyoracle.coeffsum(k, qs, qe)
+    assert vhci_amd.ladder_sum(4, 3, 3) == pytest.approx(6 * 9 + 18 + 3)
+
+
+def test_parser_errors(tmp_path):
+    bad_mode = tmp_path / "bad.inp"
+    bad_mode.write_text("c\nHCI_Eps: 0\nNstate: 2\nPT2: 0\nPT2_Eps: 0\nMax_quanta: 2\nModes: 2\n"
+                        " 0 100.0 3\n 2 200.0 3\nForce_constants: 0\n")
+    with pytest.raises(vhci_amd.VhciError, match="Expected mode 1 but read data for mode 2"):
+        vhci_amd.Problem(str(bad_mode))
+    bad_order = tmp_path / "bad2.inp"
+    bad_order.write_text("c\nHCI_Eps: 0\nNstate: 2\nPT2: 0\nPT2_Eps: 0\nMax_quanta: 2\nModes: 1\n"
+                         " 0 100.0 3\nForce_constants: 1\n 2 0 0 5.0\n")
+    with pytest.raises(vhci_amd.VhciError, match="only supported up to 6th order"):
+        vhci_amd.Problem(str(bad_order))
+    with pytest.raises(vhci_amd.VhciError, match="Could not open input file"):
+        vhci_amd.Problem(str(tmp_path / "missing.inp"))
+
+
+def test_pt2_flag_zeroes_eps2():
+    txt = open(os.path.join(GOLDEN, "clo2_vhci.inp")).read().replace("PT2_Eps: 0", "PT2_Eps: 0.5")
+    p = vhci_amd.Problem(text=txt)
+    assert not p.perturb and p.eps2 == 0.0
+
+
+def test_synthetic_generator_deterministic(tmp_path):
+    a = synth.config_input(0)
+    b = synth.config_input(0)
+    assert a == b
+    p = vhci_amd.Problem(text=a)
+    assert p.M == 12 and p.n0 == math.comb(16, 4) == 1820
+    assert p.nfc == 364 + 210
+    p1 = vhci_amd.Problem(text=synth.config_input(1))
+    assert p1.M == 15 and p1.n0 == 136 and p1.nfc == 680 + 600
+    # first draws of the xoshiro256** stream seeded by splitmix64(0)
+    r = synth.Rng(0)
+    assert [r.next() for _ in range(3)] == [11091344671253066420, 13793997310169335082, 1900383378846508768]

@@ -1,0 +1,277 @@
+"""vhci_amd -- MI355X heat-bath VCI engine.
+
+Python binding (ctypes) of ``libvhci.so``, the C ABI declared in
+``include/vhci.h``.  The library runs every stage of the select -> build ->
+diagonalize -> PT2 loop on the GPU; there is no host fallback: loading fails
+loudly when the library has not been built, and every call raises when the
+device reports an error.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvhci.so")
+CLI_PATH = os.path.join(_HERE, "bin", "vhci")
+
+FP64 = 0
+MIXED = 1
+
+_lib = None
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes); mirrors include/vhci.h
+_SIGS = {
+    "vhci_problem_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "vhci_problem_parse": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "vhci_problem_free": (None, [_vp]),
+    "vhci_problem_error": (ctypes.c_char_p, []),
+    "vhci_problem_int": (ctypes.c_int64, [_vp, ctypes.c_int]),
+    "vhci_problem_real": (ctypes.c_double, [_vp, ctypes.c_int]),
+    "vhci_problem_freqs": (ctypes.c_int, [_vp, _dp]),
+    "vhci_problem_initial_basis": (ctypes.c_int, [_vp, _i32p]),
+    "vhci_problem_fc_count": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "vhci_problem_fc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _ip, _ip]),
+    "vhci_ladder_sum": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "vhci_ctx_create": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(_vp)]),
+    "vhci_ctx_destroy": (None, [_vp]),
+    "vhci_last_error": (ctypes.c_char_p, [_vp]),
+    "vhci_set_basis": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64]),
+    "vhci_basis_size": (ctypes.c_int64, [_vp]),
+    "vhci_get_basis": (ctypes.c_int, [_vp, _i32p]),
+    "vhci_build_h": (ctypes.c_int, [_vp, _i64p]),
+    "vhci_get_h": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _dp, _dp]),
+    "vhci_eigsolve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]),
+    "vhci_dense_diag": (ctypes.c_int, [_vp, _dp, _dp]),
+    "vhci_ritz_cols": (ctypes.c_int, [_vp]),
+    "vhci_get_vectors": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "vhci_set_vectors": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "vhci_hb_select": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _i64p]),
+    "vhci_get_new_states": (ctypes.c_int, [_vp, _i32p]),
+    "vhci_append_new_states": (ctypes.c_int, [_vp]),
+    "vhci_pt2": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _dp, _i64p]),
+    "vhci_time_spmm": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    "vhci_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _i64p, _i64p, _dp]),
+    "vhci_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
+}
+
+EXPORTED = tuple(_SIGS)
+
+
+class VhciError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libvhci.so (built in-tree); raises if it is missing."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise VhciError("libvhci.so is not built; run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+class Problem:
+    """A parsed input file (the reference's ReadCIInput globals)."""
+
+    def __init__(self, path=None, text=None, restart=False):
+        L = lib()
+        h = _vp()
+        if text is not None:
+            rc = L.vhci_problem_parse(text.encode(), int(restart), ctypes.byref(h))
+        else:
+            rc = L.vhci_problem_load(os.fsencode(path), int(restart), ctypes.byref(h))
+        if rc != 0:
+            raise VhciError(L.vhci_problem_error().decode())
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.vhci_problem_free(self.h)
+            self.h = None
+
+    def _i(self, k):
+        return int(lib().vhci_problem_int(self.h, k))
+
+    M = property(lambda s: s._i(0))
+    neig = property(lambda s: s._i(1))
+    ntot = property(lambda s: s._i(2))
+    perturb = property(lambda s: bool(s._i(3)))
+    n0 = property(lambda s: s._i(4))
+    fcmax = property(lambda s: s._i(5))
+    nfc = property(lambda s: s._i(6))
+    nhb = property(lambda s: s._i(7))
+    npatterns = property(lambda s: s._i(8))
+    eps1 = property(lambda s: float(lib().vhci_problem_real(s.h, 0)))
+    eps2 = property(lambda s: float(lib().vhci_problem_real(s.h, 1)))
+
+    def freqs(self):
+        f = np.zeros(self.M)
+        lib().vhci_problem_freqs(self.h, _ptr(f, _dp))
+        return f
+
+    def initial_basis(self):
+        q = np.zeros((self.n0, self.M), dtype=np.int32)
+        lib().vhci_problem_initial_basis(self.h, _ptr(q, _i32p))
+        return q
+
+    def fc_list(self, which):
+        """[(value, order, [(mode, power), ...]), ...] of one FC list."""
+        L = lib()
+        n = L.vhci_problem_fc_count(self.h, which)
+        out = []
+        fc = ctypes.c_double()
+        order = ctypes.c_int()
+        nsm = ctypes.c_int()
+        sm = (ctypes.c_int * 6)()
+        mp = (ctypes.c_int * 6)()
+        for i in range(n):
+            L.vhci_problem_fc(self.h, which, i, ctypes.byref(fc), ctypes.byref(order), ctypes.byref(nsm), sm, mp)
+            out.append((fc.value, order.value, [(sm[t], mp[t]) for t in range(nsm.value)]))
+        return out
+
+
+def ladder_sum(k, qs, qe):
+    return float(lib().vhci_ladder_sum(k, qs, qe))
+
+
+class Context:
+    """Device state of one solve (basis hash, H, Ritz block) on one GPU."""
+
+    def __init__(self, problem, form=FP64, device=0):
+        L = lib()
+        h = _vp()
+        rc = L.vhci_ctx_create(problem.h, form, device, ctypes.byref(h))
+        if rc != 0:
+            raise VhciError(L.vhci_problem_error().decode())
+        self.h = h
+        self.problem = problem
+        self.M = problem.M
+
+    def close(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.vhci_ctx_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VhciError("%s (code %d)" % (lib().vhci_last_error(self.h).decode(), rc))
+
+    @property
+    def n(self):
+        return int(lib().vhci_basis_size(self.h))
+
+    def set_basis(self, q):
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        self._check(lib().vhci_set_basis(self.h, _ptr(q, _i32p), q.shape[0]))
+
+    def basis(self):
+        q = np.zeros((self.n, self.M), dtype=np.int32)
+        self._check(lib().vhci_get_basis(self.h, _ptr(q, _i32p)))
+        return q
+
+    def build_h(self):
+        st = np.zeros(4, dtype=np.int64)
+        self._check(lib().vhci_build_h(self.h, _ptr(st, _i64p)))
+        return {"n": int(st[0]), "nnz_offdiag": int(st[1]), "nnz_struct": int(st[2]), "us": int(st[3])}
+
+    def get_h(self):
+        L = lib()
+        nnz = ctypes.c_int64()
+        self._check(L.vhci_get_h(self.h, ctypes.byref(nnz), None, None, None, None))
+        n = self.n
+        rp = np.zeros(n + 1, dtype=np.int64)
+        col = np.zeros(max(1, nnz.value), dtype=np.int32)
+        val = np.zeros(max(1, nnz.value))
+        diag = np.zeros(n)
+        self._check(L.vhci_get_h(self.h, ctypes.byref(nnz), _ptr(rp, _i64p), _ptr(col, _i32p), _ptr(val, _dp),
+                                 _ptr(diag, _dp)))
+        return rp, col[: nnz.value], val[: nnz.value], diag
+
+    def eigsolve(self, nev, tol=1e-12, maxit=500, vectors=True):
+        n = self.n
+        nev = min(nev, n)
+        E = np.zeros(nev)
+        C = np.zeros((n, nev)) if vectors else None
+        self._check(lib().vhci_eigsolve(self.h, nev, tol, maxit, _ptr(E, _dp), _ptr(C, _dp) if vectors else None))
+        return E, C
+
+    def dense_diag(self, vectors=True):
+        n = self.n
+        E = np.zeros(n)
+        C = np.zeros((n, n)) if vectors else None
+        self._check(lib().vhci_dense_diag(self.h, _ptr(E, _dp), _ptr(C, _dp) if vectors else None))
+        return E, C
+
+    def vectors(self, ncol=None):
+        ncol = ncol or int(lib().vhci_ritz_cols(self.h))
+        C = np.zeros((self.n, ncol))
+        self._check(lib().vhci_get_vectors(self.h, _ptr(C, _dp), ncol))
+        return C
+
+    def set_vectors(self, C):
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        self._check(lib().vhci_set_vectors(self.h, _ptr(C, _dp), C.shape[1]))
+
+    def hb_select(self, eps, nopt=None, cmax=None):
+        nnew = ctypes.c_int64()
+        if cmax is not None:
+            cmax = np.ascontiguousarray(cmax, dtype=np.float64)
+        self._check(lib().vhci_hb_select(self.h, _ptr(cmax, _dp) if cmax is not None else None, int(nopt or 0),
+                                         float(eps), ctypes.byref(nnew)))
+        return nnew.value
+
+    def new_states(self, count):
+        q = np.zeros((count, self.M), dtype=np.int32)
+        self._check(lib().vhci_get_new_states(self.h, _ptr(q, _i32p)))
+        return q
+
+    def append_new_states(self):
+        self._check(lib().vhci_append_new_states(self.h))
+
+    def pt2(self, E, nopt, eps2):
+        E = np.ascontiguousarray(E[:nopt], dtype=np.float64)
+        dE = np.zeros(nopt)
+        ns2 = ctypes.c_int64()
+        self._check(lib().vhci_pt2(self.h, _ptr(E, _dp), nopt, float(eps2), _ptr(dE, _dp), ctypes.byref(ns2)))
+        return dE, ns2.value
+
+    def time_spmm(self, b, reps=20, fp32=False):
+        out = np.zeros(2)
+        self._check(lib().vhci_time_spmm(self.h, b, reps, int(fp32), _ptr(out, _dp)))
+        return float(out[0]), float(out[1])
+
+
+def solve(problem, form=FP64, device=0, tol=1e-12):
+    """Whole select -> build -> diagonalize -> PT2 run of one problem."""
+    L = lib()
+    nmax = max(problem.neig, problem.n0, 1)
+    E = np.zeros(nmax)
+    nE = ctypes.c_int()
+    nvar = ctypes.c_int64()
+    ns2 = ctypes.c_int64()
+    t = np.zeros(8)
+    rc = L.vhci_solve(problem.h, form, device, tol, _ptr(E, _dp), ctypes.byref(nE), ctypes.byref(nvar),
+                      ctypes.byref(ns2), _ptr(t, _dp))
+    if rc != 0:
+        raise VhciError(L.vhci_problem_error().decode())
+    keys = ["total_ms", "build_ms", "eig_ms", "hb_ms", "pt2_select_ms", "pt2_sum_ms", "iterations", "spmm_vectors"]
+    return {"E": E[: nE.value].copy(), "nvar": nvar.value, "ns2": ns2.value, "timings": dict(zip(keys, t.tolist()))}

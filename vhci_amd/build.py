@@ -1,0 +1,54 @@
+"""Build libvhci.so and the vhci executable in-tree with hipcc for gfx950."""
+import os
+import shlex
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+SRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_obj")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+
+# -ffp-contract=off: matrix elements and thresholds must round like the
+# reference's host code; fused multiply-adds are written explicitly.
+# The packed-FP32 target feature is switched off for every HIP translation unit.
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function",
+          "-I", os.path.join(ROOT, "include")]
+DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+SOURCES = ["problem.cpp", "engine.hip", "abi.cpp"]
+
+
+def run(cmd):
+    print("+", " ".join(shlex.quote(c) for c in cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose=True):
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    for s in SOURCES:
+        src = os.path.join(SRC, s)
+        obj = os.path.join(OBJ, s + ".o")
+        cmd = [HIPCC] + COMMON + DEVICE
+        if s.endswith(".hip"):
+            cmd += ["-x", "hip"]
+        cmd += ["-c", src, "-o", obj]
+        run(cmd)
+        objs.append(obj)
+    lib = os.path.join(HERE, "libvhci.so")
+    run([HIPCC, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", lib] + objs +
+        ["-L", os.path.join(ROCM, "lib"), "-lrocsolver", "-lrocblas",
+         "-Wl,-rpath," + os.path.join(ROCM, "lib")])
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    run([HIPCC] + COMMON + ["-o", os.path.join(bindir, "vhci"), os.path.join(SRC, "vhci_cli.cpp"),
+                             "-L", HERE, "-lvhci", "-Wl,-rpath,$ORIGIN/.."])
+    return lib
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

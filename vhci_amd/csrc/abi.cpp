@@ -1,0 +1,738 @@
+// abi.cpp -- C ABI of libvhci and the drop-in `vhci` driver.
+//
+// The driver keeps the reference's control flow and log lines
+// (src/VHCI.cpp:5-185, src/HB.cpp:176-265, src/PT2.cpp:16-154) and replaces
+// its numerical kernels by the device engine.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <functional>
+#include <iomanip>
+#include <iostream>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/vhci.h"
+#include "engine.hpp"
+
+using vhci::Engine;
+using vhci::Problem;
+
+struct vhci_problem {
+    Problem P;
+};
+
+struct vhci_ctx {
+    std::unique_ptr<Engine> eng;
+    std::string err;
+};
+
+namespace {
+
+thread_local std::string g_problem_err;
+
+int classify(const std::string& m) {
+    if (m.find("did not converge") != std::string::npos) return VHCI_E_CONVERGE;
+    if (m.find("HIP error") != std::string::npos || m.find("rocBLAS") != std::string::npos ||
+        m.find("device") != std::string::npos)
+        return VHCI_E_DEVICE;
+    if (m.find("255 quanta") != std::string::npos || m.find("at most") != std::string::npos ||
+        m.find("larger than") != std::string::npos)
+        return VHCI_E_LIMIT;
+    return VHCI_E_ARG;
+}
+
+template <class F>
+int guard(vhci_ctx* ctx, F&& f) {
+    if (!ctx || !ctx->eng) return VHCI_E_ARG;
+    try {
+        f();
+        ctx->err.clear();
+        return VHCI_OK;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return classify(ctx->err);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// the iterative select -> build -> diagonalize -> PT2 loop
+// ---------------------------------------------------------------------------
+struct RunOpts {
+    double tol = 1e-12;
+    int maxit = 500;
+    std::string cpname = "none";
+    bool restart = false;
+    bool log = false;
+};
+
+struct RunResult {
+    std::vector<double> E;  // final energies (after PT2)
+    std::vector<double> Evar;
+    int64_t nvar = 0;
+    int64_t ns2 = 0;
+    double t_build = 0, t_eig = 0, t_hb = 0, t_pt2sel = 0, t_pt2sum = 0;
+    int iters = 0;
+    int64_t spmm_vectors = 0;
+};
+
+struct Logger {
+    bool on;
+    void operator()(const std::string& s) const {
+        if (on) {
+            std::cout << s;
+            std::cout.flush();
+        }
+    }
+};
+
+std::string fmt_fixed(double v, int prec) {
+    std::ostringstream os;
+    os << std::fixed << std::setprecision(prec) << v;
+    return os.str();
+}
+
+std::string sparse_line(const vhci::HStats& h) {
+    // "% sparse" (Ham.cpp:341-342) from the structural count when it was
+    // computed, else from the stored entries plus the diagonal.
+    double nnz = h.nnz_struct >= 0 ? (double)h.nnz_struct : (double)(h.nnz_offdiag + h.n);
+    double pct = 100 * (1. - nnz / ((double)h.n * (double)h.n));
+    return "The Hamiltonian is " + fmt_fixed(pct, 2) + "% sparse.\n";
+}
+
+void save_checkpoint(Engine& eng, const Problem& P, const std::vector<double>& E, const RunOpts& o,
+                     const Logger& log) {
+    std::ofstream cf(o.cpname, std::ios::out | std::ios::trunc);
+    if (!cf.good() || o.cpname == "none") {
+        log("\n Error: checkpoint file could not be found or no name was given.\n\n");
+        return;
+    }
+    const int64_t n = eng.n();
+    const int nc = eng.ritz_cols();
+    std::vector<int32_t> q((size_t)n * P.M);
+    std::vector<double> C((size_t)n * nc);
+    eng.get_basis(q.data());
+    eng.get_vectors(C.data(), nc);
+    vhci::write_checkpoint(cf, P, q.data(), n, E, C.data(), nc);
+}
+
+void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Logger& log) {
+    using clk = std::chrono::steady_clock;
+    std::vector<double> E;
+    if (P.eps1 == 0.) {
+        // one-shot VCI (VHCI.cpp:49-76)
+        if (!o.restart) {
+            log("Constructing the Vibrational CI Hamiltonian...");
+            auto hs = eng.build_h();
+            R.t_build += hs.build_ms;
+            log(sparse_line(hs));
+            if (P.neig == 1)
+                log("done.\n\nSolving for the ground state...\n\n");
+            else
+                log("done.\n\nSolving for the lowest " + std::to_string(P.neig) + " eigenstates...\n\n");
+            auto es = eng.eigsolve(P.neig, o.tol, o.maxit, E);
+            R.t_eig += es.ms;
+            R.spmm_vectors += es.spmm_vectors;
+            if (eng.n() > 100000) {
+                log("Writing checkpoint file...");
+                save_checkpoint(eng, P, E, o, log);
+                log("done.\n\n");
+            }
+        } else {
+            E = R.E;
+        }
+    } else {
+        // Perform_HCI (HB.cpp:176-265)
+        log("Beginning Heat Bath CI procedure...\n");
+        int iter = 1;
+        if (!o.restart) {
+            log(" Initial basis size is: " + std::to_string(eng.n()) + "\n");
+            log(" Constructing Hamiltonian...");
+            auto hs = eng.build_h();
+            R.t_build += hs.build_ms;
+            log(" Diagonalizing...\n");
+            auto es = eng.dense_all(E);
+            R.t_eig += es.ms;
+            log(" ZPE is: " + fmt_fixed(E[0], 2) + "\n\n");
+        } else {
+            E = R.E;
+        }
+        while (true) {
+            log(" Performing HCI iteration: " + std::to_string(iter) + "\n");
+            const int64_t n = eng.n();
+            const int nopt = (int)std::min<int64_t>(P.neig, n);
+            auto hb = eng.hb_select(nullptr, nopt, P.eps1);
+            R.t_hb += hb.ms;
+            if ((double)eng.new_count() / (double)n <= 0.01) {
+                log(" Basis converged after " + std::to_string(iter - 1) + " iterations and contains " +
+                    std::to_string(n) + " product states.\n");
+                break;
+            }
+            eng.append_new_states();
+            log(" Basis size after iteration " + std::to_string(iter) + " is: " + std::to_string(eng.n()) + "\n");
+            ++iter;
+            log(" Constructing Hamiltonian...");
+            auto hs = eng.build_h();
+            R.t_build += hs.build_ms;
+            log(sparse_line(hs));
+            if (P.neig == 1)
+                log(" Finding the ground state...\n");
+            else
+                log(" Finding the first " + std::to_string(P.neig) + " eigenstates...\n");
+            auto es = eng.eigsolve(P.neig, o.tol, o.maxit, E);
+            R.t_eig += es.ms;
+            R.spmm_vectors += es.spmm_vectors;
+            R.iters++;
+            log(" ZPE is: " + fmt_fixed(E[0], 2) + "\n\n");
+            if (eng.n() > 100000) {
+                log(" Writing checkpoint file...");
+                save_checkpoint(eng, P, E, o, log);
+                log("done.\n\n");
+            }
+        }
+    }
+    R.Evar = E;
+    R.nvar = eng.n();
+    if (P.perturb) {
+        const int64_t n = eng.n();
+        const int nopt = (int)std::min<int64_t>(P.neig, n);
+        std::vector<double> dE;
+        if (P.eps2 > 0) {
+            std::ostringstream os;
+            os << " Using heat bath sorting of perturbative states with cutoff: " << std::scientific
+               << std::setprecision(2) << P.eps2 << "\n";
+            log(os.str());
+        } else {
+            log(" Including all connected states in PT2 calculation.\n");
+        }
+        log(" Finding connected states...\n");
+        auto ps = eng.pt2(E, nopt, P.eps2, dE);
+        R.ns2 = ps.ns2;
+        R.t_pt2sel += ps.select_ms;
+        R.t_pt2sum += ps.sum_ms;
+        log(" Perturbative space contains " + std::to_string(ps.ns2) + " states.\n");
+        if (P.neig == 1)
+            log(" Calculating the 2nd-order perturbative correction on the ground state energy\n");
+        else
+            log(" Calculating the 2nd-order perturbative correction for on first " + std::to_string(nopt) +
+                " eigenvalues.\n");
+        for (int s = 0; s < nopt; s++) E[s] += dE[s];
+        log(" New ZPE after PT2 correction is: " + fmt_fixed(E[0], 2) + "\n");
+    }
+    R.E = E;
+}
+
+std::string time_str(double secs, std::string& units) {
+    double t = secs;
+    if (t >= 3600) {
+        t /= 3600;
+        units = "hours";
+    } else if (t >= 60) {
+        t /= 60;
+        units = "minutes";
+    } else {
+        units = "seconds";
+    }
+    return fmt_fixed(t, 2);
+}
+
+void banner() {
+    std::cout << '\n'
+              << "#############################################################################\n"
+              << "#                                                                           #\n"
+              << "#              Vibrational Heat-Bath Configuration Interaction              #\n"
+              << "#                         By: Jonathan H. Fetherolf                         #\n"
+              << "#                                                                           #\n"
+              << "#                      Based on LOVCI by Eric G. Kratz                      #\n"
+              << "#                                                                           #\n"
+              << "#############################################################################\n\n";
+}
+
+void usage_short(const char* why) {
+    std::cout << '\n' << why << "\n\nUsage: vhci -n Ncpus -i Modes.inp -o Freqs.dat -c Checkpoint.txt\n\n"
+              << "Use -h or --help for detailed instructions.\n\n";
+    std::cout.flush();
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- problem
+const char* vhci_problem_error(void) { return g_problem_err.c_str(); }
+
+int vhci_problem_load(const char* path, int restart, vhci_problem** out) {
+    if (!path || !out) return VHCI_E_ARG;
+    auto p = std::make_unique<vhci_problem>();
+    std::string e = vhci::load_input(path, restart != 0, p->P);
+    if (!e.empty()) {
+        g_problem_err = e;
+        return VHCI_E_INPUT;
+    }
+    *out = p.release();
+    return VHCI_OK;
+}
+
+int vhci_problem_parse(const char* text, int restart, vhci_problem** out) {
+    if (!text || !out) return VHCI_E_ARG;
+    auto p = std::make_unique<vhci_problem>();
+    std::istringstream in(text);
+    std::string e = vhci::parse_input(in, restart != 0, p->P);
+    if (!e.empty()) {
+        g_problem_err = e;
+        return VHCI_E_INPUT;
+    }
+    *out = p.release();
+    return VHCI_OK;
+}
+
+void vhci_problem_free(vhci_problem* p) { delete p; }
+
+int64_t vhci_problem_int(const vhci_problem* p, int what) {
+    if (!p) return -1;
+    const Problem& P = p->P;
+    switch (what) {
+        case 0: return P.M;
+        case 1: return P.neig;
+        case 2: return P.ntot;
+        case 3: return P.perturb ? 1 : 0;
+        case 4: return P.n0();
+        case 5: return P.fcmax;
+        case 6: return P.nfc;
+        case 7: return (int64_t)P.hb.size();
+        case 8:
+        case 9: {
+            vhci::Tables T;
+            vhci::build_tables(P, T);
+            return what == 8 ? (int64_t)T.deltas.size() : (int64_t)T.elem.size();
+        }
+    }
+    return -1;
+}
+
+double vhci_problem_real(const vhci_problem* p, int what) {
+    if (!p) return NAN;
+    return what == 0 ? p->P.eps1 : p->P.eps2;
+}
+
+int vhci_problem_freqs(const vhci_problem* p, double* f) {
+    if (!p || !f) return VHCI_E_ARG;
+    std::memcpy(f, p->P.freq.data(), sizeof(double) * p->P.M);
+    return VHCI_OK;
+}
+
+int vhci_problem_initial_basis(const vhci_problem* p, int32_t* q) {
+    if (!p || !q) return VHCI_E_ARG;
+    std::memcpy(q, p->P.basis0.data(), sizeof(int32_t) * p->P.basis0.size());
+    return VHCI_OK;
+}
+
+static const vhci::ForceConst* fc_at(const Problem& P, int which, int idx, int& count) {
+    const std::vector<int>* lst = nullptr;
+    switch (which) {
+        case 0: count = (int)P.fcs.size(); return (idx >= 0 && idx < count) ? &P.fcs[idx] : nullptr;
+        case 1: lst = &P.cubic; break;
+        case 2: lst = &P.quartic; break;
+        case 3: lst = &P.quintic; break;
+        case 4: lst = &P.sextic; break;
+        case 5: count = (int)P.hb.size(); return (idx >= 0 && idx < count) ? &P.hb[idx] : nullptr;
+        case 6:
+            count = (int)P.hb_sorted.size();
+            return (idx >= 0 && idx < count) ? &P.hb[P.hb_sorted[idx]] : nullptr;
+        default: count = -1; return nullptr;
+    }
+    count = (int)lst->size();
+    return (idx >= 0 && idx < count) ? &P.fcs[(*lst)[idx]] : nullptr;
+}
+
+int vhci_problem_fc_count(const vhci_problem* p, int which) {
+    if (!p) return VHCI_E_ARG;
+    int c = -1;
+    fc_at(p->P, which, -1, c);
+    return c < 0 ? VHCI_E_ARG : c;
+}
+
+int vhci_problem_fc(const vhci_problem* p, int which, int idx, double* fc, int* order, int* nsm, int* sm, int* mp) {
+    if (!p) return VHCI_E_ARG;
+    int c = 0;
+    const vhci::ForceConst* f = fc_at(p->P, which, idx, c);
+    if (!f) return VHCI_E_ARG;
+    *fc = f->fc;
+    *order = f->order();
+    *nsm = (int)f->sm.size();
+    for (int i = 0; i < *nsm && i < 6; i++) {
+        sm[i] = f->sm[i];
+        mp[i] = f->mp[i];
+    }
+    return VHCI_OK;
+}
+
+double vhci_ladder_sum(int k, int qs, int qe) { return vhci::ladder_sum(k, qs, qe); }
+
+// ---------------------------------------------------------------- context
+int vhci_ctx_create(const vhci_problem* p, int form, int device, vhci_ctx** out) {
+    if (!p || !out) return VHCI_E_ARG;
+    auto c = std::make_unique<vhci_ctx>();
+    try {
+        c->eng = std::make_unique<Engine>(p->P, form, device);
+        c->eng->set_basis(p->P.basis0.data(), p->P.n0());
+    } catch (const std::exception& e) {
+        g_problem_err = e.what();
+        return classify(g_problem_err);
+    }
+    *out = c.release();
+    return VHCI_OK;
+}
+
+void vhci_ctx_destroy(vhci_ctx* ctx) { delete ctx; }
+
+const char* vhci_last_error(const vhci_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int vhci_set_basis(vhci_ctx* ctx, const int32_t* q, int64_t n) {
+    return guard(ctx, [&] { ctx->eng->set_basis(q, n); });
+}
+
+int64_t vhci_basis_size(const vhci_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->n() : -1; }
+
+int vhci_get_basis(vhci_ctx* ctx, int32_t* q) {
+    return guard(ctx, [&] { ctx->eng->get_basis(q); });
+}
+
+int vhci_build_h(vhci_ctx* ctx, int64_t* stats) {
+    return guard(ctx, [&] {
+        auto h = ctx->eng->build_h();
+        if (stats) {
+            stats[0] = h.n;
+            stats[1] = h.nnz_offdiag;
+            stats[2] = h.nnz_struct;
+            stats[3] = (int64_t)(h.build_ms * 1000);
+        }
+    });
+}
+
+int vhci_get_h(vhci_ctx* ctx, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val, double* diag) {
+    return guard(ctx, [&] {
+        std::vector<int64_t> rp;
+        std::vector<int32_t> cl;
+        std::vector<double> vl, dg;
+        ctx->eng->get_h(rp, cl, vl, dg);
+        if (nnz) *nnz = (int64_t)cl.size();
+        if (rowptr) {
+            std::memcpy(rowptr, rp.data(), rp.size() * sizeof(int64_t));
+            std::memcpy(col, cl.data(), cl.size() * sizeof(int32_t));
+            std::memcpy(val, vl.data(), vl.size() * sizeof(double));
+            std::memcpy(diag, dg.data(), dg.size() * sizeof(double));
+        }
+    });
+}
+
+int vhci_eigsolve(vhci_ctx* ctx, int nev, double tol, int maxit, double* E, double* C) {
+    return guard(ctx, [&] {
+        std::vector<double> e;
+        ctx->eng->eigsolve(nev, tol, maxit, e);
+        std::memcpy(E, e.data(), e.size() * sizeof(double));
+        if (C) ctx->eng->get_vectors(C, (int)e.size());
+    });
+}
+
+int vhci_dense_diag(vhci_ctx* ctx, double* E, double* C) {
+    return guard(ctx, [&] {
+        std::vector<double> e;
+        ctx->eng->dense_all(e);
+        std::memcpy(E, e.data(), e.size() * sizeof(double));
+        if (C) ctx->eng->get_vectors(C, (int)e.size());
+    });
+}
+
+int vhci_ritz_cols(const vhci_ctx* ctx) { return ctx && ctx->eng ? ctx->eng->ritz_cols() : -1; }
+
+int vhci_get_vectors(vhci_ctx* ctx, double* C, int ncol) {
+    return guard(ctx, [&] { ctx->eng->get_vectors(C, ncol); });
+}
+
+int vhci_set_vectors(vhci_ctx* ctx, const double* C, int ncol) {
+    return guard(ctx, [&] { ctx->eng->set_vectors(C, ncol); });
+}
+
+int vhci_hb_select(vhci_ctx* ctx, const double* cmax, int nopt, double eps, int64_t* nnew) {
+    return guard(ctx, [&] {
+        ctx->eng->hb_select(cmax, nopt, eps);
+        if (nnew) *nnew = ctx->eng->new_count();
+    });
+}
+
+int vhci_get_new_states(vhci_ctx* ctx, int32_t* q) {
+    return guard(ctx, [&] { ctx->eng->get_new_states(q); });
+}
+
+int vhci_append_new_states(vhci_ctx* ctx) {
+    return guard(ctx, [&] { ctx->eng->append_new_states(); });
+}
+
+int vhci_pt2(vhci_ctx* ctx, const double* E, int nopt, double eps2, double* dE, int64_t* ns2) {
+    return guard(ctx, [&] {
+        std::vector<double> e(E, E + nopt), d;
+        auto st = ctx->eng->pt2(e, nopt, eps2, d);
+        std::memcpy(dE, d.data(), nopt * sizeof(double));
+        if (ns2) *ns2 = st.ns2;
+    });
+}
+
+int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
+    return guard(ctx, [&] {
+        double bytes = 0;
+        out[0] = ctx->eng->time_spmm(b, reps, fp32 != 0, &bytes);
+        out[1] = bytes;
+    });
+}
+
+int vhci_solve(const vhci_problem* p, int form, int device, double tol, double* E, int* nE, int64_t* nvar,
+               int64_t* ns2, double* timings) {
+    if (!p) return VHCI_E_ARG;
+    auto t0 = std::chrono::steady_clock::now();
+    try {
+        Engine eng(p->P, form, device);
+        eng.set_basis(p->P.basis0.data(), p->P.n0());
+        RunOpts o;
+        if (tol > 0) o.tol = tol;
+        RunResult R;
+        run(eng, p->P, o, R, Logger{false});
+        if (E) std::memcpy(E, R.E.data(), R.E.size() * sizeof(double));
+        if (nE) *nE = (int)R.E.size();
+        if (nvar) *nvar = R.nvar;
+        if (ns2) *ns2 = R.ns2;
+        if (timings) {
+            timings[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            timings[1] = R.t_build;
+            timings[2] = R.t_eig;
+            timings[3] = R.t_hb;
+            timings[4] = R.t_pt2sel;
+            timings[5] = R.t_pt2sum;
+            timings[6] = R.iters;
+            timings[7] = (double)R.spmm_vectors;
+        }
+    } catch (const std::exception& e) {
+        g_problem_err = e.what();
+        return classify(g_problem_err);
+    }
+    return VHCI_OK;
+}
+
+// ---------------------------------------------------------------- driver
+int vhci_main(int argc, char** argv) {
+    const auto start = std::chrono::steady_clock::now();
+    std::cout << std::fixed << std::setprecision(12);
+    banner();
+    std::cout << "Last modification: " << __TIME__ << " on " << __DATE__ << "\n\n";
+    std::cout << "Reading input...\n";
+    // ReadCIArgs (Input_Reader.cpp:17-168)
+    if (argc == 1) {
+        usage_short("Missing arguments...");
+        return 1;
+    }
+    if ((argc % 2) != 1) {
+        std::string a1 = argv[1];
+        if (a1 != "-h" && a1 != "--help") {
+            usage_short("Wrong number of arguments...");
+            return 1;
+        }
+    }
+    int ncpus = 0;
+    std::string in_path, out_path, cpname = "none";
+    bool have_in = false, have_out = false;
+    for (int i = 0; i < argc; i++) {
+        std::string a = argv[i];
+        if (a == "-h" || a == "--help") {
+            std::cout << "\nUsage: vhci -n Ncpus -i Modes.inp -o Freqs.txt\n\nCommand line arguments:\n\n"
+                      << "  -n    Number of CPUs to use for the calculation.\n\n"
+                      << "  -i    Input file.\n\n"
+                      << "  -o    Output file for the first NEig VCI frequencies.\n\n"
+                      << "  -c    Checkpoint file for HCI iterations on large systems.\n\n";
+            std::cout.flush();
+            return 0;
+        }
+        if (i + 1 >= argc) continue;
+        if (a == "-n") ncpus = std::atoi(argv[i + 1]);
+        if (a == "-i") {
+            in_path = argv[i + 1];
+            have_in = true;
+        }
+        if (a == "-o") {
+            out_path = argv[i + 1];
+            have_out = true;
+        }
+        if (a == "-c") cpname = argv[i + 1];
+    }
+    const int maxthr = std::max(1u, std::thread::hardware_concurrency());
+    if (ncpus < 1) {
+        std::cout << " Warning: Calculations cannot run with " << ncpus << " CPUs.\n"
+                  << "  Do you know how computers work? Ncpus set to 1\n";
+        ncpus = 1;
+    }
+    if (ncpus > maxthr) {
+        std::cout << " Warning: Too many threads requested.\n  Requested: " << ncpus << ", Available: " << maxthr
+                  << "\n  Ncpus set to " << maxthr << "\n";
+        ncpus = maxthr;
+    }
+    std::ifstream vcidata;
+    std::ofstream freqfile;
+    if (have_in) vcidata.open(in_path);
+    if (have_out) freqfile.open(out_path);
+    bool quit = false;
+    if (!have_in || !vcidata.good()) {
+        std::cout << " Error: Could not open input file.\n";
+        quit = true;
+    }
+    if (!have_out || !freqfile.good()) {
+        std::cout << " Error: Could not output file.\n";
+        quit = true;
+    }
+    if (quit) {
+        std::cout << '\n';
+        std::cout.flush();
+        return 1;
+    }
+    std::cout << "\nNo fatal errors detected.\n And there was much rejoicing. Yay...\n\n";
+    bool restart = false;
+    if (cpname == "none") {
+        std::cout << "No checkpoint filename specified. Checkpoints will not be saved.\n\n";
+    } else {
+        std::cout << "Locating checkpoint file...\n";
+        std::ifstream cf(cpname);
+        restart = cf.good() && cf.peek() != std::ifstream::traits_type::eof();
+        if (restart)
+            std::cout << "Checkpoint file found. Calculation will be resumed.\n\n";
+        else
+            std::cout << "No checkpoint file found. Calculation will start from the beginning.\n\n";
+    }
+    std::cout.flush();
+    // ReadCIInput
+    Problem P;
+    std::string err = vhci::parse_input(vcidata, restart, P);
+    if (!err.empty()) {
+        std::cout << err << "\n\n";
+        return 1;
+    }
+    std::cout << "General settings:\n  CPU threads: " << ncpus << "\n\n  Normal modes: " << P.M
+              << "\n  Max quanta per state: " << P.ntot << "\n  Basis functions: " << P.n0()
+              << "\n  Force constants: ";
+    if (P.nfc == 0)
+        std::cout << "N/A\n\n";
+    else
+        std::cout << P.fcs.size() << "\n\n";
+    if (P.eps1 != 0.) {
+        std::cout << std::setprecision(3) << "  Using HCI with energy cutoff: " << P.eps1 << '\n';
+        if (P.neig == 1)
+            std::cout << "  Optimizing just the ground states (ZPE).\n";
+        else
+            std::cout << "  Optimizing the first " << P.neig << " eigenstates.\n";
+    }
+    std::cout << '\n' << std::setprecision(12);
+    std::cout.flush();
+    int form = VHCI_FP64;
+    if (const char* f = std::getenv("VHCI_FORM")) form = (std::string(f) == "mixed") ? VHCI_MIXED : VHCI_FP64;
+    int device = 0;
+    if (const char* d = std::getenv("VHCI_DEVICE")) device = std::atoi(d);
+    RunOpts o;
+    if (const char* t = std::getenv("VHCI_TOL")) o.tol = std::atof(t);
+    o.cpname = cpname;
+    o.restart = restart;
+    o.log = true;
+    Logger log{true};
+    RunResult R;
+    try {
+        Engine eng(P, form, device);
+        if (restart) {
+            std::cout << "\nLoading checkpoint file...";
+            std::cout.flush();
+            std::ifstream cf(cpname);
+            std::vector<int32_t> q;
+            std::vector<double> C;
+            std::string ce = vhci::read_checkpoint(cf, P, q, R.E, C);
+            if (!ce.empty()) {
+                std::cout << ce << "\n";
+                return 1;
+            }
+            const int64_t nb = (int64_t)q.size() / P.M;
+            eng.set_basis(q.data(), nb);
+            eng.set_vectors(C.data(), P.neig);
+            std::cout << "done.\n  Basis functions: " << nb << "\n" << std::endl;
+        } else {
+            eng.set_basis(P.basis0.data(), P.n0());
+        }
+        const bool pt = P.perturb;
+        if (pt) {
+            // the "Variational stage took" line is printed between the stages
+            RunOpts ov = o;
+            Problem Pv = P;
+            Pv.perturb = false;
+            run(eng, Pv, ov, R, log);
+            std::string units;
+            double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+            std::string t = time_str(secs, units);
+            std::cout << " Variational stage took " << t << " " << units << "\n\n";
+            std::cout << "Performing Epstein-Nesbet perturbation theory...\n";
+            std::cout.flush();
+            // PT2 on the converged space
+            const int64_t n = eng.n();
+            const int nopt = (int)std::min<int64_t>(P.neig, n);
+            std::vector<double> E = R.E, dE;
+            if (P.eps2 > 0) {
+                std::ostringstream os;
+                os << " Using heat bath sorting of perturbative states with cutoff: " << std::scientific
+                   << std::setprecision(2) << P.eps2 << "\n";
+                log(os.str());
+            } else {
+                log(" Including all connected states in PT2 calculation.\n");
+            }
+            log(" Finding connected states...\n");
+            auto ps = eng.pt2(E, nopt, P.eps2, dE);
+            log(" Perturbative space contains " + std::to_string(ps.ns2) + " states.\n");
+            if (P.neig == 1)
+                log(" Calculating the 2nd-order perturbative correction on the ground state energy\n");
+            else
+                log(" Calculating the 2nd-order perturbative correction for on first " + std::to_string(nopt) +
+                    " eigenvalues.\n");
+            for (int s = 0; s < nopt; s++) E[s] += dE[s];
+            log(" New ZPE after PT2 correction is: " + fmt_fixed(E[0], 2) + "\n");
+            R.E = E;
+        } else {
+            run(eng, P, o, R, log);
+        }
+        // PrintFreqs (VHCI.cpp:116-119)
+        const int64_t rows = eng.n();
+        const int ncol = std::min<int>(eng.ritz_cols(), std::max(1, std::min<int>(P.neig, (int)R.E.size())));
+        std::vector<int32_t> q((size_t)rows * P.M);
+        std::vector<double> C((size_t)rows * ncol);
+        eng.get_basis(q.data());
+        eng.get_vectors(C.data(), ncol);
+        vhci::write_freqs(freqfile, P, q.data(), rows, R.E, C.data(), ncol);
+    } catch (const std::exception& e) {
+        std::cout << "\n" << e.what() << "\n";
+        return 1;
+    }
+    double ezpe = R.E[0];
+    for (double e : R.E) ezpe = std::min(ezpe, e);
+    if (ezpe < 0) {
+        std::cout << "Error: The zero-point energy is negative. Something is wrong.\n"
+                  << "Try reducing the number of quanta or removing large anharmonic\n"
+                  << "force constants.\n\n";
+        return 1;
+    }
+    std::string units;
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - start).count();
+    std::string t = time_str(secs, units);
+    std::cout << "Printing the results...\n\nResults:\n  Zero-point energy: " << fmt_fixed(ezpe, 2)
+              << " (1/cm)\n  Run time: " << t << " " << units << "\n\nDone.\n\n";
+    std::cout.flush();
+    return 0;
+}
+
+}  // extern "C"

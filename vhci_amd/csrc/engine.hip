@@ -1,0 +1,1642 @@
+// engine.hip -- gfx950 kernels and device orchestration of the heat-bath VCI
+// engine: basis hash, Hamiltonian build, SpMM, block-Davidson eigensolver,
+// heat-bath selection and Epstein-Nesbet PT2.
+//
+// Compiled with -ffp-contract=off: every floating-point expression that must
+// reproduce the reference's rounding (matrix elements, diagonal energies,
+// selection thresholds) is written as separate multiplies and adds; fused
+// multiply-adds are requested explicitly with fma() where rounding is free.
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocsolver/rocsolver.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+
+#include "engine.hpp"
+
+namespace vhci {
+
+namespace {
+
+#define HIPCHK(x)                                                                                    \
+    do {                                                                                             \
+        hipError_t e_ = (x);                                                                         \
+        if (e_ != hipSuccess)                                                                        \
+            throw std::runtime_error(std::string("HIP error ") + hipGetErrorString(e_) + " at " +  \
+                                     __FILE__ + ":" + std::to_string(__LINE__));                    \
+    } while (0)
+#define BLASCHK(x)                                                                                   \
+    do {                                                                                             \
+        rocblas_status s_ = (x);                                                                     \
+        if (s_ != rocblas_status_success)                                                            \
+            throw std::runtime_error(std::string("rocBLAS/rocSOLVER status ") + std::to_string(s_) + \
+                                     " at " + __FILE__ + ":" + std::to_string(__LINE__));           \
+    } while (0)
+
+constexpr uint64_t kEmpty = ~0ull;
+constexpr int kWave = 64;
+
+using clk = std::chrono::steady_clock;
+double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
+
+unsigned grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > (1ll << 30)) g = 1ll << 30;
+    return (unsigned)g;
+}
+
+uint64_t next_pow2(uint64_t x) {
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+// ------------------------------------------------------------------------
+// key helpers: a key is NW 32-bit words (4 uint8 quanta per word)
+// ------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t fmix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdULL;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ULL;
+    x ^= x >> 33;
+    return x;
+}
+
+template <int NW>
+__device__ __forceinline__ uint64_t key_hash(const uint32_t (&w)[NW]) {
+    uint64_t h = 0x243F6A8885A308D3ULL;
+#pragma unroll
+    for (int i = 0; i < NW; i += 2) {
+        uint64_t v = (uint64_t)w[i] | ((uint64_t)w[i + 1] << 32);
+        h = (h ^ v) * 0x9E3779B97F4A7C15ULL;
+        h ^= h >> 29;
+    }
+    return fmix64(h);
+}
+
+template <int NW>
+__device__ __forceinline__ void key_load(const uint4* __restrict__ keys, int64_t i, uint32_t (&w)[NW]) {
+    const uint4* k = keys + i * (NW / 4);
+#pragma unroll
+    for (int t = 0; t < NW / 4; t++) {
+        uint4 v = k[t];
+        w[4 * t + 0] = v.x;
+        w[4 * t + 1] = v.y;
+        w[4 * t + 2] = v.z;
+        w[4 * t + 3] = v.w;
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ bool key_eq(const uint4* __restrict__ keys, uint32_t idx, const uint32_t (&w)[NW]) {
+    const uint4* k = keys + (size_t)idx * (NW / 4);
+    bool eq = true;
+#pragma unroll
+    for (int t = 0; t < NW / 4; t++) {
+        uint4 v = k[t];
+        eq = eq && v.x == w[4 * t] && v.y == w[4 * t + 1] && v.z == w[4 * t + 2] && v.w == w[4 * t + 3];
+    }
+    return eq;
+}
+
+template <int NW>
+__device__ __forceinline__ uint32_t key_word(const uint32_t (&w)[NW], int i) {
+    uint32_t r = w[0];
+#pragma unroll
+    for (int t = 1; t < NW; t++) r = (i == t) ? w[t] : r;
+    return r;
+}
+
+template <int NW>
+__device__ __forceinline__ int key_q(const uint32_t (&w)[NW], int m) {
+    return (int)((key_word<NW>(w, m >> 2) >> ((m & 3) * 8)) & 0xffu);
+}
+
+// add d to the quanta of mode m; the caller has checked 0 <= q+d <= 255,
+// so the byte add never carries or borrows into a neighbour.
+template <int NW>
+__device__ __forceinline__ void key_add(uint32_t (&w)[NW], int m, int d) {
+    const uint32_t add = ((uint32_t)d) << ((m & 3) * 8);
+    const int wi = m >> 2;
+#pragma unroll
+    for (int t = 0; t < NW; t++)
+        if (t == wi) w[t] += add;
+}
+
+template <int NW>
+__device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots, uint64_t mask,
+                                             const uint4* __restrict__ keys, const uint32_t (&w)[NW]) {
+    const uint64_t h = key_hash<NW>(w);
+    uint64_t pos = h & mask;
+    const uint32_t tag = (uint32_t)(h >> 32);
+    while (true) {
+        const uint64_t s = slots[pos];
+        if (s == kEmpty) return -1;
+        if ((uint32_t)(s >> 32) == tag && key_eq<NW>(keys, (uint32_t)s, w)) return (int64_t)(uint32_t)s;
+        pos = (pos + 1) & mask;
+    }
+}
+
+// ------------------------------------------------------------------------
+// small utility kernels
+// ------------------------------------------------------------------------
+template <class T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// copy an r x c block between row-major arrays with different leading dims
+__global__ void k_copy2d(const double* __restrict__ src, int lds, double* __restrict__ dst, int ldd, int64_t rows,
+                         int cols) {
+    int64_t total = rows * cols;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = t / cols;
+        int c = (int)(t - r * cols);
+        dst[r * ldd + c] = src[r * lds + c];
+    }
+}
+
+__global__ void k_to_f32(const double* __restrict__ src, int lds, float* __restrict__ dst, int ldd, int64_t rows,
+                         int cols) {
+    int64_t total = rows * cols;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = t / cols;
+        int c = (int)(t - r * cols);
+        dst[r * ldd + c] = (float)src[r * lds + c];
+    }
+}
+
+template <int NW>
+__global__ void k_hash_insert(const uint4* __restrict__ keys, int64_t i0, int64_t i1, uint64_t* slots, uint64_t mask) {
+    int64_t i = i0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= i1) return;
+    uint32_t w[NW];
+    key_load<NW>(keys, i, w);
+    const uint64_t h = key_hash<NW>(w);
+    uint64_t pos = h & mask;
+    const uint64_t val = (h & 0xffffffff00000000ULL) | (uint64_t)(uint32_t)i;
+    while (true) {
+        unsigned long long prev = atomicCAS((unsigned long long*)&slots[pos], (unsigned long long)kEmpty,
+                                            (unsigned long long)val);
+        if (prev == kEmpty) return;
+        pos = (pos + 1) & mask;
+    }
+}
+
+// ------------------------------------------------------------------------
+// Hamiltonian build (generate-and-lookup)
+// ------------------------------------------------------------------------
+struct HBuildArgs {
+    int64_t n;
+    int M;
+    int P;  // number of signed patterns
+    int nq;  // ladder table rows per power
+    const uint4* keys;
+    const uint64_t* slots;
+    uint64_t mask;
+    const DeltaEnt* deltas;
+    const GroupEnt* groups;
+    const uint32_t* fclist;
+    const ElemFC* elem;
+    const double* ladder;
+    int64_t* rowcnt;
+    const int64_t* rowptr;
+    int32_t* col;
+    double* val;
+    float* hi;
+    float* lo;
+    int form;
+};
+
+// sum over the group's FCs of fc * prod_t L[k][q_start][q_end], in the
+// element order (= the reference's Quartic/Sextic or Cubic/Quintic order)
+__device__ __forceinline__ double group_element(const GroupEnt ge, const uint32_t* __restrict__ fclist,
+                                                const ElemFC* __restrict__ elem, const double* __restrict__ L,
+                                                int nq, const uint8_t* __restrict__ qrow, const DeltaEnt& e, int nz,
+                                                bool row_is_start) {
+    double v = 0.0;
+    for (uint32_t f = 0; f < ge.fc_count; f++) {
+        const ElemFC fc = elem[fclist[ge.fc_start + f]];
+        double x = fc.fc;
+        for (int t = 0; t < fc.d; t++) {
+            const int m = fc.mode[t];
+            const int k = fc.pow[t];
+            const int qr = qrow[m];
+            int dq = 0;
+            for (int s = 0; s < nz; s++) dq = (e.mode[s] == m) ? (int)e.delta[s] : dq;
+            const int qs = row_is_start ? qr : qr + dq;
+            const int qe = row_is_start ? qr + dq : qr;
+            x = x * L[((size_t)k * nq + qs) * kLadderW + (qe - qs + k)];
+        }
+        v = v + x;
+    }
+    return v;
+}
+
+template <int NW, bool FILL>
+__global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
+    __shared__ uint8_t sq[64];
+    const int lane = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    uint32_t w[NW];
+    key_load<NW>(a.keys, row, w);
+    if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+    __syncthreads();
+    int64_t base = FILL ? a.rowptr[row] : 0;
+    int64_t cnt = 0;
+    for (int p0 = 0; p0 < a.P; p0 += kWave) {
+        const int p = p0 + lane;
+        bool hit = false;
+        int64_t j = -1;
+        DeltaEnt e;
+        int nz = 0;
+        if (p < a.P) {
+            e = a.deltas[p];
+            nz = (int)(e.meta >> 28);
+            uint32_t t[NW];
+#pragma unroll
+            for (int i = 0; i < NW; i++) t[i] = w[i];
+            bool ok = true;
+            for (int s = 0; s < nz; s++) {
+                const int m = e.mode[s];
+                const int nqv = (int)sq[m] + (int)e.delta[s];
+                ok = ok && nqv >= 0 && nqv <= 255;
+                key_add<NW>(t, m, e.delta[s]);
+            }
+            if (ok) {
+                j = hash_find<NW>(a.slots, a.mask, a.keys, t);
+                hit = j >= 0;
+            }
+        }
+        const uint64_t bal = __ballot(hit);
+        if (!FILL) {
+            cnt += __popcll(bal);
+        } else {
+            if (hit) {
+                const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+                const GroupEnt ge = a.groups[e.meta & 0x03ffffffu];
+                const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, row < j);
+                a.col[pos] = (int32_t)j;
+                if (a.form == kFP64) {
+                    a.val[pos] = v;
+                } else {
+                    const float h = (float)v;
+                    a.hi[pos] = h;
+                    a.lo[pos] = (float)(v - (double)h);
+                }
+            }
+            base += __popcll(bal);
+        }
+    }
+    if (!FILL && lane == 0) a.rowcnt[row] = cnt;
+}
+
+// H_ii = E_i + V_ii (Ham.cpp:203-226 plus the zero-pattern FCs), in the
+// reference's order: E_i summed over modes, V_ii summed over FCs, then added.
+template <int NW>
+__global__ void k_diag(int64_t n, int M, const uint4* __restrict__ keys, const double* __restrict__ freq,
+                       const uint32_t* __restrict__ dfc, int ndfc, const ElemFC* __restrict__ elem,
+                       const double* __restrict__ L, int nq, double* __restrict__ D) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t w[NW];
+    key_load<NW>(keys, i, w);
+    double e = 0.;
+    for (int j = 0; j < M; j++) {
+        double ej = 0.5;
+        ej = ej + (double)key_q<NW>(w, j);
+        ej = ej * freq[j];
+        e = e + ej;
+    }
+    double v = 0.0;
+    for (int f = 0; f < ndfc; f++) {
+        const ElemFC fc = elem[dfc[f]];
+        double x = fc.fc;
+        for (int t = 0; t < fc.d; t++) {
+            const int q = key_q<NW>(w, fc.mode[t]);
+            const int k = fc.pow[t];
+            x = x * L[((size_t)k * nq + q) * kLadderW + k];
+        }
+        v = v + x;
+    }
+    D[i] = e + v;
+}
+
+// ------------------------------------------------------------------------
+// SpMM: Y = (D + H_off) X, X/Y row-major; one wave per row, lanes split into
+// 64/B nonzero groups x B columns.
+// ------------------------------------------------------------------------
+template <int B>
+__global__ __launch_bounds__(256) void k_spmm64(int64_t n, const int64_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                const double* __restrict__ D, const double* __restrict__ X, int ldx,
+                                                double* __restrict__ Y, int ldy, int b) {
+    constexpr int G = kWave / B;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const int grp = lane / B, c = lane % B;
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    const double d = D[row];
+    for (int c0 = 0; c0 < b; c0 += B) {
+        const int cc = c0 + c;
+        const bool act = cc < b;
+        double acc = 0.0;
+        if (act)
+            for (int64_t k = k0 + grp; k < k1; k += G) acc = fma(val[k], X[(int64_t)col[k] * ldx + cc], acc);
+#pragma unroll
+        for (int off = B; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
+        if (grp == 0 && act) Y[row * ldy + cc] = fma(d, X[row * ldx + cc], acc);
+    }
+}
+
+// mixed form: FP32 values and FP32 X for the off-diagonal sum (FP32
+// accumulation), diagonal term in FP64
+template <int B>
+__global__ __launch_bounds__(256) void k_spmm32(int64_t n, const int64_t* __restrict__ rowptr,
+                                                const int32_t* __restrict__ col, const float* __restrict__ hi,
+                                                const double* __restrict__ D, const float* __restrict__ X32, int ldx32,
+                                                const double* __restrict__ X, int ldx, double* __restrict__ Y, int ldy,
+                                                int b) {
+    constexpr int G = kWave / B;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const int grp = lane / B, c = lane % B;
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    const double d = D[row];
+    for (int c0 = 0; c0 < b; c0 += B) {
+        const int cc = c0 + c;
+        const bool act = cc < b;
+        float acc = 0.0f;
+        if (act)
+            for (int64_t k = k0 + grp; k < k1; k += G) acc = fmaf(hi[k], X32[(int64_t)col[k] * ldx32 + cc], acc);
+#pragma unroll
+        for (int off = B; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
+        if (grp == 0 && act) Y[row * ldy + cc] = fma(d, X[row * ldx + cc], (double)acc);
+    }
+}
+
+// mixed form, refinement: value = hi + lo in FP64
+template <int B>
+__global__ __launch_bounds__(256) void k_spmm_hilo(int64_t n, const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const float* __restrict__ hi,
+                                                   const float* __restrict__ lo, const double* __restrict__ D,
+                                                   const double* __restrict__ X, int ldx, double* __restrict__ Y,
+                                                   int ldy, int b) {
+    constexpr int G = kWave / B;
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const int grp = lane / B, c = lane % B;
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    const double d = D[row];
+    for (int c0 = 0; c0 < b; c0 += B) {
+        const int cc = c0 + c;
+        const bool act = cc < b;
+        double acc = 0.0;
+        if (act)
+            for (int64_t k = k0 + grp; k < k1; k += G)
+                acc = fma((double)hi[k] + (double)lo[k], X[(int64_t)col[k] * ldx + cc], acc);
+#pragma unroll
+        for (int off = B; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
+        if (grp == 0 && act) Y[row * ldy + cc] = fma(d, X[row * ldx + cc], acc);
+    }
+}
+
+// ------------------------------------------------------------------------
+// Davidson helpers
+// ------------------------------------------------------------------------
+// partial column sums of (HX - X theta)^2 over row blocks; cols <= 256
+__global__ __launch_bounds__(256) void k_resid_partial(int64_t n, int b, const double* __restrict__ HX,
+                                                       const double* __restrict__ X, int ld,
+                                                       const double* __restrict__ theta, int64_t rows_per_blk,
+                                                       double* __restrict__ part) {
+    const int c = threadIdx.x;
+    if (c >= b) return;
+    const int64_t r0 = blockIdx.x * rows_per_blk;
+    const int64_t r1 = min(n, r0 + rows_per_blk);
+    const double th = theta[c];
+    double s = 0.0;
+    for (int64_t r = r0; r < r1; r++) {
+        const double x = HX[r * ld + c] - th * X[r * ld + c];
+        s = fma(x, x, s);
+    }
+    part[blockIdx.x * (int64_t)b + c] = s;
+}
+
+__global__ void k_reduce_cols(const double* __restrict__ part, int nblk, int b, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    double s = 0.0;
+    for (int i = 0; i < nblk; i++) s += part[(int64_t)i * b + c];
+    out[c] = s;
+}
+
+// T[:, jj] = (HX[:, J[jj]] - theta_J X[:, J[jj]]) / (theta_J - D)
+__global__ void k_precond(int64_t n, int k, const int* __restrict__ J, const double* __restrict__ HX,
+                          const double* __restrict__ X, int ldx, const double* __restrict__ theta,
+                          const double* __restrict__ D, double* __restrict__ T, int ldt) {
+    int64_t total = n * k;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / k;
+        const int jj = (int)(t - r * k);
+        const int j = J[jj];
+        const double th = theta[j];
+        const double res = HX[r * ldx + j] - th * X[r * ldx + j];
+        double den = th - D[r];
+        if (fabs(den) < 1e-6) den = den < 0 ? -1e-6 : 1e-6;
+        T[r * ldt + jj] = res / den;
+    }
+}
+
+__global__ void k_symmetrize(double* G, int m, int ld) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= m || j >= m || i <= j) return;
+    double a = G[(int64_t)j * ld + i], b = G[(int64_t)i * ld + j];
+    double s = 0.5 * (a + b);
+    G[(int64_t)j * ld + i] = s;
+    G[(int64_t)i * ld + j] = s;
+}
+
+// dense symmetric matrix (column-major n x n) from the CSR
+__global__ void k_densify(int64_t n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                          const double* __restrict__ val, const float* __restrict__ hi, const float* __restrict__ lo,
+                          const double* __restrict__ D, double* __restrict__ A) {
+    const int64_t i = blockIdx.x;
+    if (i >= n) return;
+    for (int64_t k = rowptr[i] + threadIdx.x; k < rowptr[i + 1]; k += blockDim.x) {
+        const double v = val ? val[k] : (double)hi[k] + (double)lo[k];
+        A[(int64_t)col[k] * n + i] = v;
+    }
+    if (threadIdx.x == 0) A[i * n + i] = D[i];
+}
+
+// column-major (ld) eigenvector block -> row-major n x ncol
+__global__ void k_colmajor_to_rows(const double* __restrict__ A, int64_t lda, int64_t n, int ncol,
+                                   double* __restrict__ X) {
+    int64_t total = n * ncol;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / ncol;
+        const int c = (int)(t - r * ncol);
+        X[t] = A[(int64_t)c * lda + r];
+    }
+}
+
+// ------------------------------------------------------------------------
+// heat-bath selection
+// ------------------------------------------------------------------------
+// Cmax (HB.cpp:204-214): signed entry of largest |C| over the first nopt
+// columns, earlier column wins ties
+__global__ void k_cmax(int64_t n, const double* __restrict__ X, int ld, int nopt, double* __restrict__ cm) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double c = X[i * ld];
+    for (int j = 1; j < nopt; j++) {
+        const double x = X[i * ld + j];
+        if (fabs(x) > fabs(c)) c = x;
+    }
+    cm[i] = c;
+}
+
+// number of candidates of state n: the FCs with |C_n f| >= eps form a prefix
+// of the |f|-sorted list (HB.cpp:25-26, 172); candidates = variants of that
+// prefix.
+__global__ void k_hb_count(int64_t n, const double* __restrict__ cm, const double* __restrict__ fcs, int nfc,
+                           const int64_t* __restrict__ varoff, double eps, int64_t* __restrict__ cnt) {
+    int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double c = cm[i];
+    int lo = 0, hi = nfc;  // first index where the test fails
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (fabs(c * fcs[mid]) >= eps)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    cnt[i] = varoff[lo];
+}
+
+struct HbGenArgs {
+    int64_t n;
+    int64_t total;
+    const int64_t* candoff;  // n+1
+    const uint4* keys;
+    const uint64_t* bslots;
+    uint64_t bmask;
+    const HbVar* vars;
+    uint64_t* nslots;
+    uint64_t nmask;
+    uint4* pool_keys;
+    uint64_t* pool_rank;
+    uint32_t* counters;  // [0] pool fill, [1] overflow flags
+    uint32_t pool_cap;
+};
+
+template <int NW>
+__device__ __forceinline__ bool pool_key_eq(const uint4* pool, uint32_t idx, const uint32_t (&w)[NW]) {
+    const uint64_t* p = reinterpret_cast<const uint64_t*>(pool + (size_t)idx * (NW / 4));
+    bool eq = true;
+#pragma unroll
+    for (int t = 0; t < NW / 2; t++) {
+        const uint64_t v = __hip_atomic_load(p + t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        eq = eq && v == ((uint64_t)w[2 * t] | ((uint64_t)w[2 * t + 1] << 32));
+    }
+    return eq;
+}
+
+// insert a candidate into the new-state hash; keeps the smallest rank per key
+template <int NW>
+__device__ void new_insert(const HbGenArgs& a, const uint32_t (&w)[NW], uint64_t rank) {
+    const uint64_t h = key_hash<NW>(w);
+    uint64_t pos = h & a.nmask;
+    const uint32_t tag = (uint32_t)(h >> 32);
+    int64_t mine = -1;
+    while (true) {
+        uint64_t s = __hip_atomic_load(&a.nslots[pos], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (s == kEmpty) {
+            if (mine < 0) {
+                const uint32_t id = atomicAdd(&a.counters[0], 1u);
+                if (id >= a.pool_cap) {
+                    atomicOr(&a.counters[1], 2u);
+                    return;
+                }
+                mine = id;
+                uint4* dst = a.pool_keys + (size_t)id * (NW / 4);
+#pragma unroll
+                for (int t = 0; t < NW / 4; t++) dst[t] = make_uint4(w[4 * t], w[4 * t + 1], w[4 * t + 2], w[4 * t + 3]);
+                a.pool_rank[id] = rank;
+                __threadfence();
+            }
+            const uint64_t want = ((uint64_t)tag << 32) | (uint64_t)(uint32_t)mine;
+            const unsigned long long prev = atomicCAS((unsigned long long*)&a.nslots[pos],
+                                                      (unsigned long long)kEmpty, (unsigned long long)want);
+            if (prev == kEmpty) return;
+            s = prev;
+        }
+        if ((uint32_t)(s >> 32) == tag && pool_key_eq<NW>(a.pool_keys, (uint32_t)s, w)) {
+            atomicMin((unsigned long long*)&a.pool_rank[(uint32_t)s], (unsigned long long)rank);
+            return;
+        }
+        pos = (pos + 1) & a.nmask;
+    }
+}
+
+// one thread per candidate g: g = (state n, sorted FC, variant) flattened in
+// the reference's loop order, so g itself is the first-discovery rank.
+template <int NW>
+__global__ __launch_bounds__(256) void k_hb_gen(HbGenArgs a) {
+    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < a.total;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        int64_t lo = 0, hi = a.n;  // largest n with candoff[n] <= g
+        while (hi - lo > 1) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (a.candoff[mid] <= g)
+                lo = mid;
+            else
+                hi = mid;
+        }
+        const int64_t n = lo;
+        const int64_t r = g - a.candoff[n];
+        const HbVar v = a.vars[r];
+        uint32_t w[NW];
+        key_load<NW>(a.keys, n, w);
+        bool ok = true;
+        bool over = false;
+        for (int t = 0; t < v.d; t++) {
+            const int q = key_q<NW>(w, v.mode[t]) + v.delta[t];
+            ok = ok && q >= 0;
+            over = over || q > 255;
+        }
+        if (!ok) continue;
+        if (over) {
+            atomicOr(&a.counters[1], 1u);
+            continue;
+        }
+        for (int t = 0; t < v.d; t++) key_add<NW>(w, v.mode[t], v.delta[t]);
+        if (hash_find<NW>(a.bslots, a.bmask, a.keys, w) >= 0) continue;
+        new_insert<NW>(a, w, (uint64_t)g);
+    }
+}
+
+__global__ void k_collect(const uint64_t* __restrict__ slots, uint64_t cap, const uint64_t* __restrict__ pool_rank,
+                          uint32_t* counter, uint64_t* __restrict__ rank_out, uint32_t* __restrict__ idx_out) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < cap; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = slots[i];
+        if (s == kEmpty) continue;
+        const uint32_t idx = (uint32_t)s;
+        const uint32_t o = atomicAdd(counter, 1u);
+        rank_out[o] = pool_rank[idx];
+        idx_out[o] = idx;
+    }
+}
+
+__global__ void k_gather_keys(const uint4* __restrict__ pool, const uint32_t* __restrict__ idx, int64_t u, int kw,
+                              uint4* __restrict__ out) {
+    int64_t total = u * kw;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / kw;
+        const int c = (int)(t - r * kw);
+        out[t] = pool[(size_t)idx[r] * kw + c];
+    }
+}
+
+// ------------------------------------------------------------------------
+// PT2: one wave per external state a (grid-stride); numerators over all
+// variational states i connected to a, E_a, and Kahan-compensated sums of
+// num^2 / (E_n - E_a) per column.
+// ------------------------------------------------------------------------
+struct Pt2Args {
+    int64_t ns2;
+    int M;
+    int P;
+    int nq;
+    int nopt;
+    const uint4* akeys;
+    const uint4* vkeys;
+    const uint64_t* vslots;
+    uint64_t vmask;
+    const DeltaEnt* deltas;
+    const GroupEnt* groups;
+    const uint32_t* fclist;
+    const ElemFC* elem;
+    const double* ladder;
+    const uint32_t* dfc;
+    int ndfc;
+    const double* freq;
+    const double* C;
+    const float* C32;
+    int ldc;
+    const double* E;
+    double* part;  // nwaves x nopt x 2
+};
+
+constexpr int kHitBuf = 512;
+
+template <int NW, bool MIXED>
+__global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
+    __shared__ uint8_t sq[64];
+    __shared__ int32_t hidx[kHitBuf];
+    __shared__ double hval[kHitBuf];
+    __shared__ double sdiag[64];
+    const int lane = threadIdx.x;
+    constexpr int R = 4;  // up to 256 columns
+    double ks[R], kc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) ks[r] = kc[r] = 0.0;
+    for (int64_t ia = blockIdx.x; ia < a.ns2; ia += gridDim.x) {
+        uint32_t w[NW];
+        key_load<NW>(a.akeys, ia, w);
+        if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+        __syncthreads();
+        double num[R];
+        float num32[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            num[r] = 0.0;
+            num32[r] = 0.0f;
+        }
+        int nh = 0;
+        for (int p0 = 0; p0 < a.P; p0 += kWave) {
+            const int p = p0 + lane;
+            bool hit = false;
+            int64_t j = -1;
+            DeltaEnt e;
+            int nz = 0;
+            if (p < a.P) {
+                e = a.deltas[p];
+                nz = (int)(e.meta >> 28);
+                uint32_t t[NW];
+#pragma unroll
+                for (int i = 0; i < NW; i++) t[i] = w[i];
+                bool ok = true;
+                for (int s = 0; s < nz; s++) {
+                    const int m = e.mode[s];
+                    const int nqv = (int)sq[m] + (int)e.delta[s];
+                    ok = ok && nqv >= 0 && nqv <= 255;
+                    key_add<NW>(t, m, e.delta[s]);
+                }
+                if (ok) {
+                    j = hash_find<NW>(a.vslots, a.vmask, a.vkeys, t);
+                    hit = j >= 0;
+                }
+            }
+            const uint64_t bal = __ballot(hit);
+            if (hit) {
+                const int pos = nh + __popcll(bal & ((1ull << lane) - 1ull));
+                const GroupEnt ge = a.groups[e.meta & 0x03ffffffu];
+                // PT2.cpp:70-118: AnharmPot(a, i) starts at the external state
+                hidx[pos] = (int32_t)j;
+                hval[pos] = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, true);
+            }
+            nh += __popcll(bal);
+            if (nh > kHitBuf - kWave || p0 + kWave >= a.P) {
+                __syncthreads();
+                for (int h = 0; h < nh; h++) {
+                    const int64_t i = hidx[h];
+                    const double hv = hval[h];
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int c = lane + 64 * r;
+                        if (c < a.nopt) {
+                            if (MIXED)
+                                num32[r] = fmaf((float)hv, a.C32[i * a.ldc + c], num32[r]);
+                            else
+                                num[r] = fma(hv, a.C[i * a.ldc + c], num[r]);
+                        }
+                    }
+                }
+                __syncthreads();
+                nh = 0;
+            }
+        }
+        // E_a (PT2.cpp:119-144): harmonic sum, then each zero-pattern FC
+        // term added one at a time in list order
+        double ea = 0.;
+        for (int f0 = 0; f0 < a.ndfc || f0 == 0; f0 += kWave) {
+            const int f = f0 + lane;
+            double x = 0.0;
+            if (f < a.ndfc) {
+                const ElemFC fc = a.elem[a.dfc[f]];
+                x = fc.fc;
+                for (int t = 0; t < fc.d; t++) {
+                    const int q = sq[fc.mode[t]];
+                    x = x * a.ladder[((size_t)fc.pow[t] * a.nq + q) * kLadderW + fc.pow[t]];
+                }
+            }
+            sdiag[lane] = x;
+            __syncthreads();
+            if (lane == 0) {
+                if (f0 == 0)
+                    for (int m = 0; m < a.M; m++) {
+                        double ej = 0.5;
+                        ej = ej + (double)sq[m];
+                        ej = ej * a.freq[m];
+                        ea = ea + ej;
+                    }
+                const int lim = min(kWave, a.ndfc - f0);
+                for (int t = 0; t < lim; t++) ea = ea + sdiag[t];
+            }
+            __syncthreads();
+            if (a.ndfc == 0) break;
+        }
+        ea = __shfl(ea, 0);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int c = lane + 64 * r;
+            if (c < a.nopt) {
+                const double nm = MIXED ? (double)num32[r] : num[r];
+                const double term = (nm * nm) / (a.E[c] - ea);
+                const double y = term - kc[r];
+                const double t = ks[r] + y;
+                kc[r] = (t - ks[r]) - y;
+                ks[r] = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int c = lane + 64 * r;
+        if (c < a.nopt) {
+            a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 0] = ks[r];
+            a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 1] = kc[r];
+        }
+    }
+}
+
+// Neumaier sum of the per-wave (sum, -compensation) pairs in wave order
+__global__ void k_pt2_reduce(const double* __restrict__ part, int nw, int nopt, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nopt) return;
+    double s = 0.0, cmp = 0.0;
+    for (int w = 0; w < nw; w++) {
+        for (int h = 0; h < 2; h++) {
+            const double v = h == 0 ? part[((int64_t)w * nopt + c) * 2] : -part[((int64_t)w * nopt + c) * 2 + 1];
+            const double t = s + v;
+            if (fabs(s) >= fabs(v))
+                cmp += (s - t) + v;
+            else
+                cmp += (v - t) + s;
+            s = t;
+        }
+    }
+    out[c] = s + cmp;
+}
+
+}  // namespace
+
+// ==========================================================================
+// DevBuf
+// ==========================================================================
+DevBuf::~DevBuf() { release(); }
+void DevBuf::release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+void* DevBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return p;
+    release();
+    size_t want = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        p = nullptr;
+        throw std::runtime_error("device allocation of " + std::to_string(want) + " bytes failed: " +
+                                 hipGetErrorString(e));
+    }
+    cap = want;
+    return p;
+}
+
+// ==========================================================================
+// Engine
+// ==========================================================================
+Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), device_(device) {
+    if (form != kFP64 && form != kMixed) throw std::runtime_error("unknown precision form");
+    if (P_.M < 1 || P_.M > 64) throw std::runtime_error("number of modes must be 1..64");
+    int ndev = 0;
+    HIPCHK(hipGetDeviceCount(&ndev));
+    if (ndev < 1) throw std::runtime_error("no HIP device available");
+    if (device < 0 || device >= ndev) throw std::runtime_error("device index out of range");
+    HIPCHK(hipSetDevice(device));
+    KW_ = (P_.M + 15) / 16;
+    HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    BLASCHK(rocblas_create_handle(&blas_));
+    BLASCHK(rocblas_set_stream(blas_, stream_));
+    build_tables(P_, T_);
+    upload_tables();
+}
+
+Engine::~Engine() {
+    if (blas_) rocblas_destroy_handle(blas_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+template <class T>
+static void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
+    b.ensure(std::max<size_t>(1, v.size()) * sizeof(T));
+    if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+void Engine::upload_tables() {
+    std::vector<double> L;
+    build_ladder(255, L);
+    upload(d_ladder_, L, stream_);
+    upload(d_elem_, T_.elem, stream_);
+    upload(d_deltas_, T_.deltas, stream_);
+    upload(d_groups_, T_.groups, stream_);
+    upload(d_fclist_, T_.fclist, stream_);
+    upload(d_diag_, T_.diag_fcs, stream_);
+    upload(d_freq_, P_.freq, stream_);
+    upload(d_hbfc_, T_.hb_fc, stream_);
+    upload(d_hbvaroff_, T_.hb_varoff, stream_);
+    upload(d_hbvars_, T_.hb_vars, stream_);
+    P_deltas_ = (int)T_.deltas.size();
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+void Engine::set_basis(const int32_t* q, int64_t n) {
+    if (n < 1) throw std::runtime_error("empty basis");
+    if (n >= (int64_t)INT32_MAX) throw std::runtime_error("basis larger than 2^31-1 states");
+    const int M = P_.M;
+    const int K = KW_ * 16;
+    std::vector<uint8_t> packed((size_t)n * K, 0);
+    for (int64_t i = 0; i < n; i++)
+        for (int m = 0; m < M; m++) {
+            int v = q[i * M + m];
+            if (v < 0 || v > 255) throw std::runtime_error("quanta must lie in 0..255");
+            packed[(size_t)i * K + m] = (uint8_t)v;
+        }
+    keys_.ensure(packed.size());
+    HIPCHK(hipMemcpyAsync(keys_.p, packed.data(), packed.size(), hipMemcpyHostToDevice, stream_));
+    n_ = n;
+    h_valid_ = false;
+    xrows_ = 0;
+    xcols_ = 0;
+    rebuild_hash(n);
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+void Engine::get_basis(int32_t* q) const {
+    const int M = P_.M, K = KW_ * 16;
+    std::vector<uint8_t> packed((size_t)n_ * K);
+    HIPCHK(hipMemcpyAsync(packed.data(), keys_.p, packed.size(), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int64_t i = 0; i < n_; i++)
+        for (int m = 0; m < M; m++) q[i * M + m] = packed[(size_t)i * K + m];
+}
+
+void Engine::rebuild_hash(int64_t n) {
+    uint64_t cap = next_pow2(std::max<uint64_t>(1024, (uint64_t)n * 2));
+    slots_.ensure(cap * sizeof(uint64_t));
+    hmask_ = cap - 1;
+    k_fill<uint64_t><<<grid_for(cap, 256), 256, 0, stream_>>>(slots_.as<uint64_t>(), (int64_t)cap, kEmpty);
+    const uint4* k = keys_.as<uint4>();
+    switch (KW_) {
+        case 1: k_hash_insert<4><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
+        case 2: k_hash_insert<8><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
+        case 3: k_hash_insert<12><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
+        default: k_hash_insert<16><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- H build
+template <int NW>
+void Engine::build_h_t(HStats& st) {
+    const int64_t n = n_;
+    HBuildArgs a{};
+    a.n = n;
+    a.M = P_.M;
+    a.P = P_deltas_;
+    a.nq = 256;
+    a.keys = keys_.as<uint4>();
+    a.slots = slots_.as<uint64_t>();
+    a.mask = hmask_;
+    a.deltas = d_deltas_.as<DeltaEnt>();
+    a.groups = d_groups_.as<GroupEnt>();
+    a.fclist = d_fclist_.as<uint32_t>();
+    a.elem = d_elem_.as<ElemFC>();
+    a.ladder = d_ladder_.as<double>();
+    a.form = form_;
+    rowcnt_.ensure((n + 1) * sizeof(int64_t));
+    rowptr_.ensure((n + 1) * sizeof(int64_t));
+    a.rowcnt = rowcnt_.as<int64_t>();
+    if (a.P > 0) {
+        k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
+        HIPCHK(hipGetLastError());
+    } else {
+        k_fill<int64_t><<<grid_for(n, 256), 256, 0, stream_>>>(a.rowcnt, n, 0);
+    }
+    k_fill<int64_t><<<1, 1, 0, stream_>>>(a.rowcnt + n, 1, 0);
+    size_t tmp = 0;
+    HIPCHK(rocprim::exclusive_scan(nullptr, tmp, a.rowcnt, rowptr_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                   rocprim::plus<int64_t>(), stream_));
+    scan_tmp_.ensure(tmp);
+    HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, a.rowcnt, rowptr_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                   rocprim::plus<int64_t>(), stream_));
+    int64_t nnz = 0;
+    HIPCHK(hipMemcpyAsync(&nnz, rowptr_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    nnz_ = nnz;
+    col_.ensure(std::max<int64_t>(1, nnz) * sizeof(int32_t));
+    if (form_ == kFP64) {
+        val_.ensure(std::max<int64_t>(1, nnz) * sizeof(double));
+    } else {
+        hi_.ensure(std::max<int64_t>(1, nnz) * sizeof(float));
+        lo_.ensure(std::max<int64_t>(1, nnz) * sizeof(float));
+    }
+    a.rowptr = rowptr_.as<int64_t>();
+    a.col = col_.as<int32_t>();
+    a.val = val_.as<double>();
+    a.hi = hi_.as<float>();
+    a.lo = lo_.as<float>();
+    if (a.P > 0 && nnz > 0) {
+        k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
+        HIPCHK(hipGetLastError());
+    }
+    D_.ensure(n * sizeof(double));
+    k_diag<NW><<<grid_for(n, 128), 128, 0, stream_>>>(n, P_.M, a.keys, d_freq_.as<double>(), d_diag_.as<uint32_t>(),
+                                                      (int)T_.diag_fcs.size(), a.elem, a.ladder, 256,
+                                                      D_.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    st.n = n;
+    st.nnz_offdiag = nnz;
+}
+
+HStats Engine::build_h() {
+    HIPCHK(hipSetDevice(device_));
+    auto t0 = clk::now();
+    HStats st;
+    switch (KW_) {
+        case 1: build_h_t<4>(st); break;
+        case 2: build_h_t<8>(st); break;
+        case 3: build_h_t<12>(st); break;
+        default: build_h_t<16>(st); break;
+    }
+    h_valid_ = true;
+    st.build_ms = ms_since(t0);
+    return st;
+}
+
+void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std::vector<double>& val,
+                   std::vector<double>& diag) {
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    rowptr.resize(n_ + 1);
+    col.resize(nnz_);
+    val.resize(nnz_);
+    diag.resize(n_);
+    HIPCHK(hipMemcpyAsync(rowptr.data(), rowptr_.p, (n_ + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    if (nnz_) HIPCHK(hipMemcpyAsync(col.data(), col_.p, nnz_ * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(diag.data(), D_.p, n_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (nnz_) {
+        if (form_ == kFP64) {
+            HIPCHK(hipMemcpyAsync(val.data(), val_.p, nnz_ * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        } else {
+            std::vector<float> h(nnz_), l(nnz_);
+            HIPCHK(hipMemcpyAsync(h.data(), hi_.p, nnz_ * sizeof(float), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipMemcpyAsync(l.data(), lo_.p, nnz_ * sizeof(float), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            for (int64_t k = 0; k < nnz_; k++) val[k] = (double)h[k] + (double)l[k];
+        }
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+int64_t Engine::structural_nnz() { return -1; }
+
+// ---------------------------------------------------------------- SpMM
+void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low) {
+    const int64_t n = n_;
+    const unsigned grid = grid_for(n, 4);
+    const int64_t* rp = rowptr_.as<int64_t>();
+    const int32_t* cl = col_.as<int32_t>();
+    const double* D = D_.as<double>();
+    auto launch64 = [&](auto tag) {
+        constexpr int B = decltype(tag)::value;
+        k_spmm64<B><<<grid, 256, 0, stream_>>>(n, rp, cl, val_.as<double>(), D, X, ldx, Y, ldy, b);
+    };
+    auto launch32 = [&](auto tag, const float* X32, int ld32) {
+        constexpr int B = decltype(tag)::value;
+        k_spmm32<B><<<grid, 256, 0, stream_>>>(n, rp, cl, hi_.as<float>(), D, X32, ld32, X, ldx, Y, ldy, b);
+    };
+    auto launchhl = [&](auto tag) {
+        constexpr int B = decltype(tag)::value;
+        k_spmm_hilo<B><<<grid, 256, 0, stream_>>>(n, rp, cl, hi_.as<float>(), lo_.as<float>(), D, X, ldx, Y, ldy, b);
+    };
+    auto dispatch = [&](auto fn) {
+        if (b <= 1) fn(std::integral_constant<int, 1>());
+        else if (b <= 2) fn(std::integral_constant<int, 2>());
+        else if (b <= 4) fn(std::integral_constant<int, 4>());
+        else if (b <= 8) fn(std::integral_constant<int, 8>());
+        else if (b <= 16) fn(std::integral_constant<int, 16>());
+        else if (b <= 32) fn(std::integral_constant<int, 32>());
+        else fn(std::integral_constant<int, 64>());
+    };
+    if (form_ == kFP64) {
+        dispatch(launch64);
+    } else if (low) {
+        X32_.ensure((size_t)n * b * sizeof(float));
+        k_to_f32<<<grid_for(n * b, 256), 256, 0, stream_>>>(X, ldx, X32_.as<float>(), b, n, b);
+        dispatch([&](auto tag) { launch32(tag, X32_.as<float>(), b); });
+    } else {
+        dispatch(launchhl);
+    }
+    HIPCHK(hipGetLastError());
+}
+
+double Engine::time_spmm(int b, int reps, bool fp32, double* bytes_per_call) {
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    const int64_t n = n_;
+    DevBuf xb, yb;
+    xb.ensure((size_t)n * b * sizeof(double));
+    yb.ensure((size_t)n * b * sizeof(double));
+    k_fill<double><<<grid_for(n * b, 256), 256, 0, stream_>>>(xb.as<double>(), n * b, 1.0 / std::sqrt((double)n));
+    bool low = fp32 && form_ == kMixed;
+    spmm(xb.as<double>(), b, yb.as<double>(), b, b, low);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, stream_));
+    for (int r = 0; r < reps; r++) spmm(xb.as<double>(), b, yb.as<double>(), b, b, low);
+    HIPCHK(hipEventRecord(e1, stream_));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    const double sv = low ? 4.0 : (form_ == kMixed ? 8.0 : 8.0);
+    const double sx = low ? 4.0 : 8.0;
+    if (bytes_per_call) *bytes_per_call = (double)nnz_ * (sv + 4) + 8.0 * (n + 1) + 2.0 * n * b * sx + 8.0 * n;
+    return ms / reps;
+}
+
+// ---------------------------------------------------------------- eigensolver
+void Engine::dense_solve(int nev, std::vector<double>& E, EigStats& st) {
+    const int64_t n = n_;
+    DevBuf A, w, e;
+    A.ensure((size_t)n * n * sizeof(double));
+    w.ensure(n * sizeof(double));
+    e.ensure(n * sizeof(double));
+    info_.ensure(sizeof(int));
+    k_fill<double><<<grid_for(n * n, 256), 256, 0, stream_>>>(A.as<double>(), n * n, 0.0);
+    k_densify<<<(unsigned)n, 64, 0, stream_>>>(n, rowptr_.as<int64_t>(), col_.as<int32_t>(),
+                                               form_ == kFP64 ? val_.as<double>() : nullptr, hi_.as<float>(),
+                                               lo_.as<float>(), D_.as<double>(), A.as<double>());
+    HIPCHK(hipGetLastError());
+    BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, A.as<double>(),
+                             (rocblas_int)n, w.as<double>(), e.as<double>(), info_.as<rocblas_int>()));
+    int info = 0;
+    HIPCHK(hipMemcpyAsync(&info, info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    std::vector<double> wh(n);
+    HIPCHK(hipMemcpyAsync(wh.data(), w.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    if (info != 0) throw std::runtime_error("Error: Eigenvalues did not converge.");
+    E.assign(wh.begin(), wh.begin() + nev);
+    X_.ensure((size_t)n * nev * sizeof(double));
+    k_colmajor_to_rows<<<grid_for(n * nev, 256), 256, 0, stream_>>>(A.as<double>(), n, n, nev, X_.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    xrows_ = n;
+    xcols_ = nev;
+    st.dense = true;
+}
+
+EigStats Engine::dense_all(std::vector<double>& E) {
+    HIPCHK(hipSetDevice(device_));
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    auto t0 = clk::now();
+    EigStats st;
+    dense_solve((int)n_, E, st);
+    st.ms = ms_since(t0);
+    return st;
+}
+
+EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E) {
+    HIPCHK(hipSetDevice(device_));
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    auto t0 = clk::now();
+    EigStats st;
+    const int64_t n = n_;
+    if (nev < 1) throw std::runtime_error("nev must be positive");
+    if (nev > n) nev = (int)n;
+    if (nev > 256) throw std::runtime_error("at most 256 eigenpairs are supported");
+    const int b = (int)std::min<int64_t>(n, std::min(256, nev + std::max(4, nev / 5)));
+    if (n <= std::max<int64_t>(4 * b, 1024)) {
+        dense_solve(nev, E, st);
+        st.ms = ms_since(t0);
+        return st;
+    }
+    const int mmax = (int)std::min<int64_t>(n, 3 * b);
+    const int ld = mmax;
+    V_.ensure((size_t)n * ld * sizeof(double));
+    W_.ensure((size_t)n * ld * sizeof(double));
+    HX_.ensure((size_t)n * b * sizeof(double));
+    Xw_.ensure((size_t)n * b * sizeof(double));
+    T_b_.ensure((size_t)n * std::max(b, mmax) * sizeof(double));
+    G_.ensure((size_t)mmax * mmax * sizeof(double));
+    S_.ensure((size_t)mmax * std::max(b, mmax) * sizeof(double));
+    theta_.ensure((size_t)mmax * sizeof(double));
+    work_.ensure((size_t)mmax * sizeof(double) * 2);
+    info_.ensure(sizeof(int) * 4);
+    double* V = V_.as<double>();
+    double* W = W_.as<double>();
+    double* HX = HX_.as<double>();
+    double* X = Xw_.as<double>();
+    double* Tt = T_b_.as<double>();
+    double* G = G_.as<double>();
+    double* S = S_.as<double>();
+    double* th = theta_.as<double>();
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+
+    // host copy of the diagonal for start vectors
+    std::vector<double> Dh(n);
+    HIPCHK(hipMemcpyAsync(Dh.data(), D_.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+
+    // SVQB orthonormalisation of V[:, m0:m0+k] after Gram-Schmidt against V[:, :m0]
+    auto ortho = [&](int m0, int k) -> int {
+        for (int pass = 0; pass < 2; pass++) {
+            if (m0 > 0) {
+                // S (m0 x k) = Vc[0:m0] * Tc^T ; Tc -= S^T Vc
+                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, m0, k, (int)n, &one,
+                                      V, ld, V + m0, ld, &zero, S, m0));
+                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, k, (int)n, m0,
+                                      &mone, S, m0, V, ld, &one, V + m0, ld));
+            }
+            // Gram matrix of the block
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, k, k, (int)n, &one,
+                                  V + m0, ld, V + m0, ld, &zero, G, k));
+            std::vector<double> g((size_t)k * k);
+            HIPCHK(hipMemcpyAsync(g.data(), G, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            std::vector<double> dsc(k);
+            for (int i = 0; i < k; i++) dsc[i] = g[(size_t)i * k + i] > 0 ? 1.0 / std::sqrt(g[(size_t)i * k + i]) : 0.0;
+            for (int i = 0; i < k; i++)
+                for (int j = 0; j < k; j++) g[(size_t)j * k + i] *= dsc[i] * dsc[j];
+            HIPCHK(hipMemcpyAsync(G, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+            BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, k, G, k, th,
+                                     work_.as<double>(), info_.as<rocblas_int>()));
+            std::vector<double> lam(k), U((size_t)k * k);
+            HIPCHK(hipMemcpyAsync(lam.data(), th, k * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipMemcpyAsync(U.data(), G, U.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            const double lmax = lam[k - 1];
+            std::vector<int> keep;
+            for (int c = k - 1; c >= 0; c--)
+                if (lam[c] > 1e-10 * std::max(1.0, lmax)) keep.push_back(c);
+            const int kk = (int)keep.size();
+            if (kk == 0) return 0;
+            // Z (k x kk) col-major = diag(dsc) U[:, keep] / sqrt(lam)
+            std::vector<double> Z((size_t)k * kk);
+            for (int c = 0; c < kk; c++) {
+                const int src = keep[c];
+                const double s = 1.0 / std::sqrt(lam[src]);
+                for (int i = 0; i < k; i++) Z[(size_t)c * k + i] = dsc[i] * U[(size_t)src * k + i] * s;
+            }
+            HIPCHK(hipMemcpyAsync(S, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+            // Tt (kk x n, ld kk) = Z^T Tc ; then copy back into V columns m0..
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, kk, (int)n, k, &one, S, k,
+                                  V + m0, ld, &zero, Tt, kk));
+            k_copy2d<<<grid_for(n * kk, 256), 256, 0, stream_>>>(Tt, kk, V + m0, ld, n, kk);
+            HIPCHK(hipGetLastError());
+            k = kk;
+        }
+        return k;
+    };
+
+    // ---- start block
+    int k0 = 0;
+    std::vector<double> start((size_t)n * b, 0.0);
+    std::vector<char> used(n, 0);
+    if (xcols_ > 0 && xrows_ > 0 && xrows_ <= n) {
+        k0 = std::min(xcols_, b);
+        std::vector<double> xo((size_t)xrows_ * xcols_);
+        HIPCHK(hipMemcpyAsync(xo.data(), X_.p, xo.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        for (int64_t i = 0; i < xrows_; i++)
+            for (int c = 0; c < k0; c++) start[i * b + c] = xo[i * xcols_ + c];
+    }
+    {
+        // unit vectors on the lowest diagonal entries, new states first
+        std::vector<int64_t> idx(n);
+        std::iota(idx.begin(), idx.end(), 0);
+        const int64_t first = (k0 > 0) ? xrows_ : 0;
+        auto cmp = [&](int64_t a, int64_t c) { return Dh[a] < Dh[c] || (Dh[a] == Dh[c] && a < c); };
+        std::vector<int64_t> cand(idx.begin() + first, idx.end());
+        const int need = b - k0;
+        if ((int64_t)cand.size() < need) cand = idx;
+        std::partial_sort(cand.begin(), cand.begin() + std::min<int64_t>(need, cand.size()), cand.end(), cmp);
+        for (int c = 0; c < need && c < (int)cand.size(); c++) start[cand[c] * b + k0 + c] = 1.0;
+    }
+    HIPCHK(hipMemcpyAsync(Tt, start.data(), start.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    k_copy2d<<<grid_for(n * b, 256), 256, 0, stream_>>>(Tt, b, V, ld, n, b);
+    int m = ortho(0, b);
+    if (m == 0) throw std::runtime_error("Davidson: empty start block");
+    const bool mixed = form_ == kMixed;
+    bool low = mixed;
+    spmm(V, ld, W, ld, m, low);
+    st.spmm_calls++;
+    st.spmm_vectors += m;
+    const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+    const int rpb = 256;
+    const int nblk = (int)((n + rpb - 1) / rpb);
+    part_.ensure((size_t)nblk * b * sizeof(double));
+    std::vector<double> thh(mmax), rn(b);
+    DevBuf Jd;
+    Jd.ensure(b * sizeof(int));
+    double prev_max = 1e300;
+    int stall = 0;
+    for (int it = 0; it < maxit; it++) {
+        st.iterations = it + 1;
+        // Rayleigh-Ritz
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, m, m, (int)n, &one, V, ld, W,
+                              ld, &zero, G, m));
+        dim3 sg(grid_for(m, 64), m);
+        k_symmetrize<<<sg, 64, 0, stream_>>>(G, m, m);
+        BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, m, G, m, th, work_.as<double>(),
+                                 info_.as<rocblas_int>()));
+        const int nb = std::min(b, m);
+        // X = V Y[:, :nb], HX = W Y[:, :nb]  (row-major n x nb, ld b)
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, nb, (int)n, m, &one, G, m, V,
+                              ld, &zero, X, b));
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, nb, (int)n, m, &one, G, m, W,
+                              ld, &zero, HX, b));
+        k_resid_partial<<<nblk, 256, 0, stream_>>>(n, nb, HX, X, b, th, rpb, part_.as<double>());
+        k_reduce_cols<<<grid_for(nb, 64), 64, 0, stream_>>>(part_.as<double>(), nblk, nb, work_.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(thh.data(), th, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(rn.data(), work_.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        std::vector<int> J;
+        double maxrel = 0;
+        bool all = true;
+        for (int j = 0; j < nb; j++) {
+            const double r = std::sqrt(rn[j]);
+            const double lim = tol * std::max(eps23, std::abs(thh[j]));
+            if (j < nev) maxrel = std::max(maxrel, r / std::max(eps23, std::abs(thh[j])));
+            if (r >= lim) {
+                J.push_back(j);
+                if (j < nev) all = false;
+            }
+        }
+        st.max_resid = maxrel;
+        if (all && !low) break;
+        if (low) {
+            // FP32 phase ends once the FP32 SpMM no longer improves the
+            // residuals; the rest runs with the FP64 (hi+lo) product.
+            const bool floor_hit = maxrel < 2e-7 || (maxrel > 0.5 * prev_max && ++stall >= 2) || all;
+            if (floor_hit) {
+                low = false;
+                k_copy2d<<<grid_for(n * nb, 256), 256, 0, stream_>>>(X, b, V, ld, n, nb);
+                spmm(V, ld, W, ld, nb, false);
+                st.spmm_calls++;
+                st.spmm_vectors += nb;
+                st.fp64_spmm_vectors += nb;
+                m = nb;
+                prev_max = 1e300;
+                stall = 0;
+                continue;
+            }
+        }
+        if (!low) {
+            if (maxrel > 0.9 * prev_max) stall++;
+            else stall = 0;
+        }
+        prev_max = std::min(prev_max, maxrel);
+        if (J.empty()) J.push_back(0);
+        const int k = (int)J.size();
+        if (m + k > mmax) {
+            k_copy2d<<<grid_for(n * nb, 256), 256, 0, stream_>>>(X, b, V, ld, n, nb);
+            k_copy2d<<<grid_for(n * nb, 256), 256, 0, stream_>>>(HX, b, W, ld, n, nb);
+            m = nb;
+        }
+        HIPCHK(hipMemcpyAsync(Jd.p, J.data(), k * sizeof(int), hipMemcpyHostToDevice, stream_));
+        k_precond<<<grid_for(n * k, 256), 256, 0, stream_>>>(n, k, Jd.as<int>(), HX, X, b, th, D_.as<double>(), V + m,
+                                                           ld);
+        HIPCHK(hipGetLastError());
+        const int kept = ortho(m, k);
+        if (kept == 0) {
+            if (all) break;
+            throw std::runtime_error("Error: Eigenvalues did not converge.");
+        }
+        spmm(V + m, ld, W + m, ld, kept, low);
+        st.spmm_calls++;
+        st.spmm_vectors += kept;
+        if (!low) st.fp64_spmm_vectors += kept;
+        m += kept;
+        if (it + 1 == maxit) throw std::runtime_error("Error: Eigenvalues did not converge.");
+    }
+    E.assign(thh.begin(), thh.begin() + nev);
+    X_.ensure((size_t)n * nev * sizeof(double));
+    k_copy2d<<<grid_for(n * nev, 256), 256, 0, stream_>>>(X, b, X_.as<double>(), nev, n, nev);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    xrows_ = n;
+    xcols_ = nev;
+    st.ms = ms_since(t0);
+    return st;
+}
+
+void Engine::get_vectors(double* C, int ncol) const {
+    if (ncol > xcols_) throw std::runtime_error("requested more eigenvector columns than resident");
+    std::vector<double> xo((size_t)xrows_ * xcols_);
+    HIPCHK(hipMemcpyAsync(xo.data(), X_.p, xo.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int64_t i = 0; i < xrows_; i++)
+        for (int c = 0; c < ncol; c++) C[i * ncol + c] = xo[i * xcols_ + c];
+}
+
+void Engine::set_vectors(const double* C, int ncol) {
+    X_.ensure((size_t)n_ * ncol * sizeof(double));
+    HIPCHK(hipMemcpyAsync(X_.p, C, (size_t)n_ * ncol * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    xrows_ = n_;
+    xcols_ = ncol;
+}
+
+// ---------------------------------------------------------------- heat bath
+template <int NW>
+void Engine::hb_select_t(double eps, HbStats& st) {
+    const int64_t n = n_;
+    const int nhb = (int)T_.hb_fc.size();
+    candcnt_.ensure((n + 1) * sizeof(int64_t));
+    candoff_.ensure((n + 1) * sizeof(int64_t));
+    k_hb_count<<<grid_for(n, 256), 256, 0, stream_>>>(n, cmax_.as<double>(), d_hbfc_.as<double>(), nhb,
+                                                      d_hbvaroff_.as<int64_t>(), eps, candcnt_.as<int64_t>());
+    k_fill<int64_t><<<1, 1, 0, stream_>>>(candcnt_.as<int64_t>() + n, 1, 0);
+    size_t tmp = 0;
+    HIPCHK(rocprim::exclusive_scan(nullptr, tmp, candcnt_.as<int64_t>(), candoff_.as<int64_t>(), (int64_t)0,
+                                   (size_t)(n + 1), rocprim::plus<int64_t>(), stream_));
+    scan_tmp_.ensure(tmp);
+    HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, candcnt_.as<int64_t>(), candoff_.as<int64_t>(), (int64_t)0,
+                                   (size_t)(n + 1), rocprim::plus<int64_t>(), stream_));
+    int64_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, candoff_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    st.candidates = total;
+    nnew_ = 0;
+    if (total == 0) return;
+    uint64_t ucap = (uint64_t)std::min<int64_t>(total, std::max<int64_t>(1 << 20, 32 * n));
+    counters_.ensure(4 * sizeof(uint32_t));
+    for (int attempt = 0; attempt < 8; attempt++) {
+        uint64_t cap = next_pow2(std::max<uint64_t>(2048, 2 * ucap));
+        if (cap > (1ull << 32)) throw std::runtime_error("heat-bath candidate set too large");
+        const uint32_t pool_cap = (uint32_t)std::min<uint64_t>(cap / 2, 0xffffffffull - 1);
+        nslots_.ensure(cap * sizeof(uint64_t));
+        pool_keys_.ensure((size_t)pool_cap * KW_ * 16);
+        pool_rank_.ensure((size_t)pool_cap * sizeof(uint64_t));
+        k_fill<uint64_t><<<grid_for(cap, 256), 256, 0, stream_>>>(nslots_.as<uint64_t>(), (int64_t)cap, kEmpty);
+        k_fill<uint32_t><<<1, 4, 0, stream_>>>(counters_.as<uint32_t>(), 4, 0u);
+        HbGenArgs a{};
+        a.n = n;
+        a.total = total;
+        a.candoff = candoff_.as<int64_t>();
+        a.keys = keys_.as<uint4>();
+        a.bslots = slots_.as<uint64_t>();
+        a.bmask = hmask_;
+        a.vars = d_hbvars_.as<HbVar>();
+        a.nslots = nslots_.as<uint64_t>();
+        a.nmask = cap - 1;
+        a.pool_keys = pool_keys_.as<uint4>();
+        a.pool_rank = pool_rank_.as<uint64_t>();
+        a.counters = counters_.as<uint32_t>();
+        a.pool_cap = pool_cap;
+        k_hb_gen<NW><<<grid_for(total, 256), 256, 0, stream_>>>(a);
+        HIPCHK(hipGetLastError());
+        uint32_t cnt[4];
+        HIPCHK(hipMemcpyAsync(cnt, counters_.p, sizeof(cnt), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        if (cnt[1] & 1u) throw std::runtime_error("Error: a heat-bath state exceeds 255 quanta in one mode.");
+        if (cnt[1] & 2u) {
+            ucap *= 4;
+            continue;
+        }
+        // collect, sort by first-discovery rank, gather keys
+        col_rank_.ensure((size_t)pool_cap * sizeof(uint64_t));
+        col_idx_.ensure((size_t)pool_cap * sizeof(uint32_t));
+        k_fill<uint32_t><<<1, 1, 0, stream_>>>(counters_.as<uint32_t>() + 2, 1, 0u);
+        k_collect<<<grid_for(cap, 256), 256, 0, stream_>>>(nslots_.as<uint64_t>(), cap, pool_rank_.as<uint64_t>(),
+                                                          counters_.as<uint32_t>() + 2, col_rank_.as<uint64_t>(),
+                                                          col_idx_.as<uint32_t>());
+        uint32_t u = 0;
+        HIPCHK(hipMemcpyAsync(&u, counters_.as<uint32_t>() + 2, sizeof(u), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        srt_rank_.ensure((size_t)std::max<uint32_t>(u, 1) * sizeof(uint64_t));
+        srt_idx_.ensure((size_t)std::max<uint32_t>(u, 1) * sizeof(uint32_t));
+        unsigned endbit = 1;
+        while (endbit < 64 && ((uint64_t)total >> endbit) != 0) endbit++;
+        size_t stmp = 0;
+        HIPCHK(rocprim::radix_sort_pairs(nullptr, stmp, col_rank_.as<uint64_t>(), srt_rank_.as<uint64_t>(),
+                                         col_idx_.as<uint32_t>(), srt_idx_.as<uint32_t>(), (size_t)u, 0, endbit,
+                                         stream_));
+        scan_tmp_.ensure(stmp);
+        HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, stmp, col_rank_.as<uint64_t>(), srt_rank_.as<uint64_t>(),
+                                         col_idx_.as<uint32_t>(), srt_idx_.as<uint32_t>(), (size_t)u, 0, endbit,
+                                         stream_));
+        newkeys_.ensure((size_t)std::max<uint32_t>(u, 1) * KW_ * 16);
+        k_gather_keys<<<grid_for((int64_t)u * KW_, 256), 256, 0, stream_>>>(
+            pool_keys_.as<uint4>(), srt_idx_.as<uint32_t>(), u, KW_, newkeys_.as<uint4>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream_));
+        nnew_ = u;
+        st.unique = u;
+        return;
+    }
+    throw std::runtime_error("heat-bath selection: candidate pool kept overflowing");
+}
+
+HbStats Engine::hb_select(const double* cmax_host, int nopt, double eps) {
+    HIPCHK(hipSetDevice(device_));
+    auto t0 = clk::now();
+    HbStats st;
+    const int64_t n = n_;
+    cmax_.ensure(n * sizeof(double));
+    if (cmax_host) {
+        HIPCHK(hipMemcpyAsync(cmax_.p, cmax_host, n * sizeof(double), hipMemcpyHostToDevice, stream_));
+    } else {
+        if (xrows_ != n || xcols_ < 1) throw std::runtime_error("no resident eigenvectors for this basis");
+        if (nopt < 1 || nopt > xcols_) nopt = std::min(xcols_, std::max(1, nopt));
+        k_cmax<<<grid_for(n, 256), 256, 0, stream_>>>(n, X_.as<double>(), xcols_, nopt, cmax_.as<double>());
+        HIPCHK(hipGetLastError());
+    }
+    switch (KW_) {
+        case 1: hb_select_t<4>(eps, st); break;
+        case 2: hb_select_t<8>(eps, st); break;
+        case 3: hb_select_t<12>(eps, st); break;
+        default: hb_select_t<16>(eps, st); break;
+    }
+    st.ms = ms_since(t0);
+    return st;
+}
+
+void Engine::get_new_states(int32_t* q) const {
+    const int M = P_.M, K = KW_ * 16;
+    std::vector<uint8_t> packed((size_t)nnew_ * K);
+    if (nnew_)
+        HIPCHK(hipMemcpyAsync(packed.data(), newkeys_.p, packed.size(), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int64_t i = 0; i < nnew_; i++)
+        for (int m = 0; m < M; m++) q[i * M + m] = packed[(size_t)i * K + m];
+}
+
+void Engine::append_new_states() {
+    HIPCHK(hipSetDevice(device_));
+    if (nnew_ == 0) return;
+    const int64_t n1 = n_ + nnew_;
+    if (n1 >= (int64_t)INT32_MAX) throw std::runtime_error("basis larger than 2^31-1 states");
+    const size_t K = (size_t)KW_ * 16;
+    if (keys_.cap < n1 * K) {
+        DevBuf nk;
+        nk.ensure(std::max<size_t>(n1 * K, keys_.cap * 3 / 2));
+        HIPCHK(hipMemcpyAsync(nk.p, keys_.p, n_ * K, hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        std::swap(nk.p, keys_.p);
+        std::swap(nk.cap, keys_.cap);
+    }
+    HIPCHK(hipMemcpyAsync(keys_.as<uint8_t>() + n_ * K, newkeys_.p, nnew_ * K, hipMemcpyDeviceToDevice, stream_));
+    const int64_t n0 = n_;
+    n_ = n1;
+    nnew_ = 0;
+    h_valid_ = false;
+    if ((uint64_t)n1 * 2 > hmask_ + 1) {
+        rebuild_hash(n1);
+    } else {
+        const uint4* k = keys_.as<uint4>();
+        switch (KW_) {
+            case 1: k_hash_insert<4><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
+            case 2: k_hash_insert<8><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
+            case 3: k_hash_insert<12><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
+            default: k_hash_insert<16><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
+        }
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------------- PT2
+template <int NW>
+void Engine::pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::vector<double>& dE) {
+    const int64_t ns2 = nnew_;
+    dE.assign(nopt, 0.0);
+    st.ns2 = ns2;
+    if (ns2 == 0) return;
+    auto t0 = clk::now();
+    DevBuf Ed, out;
+    Ed.ensure(nopt * sizeof(double));
+    out.ensure(nopt * sizeof(double));
+    HIPCHK(hipMemcpyAsync(Ed.p, E.data(), nopt * sizeof(double), hipMemcpyHostToDevice, stream_));
+    const bool mixed = form_ == kMixed;
+    if (mixed) {
+        C32_.ensure((size_t)n_ * nopt * sizeof(float));
+        k_to_f32<<<grid_for(n_ * nopt, 256), 256, 0, stream_>>>(X_.as<double>(), xcols_, C32_.as<float>(), nopt, n_,
+                                                               nopt);
+    }
+    int nwaves = (int)std::min<int64_t>(ns2, 256 * 32);
+    pt2_part_.ensure((size_t)nwaves * nopt * 2 * sizeof(double));
+    Pt2Args a{};
+    a.ns2 = ns2;
+    a.M = P_.M;
+    a.P = P_deltas_;
+    a.nq = 256;
+    a.nopt = nopt;
+    a.akeys = newkeys_.as<uint4>();
+    a.vkeys = keys_.as<uint4>();
+    a.vslots = slots_.as<uint64_t>();
+    a.vmask = hmask_;
+    a.deltas = d_deltas_.as<DeltaEnt>();
+    a.groups = d_groups_.as<GroupEnt>();
+    a.fclist = d_fclist_.as<uint32_t>();
+    a.elem = d_elem_.as<ElemFC>();
+    a.ladder = d_ladder_.as<double>();
+    a.dfc = d_diag_.as<uint32_t>();
+    a.ndfc = (int)T_.diag_fcs.size();
+    a.freq = d_freq_.as<double>();
+    a.C = X_.as<double>();
+    a.C32 = C32_.as<float>();
+    a.ldc = mixed ? nopt : xcols_;
+    a.E = Ed.as<double>();
+    a.part = pt2_part_.as<double>();
+    if (mixed)
+        k_pt2<NW, true><<<nwaves, 64, 0, stream_>>>(a);
+    else
+        k_pt2<NW, false><<<nwaves, 64, 0, stream_>>>(a);
+    HIPCHK(hipGetLastError());
+    k_pt2_reduce<<<grid_for(nopt, 64), 64, 0, stream_>>>(pt2_part_.as<double>(), nwaves, nopt, out.as<double>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dE.data(), out.p, nopt * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    st.sum_ms = ms_since(t0);
+}
+
+Pt2Stats Engine::pt2(const std::vector<double>& E, int nopt, double eps2, std::vector<double>& dE) {
+    HIPCHK(hipSetDevice(device_));
+    if (nopt < 1 || nopt > xcols_ || (int)E.size() < nopt) throw std::runtime_error("PT2: bad number of states");
+    if (nopt > 256) throw std::runtime_error("PT2: at most 256 states");
+    auto t0 = clk::now();
+    HbStats hs = hb_select(nullptr, nopt, eps2);
+    Pt2Stats st;
+    st.candidates = hs.candidates;
+    st.select_ms = ms_since(t0);
+    switch (KW_) {
+        case 1: pt2_t<4>(E, nopt, st, dE); break;
+        case 2: pt2_t<8>(E, nopt, st, dE); break;
+        case 3: pt2_t<12>(E, nopt, st, dE); break;
+        default: pt2_t<16>(E, nopt, st, dE); break;
+    }
+    return st;
+}
+
+}  // namespace vhci

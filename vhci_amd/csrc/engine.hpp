@@ -1,0 +1,158 @@
+// engine.hpp -- device-resident state of one heat-bath VCI solve.
+//
+// One Engine per request.  It owns, on its own HIP stream, the basis (packed
+// uint8 quanta keys plus an open-addressing hash), the Hamiltonian in CSR form,
+// the Ritz block and the heat-bath / PT2 work buffers, so they stay on the GPU
+// across heat-bath iterations.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "problem.hpp"
+
+namespace vhci {
+
+enum Form { kFP64 = 0, kMixed = 1 };
+
+// Growable device allocation; contents are not preserved on growth.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf();
+    void* ensure(size_t bytes);  // throws std::runtime_error on failure
+    void release();
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+struct HStats {
+    int64_t n = 0;
+    int64_t nnz_offdiag = 0;     // stored off-diagonal entries (both triangles)
+    int64_t nnz_struct = -1;     // reference's structural count, if computed
+    double build_ms = 0;
+};
+
+struct EigStats {
+    int iterations = 0;
+    int spmm_calls = 0;
+    int64_t spmm_vectors = 0;
+    int fp64_spmm_vectors = 0;
+    double max_resid = 0;
+    double ms = 0;
+    bool dense = false;
+};
+
+struct Pt2Stats {
+    int64_t ns2 = 0;
+    int64_t candidates = 0;
+    double select_ms = 0, sum_ms = 0;
+};
+
+struct HbStats {
+    int64_t candidates = 0;
+    int64_t unique = 0;
+    double ms = 0;
+};
+
+class Engine {
+  public:
+    Engine(const Problem& P, int form, int device);
+    ~Engine();
+
+    const Problem& problem() const { return P_; }
+    int form() const { return form_; }
+    int64_t n() const { return n_; }
+
+    // BasisSet (VCI_headers.h:149): replace the basis; quanta rows n x M.
+    void set_basis(const int32_t* q, int64_t n);
+    void get_basis(int32_t* q) const;
+
+    // MakeHamSparse (Ham.cpp:328-344) on the current basis.
+    HStats build_h();
+    // Structural nonzero count of the reference's triplet list (pairs that
+    // pass its prefilter, explicit zeros included), for the "% sparse" line.
+    int64_t structural_nnz();
+    // copy of the device CSR (off-diagonal part, stored order) and diagonal
+    void get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std::vector<double>& val,
+               std::vector<double>& diag);
+
+    // SparseDiagonalize (Ham.cpp:347-370): nev lowest eigenpairs.  Resident
+    // Ritz vectors seed the next call (warm start) after the basis grows.
+    EigStats eigsolve(int nev, double tol, int maxit, std::vector<double>& E);
+    // DenseDiagonalize (Ham.cpp:372-379): all n eigenpairs.
+    EigStats dense_all(std::vector<double>& E);
+    // resident eigenvectors, row-major n x ncol
+    int ritz_cols() const { return xcols_; }
+    void get_vectors(double* C, int ncol) const;
+    void set_vectors(const double* C, int ncol);  // upload (restart)
+
+    // AddStatesHB over every basis state (HB.cpp:221-223).  cmax may be null
+    // (then computed from the resident vectors with nopt leading columns).
+    HbStats hb_select(const double* cmax_host, int nopt, double eps);
+    int64_t new_count() const { return nnew_; }
+    void get_new_states(int32_t* q) const;
+    void append_new_states();
+
+    // DoPT2 (PT2.cpp:16-154) with the resident vectors and E[0..nopt).
+    Pt2Stats pt2(const std::vector<double>& E, int nopt, double eps2, std::vector<double>& dE);
+
+    hipStream_t stream() const { return stream_; }
+    double last_spmm_ms() const { return last_spmm_ms_; }
+    // time one SpMM of b vectors, averaged over reps (for the roofline figure)
+    double time_spmm(int b, int reps, bool fp32, double* bytes_per_call);
+
+  private:
+    void upload_tables();
+    void rebuild_hash(int64_t cap_hint);
+    void spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low);
+    void dense_solve(int nev, std::vector<double>& E, EigStats& st);
+    template <int NW>
+    void hb_select_t(double eps, HbStats& st);
+    template <int NW>
+    void build_h_t(HStats& st);
+    template <int NW>
+    void pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::vector<double>& dE);
+
+    Problem P_;
+    Tables T_;
+    int form_;
+    int device_;
+    int KW_;  // uint4 words per key
+    hipStream_t stream_ = nullptr;
+    rocblas_handle blas_ = nullptr;
+
+    // basis
+    int64_t n_ = 0;
+    DevBuf keys_, slots_;
+    uint64_t hmask_ = 0;
+    // tables
+    DevBuf d_ladder_, d_elem_, d_deltas_, d_groups_, d_fclist_, d_diag_, d_freq_;
+    DevBuf d_hbfc_, d_hbvaroff_, d_hbvars_;
+    int P_deltas_ = 0;
+    // Hamiltonian
+    bool h_valid_ = false;
+    int64_t nnz_ = 0;
+    DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_;
+    // Ritz block (row-major n x xcols_, ld = xcols_)
+    DevBuf X_;
+    int64_t xrows_ = 0;
+    int xcols_ = 0;
+    // eigensolver work
+    DevBuf V_, W_, HX_, Xw_, T_b_, G_, S_, theta_, work_, part_, X32_, info_;
+    // heat-bath work
+    DevBuf cmax_, candcnt_, candoff_, nslots_, pool_keys_, pool_rank_, counters_, col_rank_, col_idx_,
+        srt_rank_, srt_idx_, newkeys_, scan_tmp_;
+    int64_t nnew_ = 0;
+    // PT2 work
+    DevBuf s2keys_, pt2_part_, C32_;
+    double last_spmm_ms_ = 0;
+};
+
+}  // namespace vhci

@@ -1,0 +1,427 @@
+// problem.cpp -- input parsing, force-constant preprocessing, device tables
+// and the text output formats of the heat-bath VCI engine.
+#include "problem.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <fstream>
+#include <iomanip>
+#include <istream>
+#include <limits>
+#include <map>
+#include <ostream>
+#include <sstream>
+
+namespace vhci {
+
+namespace {
+
+// Fact (Core_functions.cpp:73-83): descending product in double
+double factorial(int n) {
+    double v = 1;
+    for (; n > 0; --n) v *= n;
+    return v;
+}
+
+// ScaleFC (Input_Reader.cpp:494-515): f *= 1/sqrt(2^p) / prod_j k_j!, in that
+// operation order so the scaled values are bit-identical to the reference.
+void scale(ForceConst& f) {
+    double s = (double)f.fcpow.size();
+    s = std::pow(2, s);
+    s = 1 / std::sqrt(s);
+    for (int k : f.mp) s /= factorial(k);
+    f.fc *= s;
+}
+
+}  // namespace
+
+std::string parse_input(std::istream& in, bool restart, Problem& P) {
+    std::string tok;
+    std::getline(in, tok);  // comment line
+    in >> tok >> P.eps1;
+    in >> tok >> P.neig;
+    in >> tok >> P.perturb;
+    in >> tok >> P.eps2;
+    if (!P.perturb) P.eps2 = 0;  // Input_Reader.cpp:199-201
+    in >> tok >> P.ntot;
+    int nm = 0;
+    in >> tok >> nm;
+    if (!in || nm < 1) return "Error: could not read the input header.";
+    if (nm > 64) return "Error: at most 64 modes are supported.";
+    P.M = nm;
+    for (int i = 0; i < nm; i++) {
+        int id = -1;
+        in >> id;
+        if (id != i) {
+            std::ostringstream os;
+            os << "Error: Expected mode " << i << " but read data for mode " << id;
+            return os.str();
+        }
+        double f = 0;
+        int c = 0;
+        in >> f >> c;
+        P.freq.push_back(f);
+        P.cap.push_back(c);
+    }
+    in >> tok >> P.nfc;
+    for (int i = 0; i < P.nfc; i++) {
+        ForceConst f;
+        int p = 0;
+        in >> p;
+        for (int j = 0; j < p; j++) {
+            int m = 0;
+            in >> m;
+            f.fcpow.push_back(m);
+        }
+        in >> f.fc;
+        if (!in) return "Error: could not read force constant " + std::to_string(i) + ".";
+        // ShortModes / ModePowers in first-seen order (Input_Reader.cpp:248-271)
+        for (int m : f.fcpow) {
+            auto it = std::find(f.sm.begin(), f.sm.end(), m);
+            if (it == f.sm.end()) {
+                f.sm.push_back(m);
+                f.mp.push_back(1);
+            } else {
+                f.mp[it - f.sm.begin()] += 1;
+            }
+            if (m < 0 || m >= nm) return "Error: force constant " + std::to_string(i) + " names a mode outside 0..M-1.";
+        }
+        P.fcs.push_back(f);
+    }
+    for (auto& f : P.fcs) scale(f);
+    for (int i = 0; i < (int)P.fcs.size(); i++) {
+        switch (P.fcs[i].order()) {  // Input_Reader.cpp:273-291
+            case 3: P.cubic.push_back(i); break;
+            case 4: P.quartic.push_back(i); break;
+            case 5: P.quintic.push_back(i); break;
+            case 6: P.sextic.push_back(i); break;
+            default: return "Error: Force constants are only supported up to 6th order!";
+        }
+    }
+    P.restart = restart;
+    const int M = P.M;
+    // initial basis (Input_Reader.cpp:296-364): mode 0 varies fastest, the
+    // last mode is the slowest key; total quanta <= Ntot, per-mode cap.
+    P.basis0.clear();
+    if (restart) {
+        P.basis0.assign(M, 0);
+    } else {
+        std::vector<int32_t> cur;
+        int lim0 = std::min(P.cap[0], P.ntot);
+        for (int q = 0; q <= lim0; q++) cur.push_back(q);
+        int width = 1;
+        for (int m = 1; m < M; m++) {
+            std::vector<int32_t> nxt;
+            int lim = std::min(P.cap[m], P.ntot);
+            int64_t ncur = (int64_t)cur.size() / width;
+            for (int q = 0; q <= lim; q++)
+                for (int64_t s = 0; s < ncur; s++) {
+                    int tot = q;
+                    for (int t = 0; t < width; t++) tot += cur[s * width + t];
+                    if (tot <= P.ntot) {
+                        nxt.insert(nxt.end(), cur.begin() + s * width, cur.begin() + (s + 1) * width);
+                        nxt.push_back(q);
+                    }
+                }
+            cur.swap(nxt);
+            width++;
+        }
+        P.basis0 = cur;
+    }
+    // AnharmHB = AnharmFC + effective single and double terms
+    // (Input_Reader.cpp:366-460), summed in list order, kept if |f| > 1e-8.
+    P.hb = P.fcs;
+    auto has = [](const std::vector<int>& v, int x) { return std::find(v.begin(), v.end(), x) != v.end(); };
+    for (int m = 0; m < M; m++) {
+        ForceConst t;
+        t.fcpow = {m};
+        t.sm = {m};
+        t.mp = {1};
+        for (int id : P.cubic) {
+            const ForceConst& c = P.fcs[id];
+            if (c.sm.size() == 1 && c.sm[0] == m) t.fc += 3 * c.fc;
+            if (c.sm.size() == 2) {
+                if (c.sm[0] == m && c.mp[0] == 1)
+                    t.fc += 2 * c.fc;
+                else if (c.sm[1] == m && c.mp[1] == 1)
+                    t.fc += 2 * c.fc;
+            }
+        }
+        if (std::abs(t.fc) > 1e-8) P.hb.push_back(t);
+    }
+    for (int m = 0; m < M; m++) {
+        ForceConst t;
+        t.fcpow = {m, m};
+        t.sm = {m};
+        t.mp = {2};
+        for (int id : P.quartic) {
+            const ForceConst& c = P.fcs[id];
+            if (c.sm.size() == 1 && c.sm[0] == m) t.fc += 4 * c.fc;
+            if (c.sm.size() == 2) {
+                if (c.sm[0] == m && c.mp[0] == 2)
+                    t.fc += 2 * c.fc;
+                else if (c.sm[1] == m && c.mp[1] == 2)
+                    t.fc += 2 * c.fc;
+            }
+        }
+        if (std::abs(t.fc) > 1e-8) P.hb.push_back(t);
+    }
+    for (int m = 0; m + 1 < M; m++)
+        for (int n = m + 1; n < M; n++) {
+            ForceConst t;
+            t.fcpow = {m, n};
+            t.sm = {m, n};
+            t.mp = {1, 1};
+            for (int id : P.quartic) {
+                const ForceConst& c = P.fcs[id];
+                bool hm = has(c.sm, m), hn = has(c.sm, n);
+                if (c.sm.size() == 2 && has(c.mp, 3) && hm && hn) t.fc += 3 * c.fc;
+                if (c.sm.size() == 3 && hm && hn) {
+                    int im = (int)(std::find(c.sm.begin(), c.sm.end(), m) - c.sm.begin());
+                    int in_ = (int)(std::find(c.sm.begin(), c.sm.end(), n) - c.sm.begin());
+                    if (c.mp[im] == 1 && c.mp[in_] == 1) t.fc += 2 * c.fc;
+                }
+            }
+            if (std::abs(t.fc) > 1e-8) P.hb.push_back(t);
+        }
+    // HeatBath_Sort_FC (HB.cpp:17-22), stable so ties keep construction order
+    P.hb_sorted.resize(P.hb.size());
+    for (size_t i = 0; i < P.hb.size(); i++) P.hb_sorted[i] = (int)i;
+    std::stable_sort(P.hb_sorted.begin(), P.hb_sorted.end(),
+                     [&](int a, int b) { return std::abs(P.hb[a].fc) > std::abs(P.hb[b].fc); });
+    P.fcmax = 0;
+    for (auto& f : P.fcs) P.fcmax = std::max(P.fcmax, f.order());
+    return "";
+}
+
+std::string load_input(const std::string& path, bool restart, Problem& P) {
+    std::ifstream in(path);
+    if (!in.good()) return "Error: Could not open input file.";
+    return parse_input(in, restart, P);
+}
+
+double ladder_sum(int k, int qs, int qe) {
+    // Expand the path list level by level: every entry first spawns its
+    // creation branch (coefficient *= sqrt(q+1)), then, if q >= 1, its
+    // annihilation branch (coefficient *= sqrt(q)).
+    std::vector<int> q{qs}, nq;
+    std::vector<double> c{1.0}, nc;
+    for (int s = 0; s < k; s++) {
+        nq.clear();
+        nc.clear();
+        for (size_t e = 0; e < q.size(); e++) {
+            nq.push_back(q[e] + 1);
+            nc.push_back(c[e] * std::sqrt(q[e] + 1));
+            if (q[e] >= 1) {
+                nq.push_back(q[e] - 1);
+                nc.push_back(c[e] * std::sqrt(q[e]));
+            }
+        }
+        q.swap(nq);
+        c.swap(nc);
+    }
+    double sum = 0.0;
+    for (size_t e = 0; e < q.size(); e++)
+        if (q[e] == qe) sum += c[e];
+    return sum;
+}
+
+void build_ladder(int qmax, std::vector<double>& L) {
+    const int nq = qmax + 1;
+    L.assign((size_t)7 * nq * kLadderW, 0.0);
+    for (int k = 0; k <= 6; k++)
+        for (int qs = 0; qs < nq; qs++)
+            for (int d = -k; d <= k; d++) {
+                int qe = qs + d;
+                if (qe < 0 || ((d + k) & 1)) continue;
+                L[((size_t)k * nq + qs) * kLadderW + (d + k)] = ladder_sum(k, qs, qe);
+            }
+}
+
+void build_tables(const Problem& P, Tables& T) {
+    T = Tables();
+    // element FCs: Quartic, Sextic, Cubic, Quintic
+    std::vector<int> order;
+    for (auto* lst : {&P.quartic, &P.sextic, &P.cubic, &P.quintic}) order.insert(order.end(), lst->begin(), lst->end());
+    for (int id : order) {
+        const ForceConst& f = P.fcs[id];
+        ElemFC e{};
+        e.fc = f.fc;
+        e.d = (uint8_t)f.sm.size();
+        for (size_t t = 0; t < f.sm.size(); t++) {
+            e.mode[t] = (uint8_t)f.sm[t];
+            e.pow[t] = (uint8_t)f.mp[t];
+        }
+        T.elem.push_back(e);
+    }
+    // connection groups: the set of |delta| patterns every element FC can
+    // bridge (ScreenState, Ham.cpp:401-446, is true exactly on these), with
+    // each group's FCs in element order.  Odd patterns with qdiff > fcmax-1
+    // fail the reference's odd prefilter (Ham.cpp:283) and are dropped.
+    std::map<std::vector<std::pair<int, int>>, std::vector<uint32_t>> gm;
+    for (uint32_t e = 0; e < T.elem.size(); e++) {
+        const ElemFC& f = T.elem[e];
+        int d = f.d;
+        int a[6];
+        for (int t = 0; t < d; t++) a[t] = f.pow[t] & 1;
+        while (true) {
+            std::vector<std::pair<int, int>> pat;
+            int qdiff = 0;
+            for (int t = 0; t < d; t++)
+                if (a[t] > 0) {
+                    pat.push_back({f.mode[t], a[t]});
+                    qdiff += a[t];
+                }
+            std::sort(pat.begin(), pat.end());
+            bool odd = qdiff & 1;
+            if (pat.empty())
+                T.diag_fcs.push_back(e);
+            else if (!odd || qdiff <= P.fcmax - 1)
+                gm[pat].push_back(e);
+            int t = d - 1;
+            while (t >= 0) {
+                a[t] += 2;
+                if (a[t] <= f.pow[t]) break;
+                a[t] = f.pow[t] & 1;
+                t--;
+            }
+            if (t < 0) break;
+        }
+    }
+    uint32_t gid = 0;
+    for (auto& kv : gm) {
+        const auto& pat = kv.first;
+        GroupEnt g{(uint32_t)T.fclist.size(), (uint32_t)kv.second.size()};
+        T.fclist.insert(T.fclist.end(), kv.second.begin(), kv.second.end());
+        T.groups.push_back(g);
+        int nz = (int)pat.size();
+        for (int s = 0; s < (1 << nz); s++) {
+            DeltaEnt de{};
+            for (int t = 0; t < nz; t++) {
+                de.mode[t] = (uint8_t)pat[t].first;
+                int v = pat[t].second;
+                de.delta[t] = (int8_t)(((s >> t) & 1) ? -v : v);
+                T.max_delta = std::max(T.max_delta, v);
+            }
+            de.meta = gid | ((uint32_t)nz << 28);
+            T.deltas.push_back(de);
+        }
+        gid++;
+    }
+    // heat-bath variants in sorted-FC order (HB.cpp:27-164): delta_t runs
+    // -k..k in steps of 2 with the last mode fastest; all-zero is skipped.
+    T.hb_varoff.push_back(0);
+    for (int idx : P.hb_sorted) {
+        const ForceConst& f = P.hb[idx];
+        T.hb_fc.push_back(f.fc);
+        int d = (int)f.sm.size();
+        int dl[6];
+        for (int t = 0; t < d; t++) dl[t] = -f.mp[t];
+        int64_t cnt = 0;
+        while (d > 0) {
+            bool zero = true;
+            for (int t = 0; t < d; t++) zero = zero && dl[t] == 0;
+            if (!zero) {
+                HbVar v{};
+                v.d = (uint8_t)d;
+                for (int t = 0; t < d; t++) {
+                    v.mode[t] = (uint8_t)f.sm[t];
+                    v.delta[t] = (int8_t)dl[t];
+                    T.max_hb_delta = std::max(T.max_hb_delta, std::abs(dl[t]));
+                }
+                T.hb_vars.push_back(v);
+                cnt++;
+            }
+            int t = d - 1;
+            while (t >= 0) {
+                dl[t] += 2;
+                if (dl[t] <= f.mp[t]) break;
+                dl[t] = -f.mp[t];
+                t--;
+            }
+            if (t < 0) break;
+        }
+        T.hb_varoff.push_back(T.hb_varoff.back() + cnt);
+    }
+}
+
+void write_freqs(std::ostream& os, const Problem& P, const int32_t* basis, int64_t rows,
+                 const std::vector<double>& E, const double* C, int ncol) {
+    const int M = P.M;
+    int nf = std::min<int>(P.neig, (int)E.size());
+    auto labels = [&](int col, const char* sep) {
+        double mx = -std::numeric_limits<double>::infinity(), mn = std::numeric_limits<double>::infinity();
+        for (int64_t m = 0; m < rows; m++) {
+            mx = std::max(mx, C[m * ncol + col]);
+            mn = std::min(mn, C[m * ncol + col]);
+        }
+        const double half = std::max(std::abs(mx), std::abs(mn)) / 2;
+        int written = 0;
+        for (int64_t m = 0; m < rows; m++) {
+            double c = std::abs(C[m * ncol + col]);
+            if (!(c >= half)) continue;
+            if (written++ > 0) os << sep;
+            os << std::fixed << std::setprecision(2) << c << " (";
+            int nm = 0;
+            for (int l = 0; l < M; l++) {
+                int q = basis[m * M + l];
+                if (q > 0) os << (nm++ ? "+" : "") << q << "w" << l;
+            }
+            os << ")";
+        }
+    };
+    // line 0: ZPE, assignments written back to back (Ham.cpp:454-472)
+    os << std::fixed << std::setprecision(10) << E[0] << "   ";
+    labels(0, "");
+    os << '\n';
+    for (int n = 1; n < nf; n++) {
+        os << std::fixed << std::setprecision(12) << E[n] - E[0] << "   ";
+        labels(n, "  ");
+        os << '\n';
+    }
+    os.flush();
+}
+
+void write_checkpoint(std::ostream& os, const Problem& P, const int32_t* basis, int64_t n,
+                      const std::vector<double>& E, const double* C, int ncol) {
+    const int M = P.M;
+    os << n << " " << M << " " << P.neig << '\n';
+    os << std::fixed << std::setprecision(17);
+    for (int j = 0; j < M; j++) os << P.freq[j] << " ";
+    for (double e : E) os << e << " ";
+    os << '\n';
+    for (int64_t i = 0; i < n; i++) {
+        for (int j = 0; j < M; j++) os << basis[i * M + j] << " ";
+        for (int j = 0; j < ncol; j++) os << C[i * ncol + j] << " ";
+        os << '\n';
+    }
+    os.flush();
+}
+
+std::string read_checkpoint(std::istream& is, const Problem& P, std::vector<int32_t>& basis,
+                            std::vector<double>& E, std::vector<double>& C) {
+    int64_t nb = 0;
+    int par = 0;
+    is >> nb >> par;
+    if (par != P.M) return "Error: Checkpoint file contains wrong number of fundamental modes!";
+    is >> par;
+    if (par != P.neig) return "Error: Checkpoint file contains wrong number of eigenstates!";
+    for (int i = 0; i < P.M; i++) {
+        double f = 0;
+        is >> f;
+        if (f != P.freq[i]) return "Error: Checkpoint file contains wrong fundamental frequencies!";
+    }
+    E.assign(P.neig, 0.0);
+    for (int i = 0; i < P.neig; i++) is >> E[i];
+    if (nb < 1 || !is) return "Error: checkpoint file is truncated.";
+    basis.assign((size_t)nb * P.M, 0);
+    C.assign((size_t)nb * P.neig, 0.0);
+    for (int64_t i = 0; i < nb; i++) {
+        for (int j = 0; j < P.M; j++) is >> basis[i * P.M + j];
+        for (int j = 0; j < P.neig; j++) is >> C[i * P.neig + j];
+    }
+    if (!is) return "Error: checkpoint file is truncated.";
+    return "";
+}
+
+}  // namespace vhci

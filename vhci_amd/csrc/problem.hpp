@@ -1,0 +1,123 @@
+// problem.hpp -- host-side problem model of the heat-bath VCI engine.
+//
+// Holds what the reference keeps in process globals (VCI_headers.h:147-160):
+// frequencies, force constants split by order, the heat-bath list, the
+// initial basis and the run settings, plus the tables the device kernels use
+// (connection patterns, per-mode ladder tables, heat-bath variants).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace vhci {
+
+// One force constant in the reference's FConst layout (VCI_headers.h:121-129):
+// fcpow lists the mode of every power; sm/mp are the distinct modes in
+// first-seen order and their powers.
+struct ForceConst {
+    double fc = 0.0;
+    std::vector<int> fcpow;
+    std::vector<int> sm;
+    std::vector<int> mp;
+    int order() const { return (int)fcpow.size(); }
+};
+
+struct Problem {
+    double eps1 = 0.0, eps2 = 0.0;
+    int neig = 0;
+    bool perturb = false;
+    int ntot = 0;
+    int M = 0;
+    int nfc = 0;
+    bool restart = false;
+    std::vector<double> freq;
+    std::vector<int> cap;
+    std::vector<ForceConst> fcs;     // AnharmFC, file order, scaled
+    std::vector<int> cubic, quartic, quintic, sextic;  // indices into fcs
+    std::vector<ForceConst> hb;      // AnharmHB in construction order
+    std::vector<int> hb_sorted;      // indices into hb, |fc| descending, stable
+    std::vector<int32_t> basis0;     // initial basis, N0 x M
+    int fcmax = 0;
+    int64_t n0() const { return M ? (int64_t)basis0.size() / M : 0; }
+};
+
+// Parse an input file with the reference's token order (Input_Reader.cpp:170-492).
+// Returns an empty string on success, else the reference's error message.
+std::string parse_input(std::istream& in, bool restart, Problem& P);
+std::string load_input(const std::string& path, bool restart, Problem& P);
+
+// Sum over ladder paths of (a + a^dag)^k from quanta qs to qe, enumerated and
+// accumulated in the reference's order (Ham.cpp:16-95).
+double ladder_sum(int k, int qs, int qe);
+
+// ---------------------------------------------------------------------------
+// Device-side tables.
+//
+// Element FCs: the four order lists concatenated as Quartic, Sextic, Cubic,
+// Quintic, so that every connection group's FC list is ascending in this
+// numbering and matches the reference's summation order (Ham.cpp:251-313).
+struct ElemFC {
+    double fc;
+    uint8_t d;
+    uint8_t mode[6];
+    uint8_t pow[6];
+    uint8_t pad[3];
+};
+static_assert(sizeof(ElemFC) == 24, "ElemFC layout");
+
+// One signed connection pattern: up to 6 modes change by delta.  meta packs
+// the group id (low 26 bits) and the number of changed modes (bits 28..30).
+struct DeltaEnt {
+    uint8_t mode[6];
+    int8_t delta[6];
+    uint32_t meta;
+};
+static_assert(sizeof(DeltaEnt) == 16, "DeltaEnt layout");
+
+struct GroupEnt {
+    uint32_t fc_start;
+    uint32_t fc_count;
+};
+
+// One heat-bath candidate displacement of one HB force constant, in the
+// reference's enumeration order (HB.cpp:27-164).
+struct HbVar {
+    uint8_t d;
+    uint8_t mode[6];
+    int8_t delta[6];
+    uint8_t pad[3];
+};
+static_assert(sizeof(HbVar) == 16, "HbVar layout");
+
+struct Tables {
+    std::vector<ElemFC> elem;          // element FC table
+    std::vector<DeltaEnt> deltas;      // signed patterns, grouped
+    std::vector<GroupEnt> groups;      // per group: range in fclist
+    std::vector<uint32_t> fclist;      // element FC ids per group
+    std::vector<uint32_t> diag_fcs;    // element FC ids of the zero pattern
+    // heat bath: sorted FC values and the flattened variant table
+    std::vector<double> hb_fc;         // |fc| descending (signed values)
+    std::vector<int64_t> hb_varoff;    // prefix of variant counts, size nhb+1
+    std::vector<HbVar> hb_vars;
+    int max_hb_delta = 0;              // largest |delta| of any HB variant
+    int max_delta = 0;                 // largest |delta| of any pattern
+};
+
+void build_tables(const Problem& P, Tables& T);
+
+// Ladder table T[k][qs][qe-qs+k], k = 0..6, qs = 0..qmax, 13 entries per row.
+void build_ladder(int qmax, std::vector<double>& L);
+constexpr int kLadderW = 13;
+
+// Output files -----------------------------------------------------------
+// PrintFreqs (Ham.cpp:448-502): C row-major rows x ncol.
+void write_freqs(std::ostream& os, const Problem& P, const int32_t* basis, int64_t rows,
+                 const std::vector<double>& E, const double* C, int ncol);
+// WriteCheckpoint (Core_functions.cpp:94-117)
+void write_checkpoint(std::ostream& os, const Problem& P, const int32_t* basis, int64_t n,
+                      const std::vector<double>& E, const double* C, int ncol);
+// ReadCheckpoint (Core_functions.cpp:119-167); returns "" or the error text
+std::string read_checkpoint(std::istream& is, const Problem& P, std::vector<int32_t>& basis,
+                            std::vector<double>& E, std::vector<double>& C);
+
+}  // namespace vhci

@@ -1,0 +1,171 @@
+"""Seeded synthetic force fields in the reference's input format.
+
+The generator is deterministic across machines: a splitmix64-seeded
+xoshiro256** stream (Blackman & Vigna) in pure Python integer arithmetic, so
+every box that runs it writes byte-identical input files.
+
+Workloads (seed = 0x5648434900 + config):
+  c0  conventional VCI: 12 modes, all cubic i<=j<=k (364) and all
+      semi-diagonal quartic iiii/iiij/iijj (210); caps 4, Ntot 4 (N = 1820);
+      eps1 = 0, Nstate = 50, no PT2.
+  c1  heat-bath VCI + PT2: 15 modes, all 680 cubic, the 330 semi-diagonal
+      quartics plus 270 distinct f_iijk (600 quartic); Ntot 2, caps 6,
+      eps1 = 1.0, Nstate = 100, eps2 = 0.01.
+  c2  48 modes: cubic and quartic (semi-diagonal + iijk) pools sampled with
+      |f| >= 0.5 kept; Ntot 2, caps 6, eps1 = 1, Nstate = 128, eps2 = 0.2.
+Frequencies are uniform in [lo, hi]; |f| is log-uniform in [a, b] with a
+random sign.  Values are written with 9 decimals.
+"""
+
+MASK = (1 << 64) - 1
+BASE_SEED = 0x5648434900
+
+
+def _splitmix(state):
+    state = (state + 0x9E3779B97F4A7C15) & MASK
+    z = state
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK
+    return state, z ^ (z >> 31)
+
+
+def _rotl(x, k):
+    return ((x << k) | (x >> (64 - k))) & MASK
+
+
+class Rng:
+    """xoshiro256** seeded by splitmix64."""
+
+    def __init__(self, seed):
+        st = seed & MASK
+        self.s = []
+        for _ in range(4):
+            st, z = _splitmix(st)
+            self.s.append(z)
+
+    def next(self):
+        s = self.s
+        result = (_rotl((s[1] * 5) & MASK, 7) * 9) & MASK
+        t = (s[1] << 17) & MASK
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = _rotl(s[3], 45)
+        return result
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * ((self.next() >> 11) * (1.0 / 9007199254740992.0))
+
+    def logmag(self, a, b):
+        import math
+        mag = 10 ** self.uniform(math.log10(a), math.log10(b))
+        sign = -1.0 if (self.next() >> 63) else 1.0
+        return sign * mag
+
+    def randint(self, n):
+        return self.next() % n
+
+
+def _cubic_all(M):
+    return [(i, j, k) for i in range(M) for j in range(i, M) for k in range(j, M)]
+
+
+def _semidiag_quartic(M):
+    out = []
+    for i in range(M):
+        out.append((i, i, i, i))
+    for i in range(M):
+        for j in range(M):
+            if i != j:
+                out.append((i, i, i, j))
+    for i in range(M):
+        for j in range(i + 1, M):
+            out.append((i, i, j, j))
+    return out
+
+
+def _iijk_pool(M):
+    out = []
+    for i in range(M):
+        for j in range(M):
+            for k in range(j + 1, M):
+                if j != i and k != i:
+                    out.append((i, i, j, k))
+    return out
+
+
+def write_input(path_or_none, comment, eps1, nstate, pt2, eps2, ntot, freqs, caps, fcs):
+    lines = ["Comment: " + comment,
+             "HCI_Eps: %s" % repr(float(eps1)),
+             "Nstate: %d" % nstate,
+             "PT2: %d" % (1 if pt2 else 0),
+             "PT2_Eps: %s" % repr(float(eps2)),
+             "Max_quanta: %d" % ntot,
+             "Modes: %d" % len(freqs)]
+    for i, (f, c) in enumerate(zip(freqs, caps)):
+        lines.append(" %d %.6f %d" % (i, f, c))
+    lines.append("Force_constants: %d" % len(fcs))
+    for modes, v in fcs:
+        lines.append(" %d %s %.9f" % (len(modes), " ".join(str(m) for m in modes), v))
+    text = "\n".join(lines) + "\n"
+    if path_or_none:
+        with open(path_or_none, "w") as f:
+            f.write(text)
+    return text
+
+
+def field(M, flo, fhi, seed, cubic=(1.0, 300.0), quartic=(0.1, 60.0), n_extra_iijk=0, cutoff=0.0,
+          cubic_pool=None, quartic_pool=None):
+    rng = Rng(seed)
+    freqs = [rng.uniform(flo, fhi) for _ in range(M)]
+    fcs = []
+    for t in (cubic_pool if cubic_pool is not None else _cubic_all(M)):
+        v = rng.logmag(*cubic)
+        if abs(v) >= cutoff:
+            fcs.append((t, v))
+    qp = quartic_pool if quartic_pool is not None else _semidiag_quartic(M)
+    for t in qp:
+        v = rng.logmag(*quartic)
+        if abs(v) >= cutoff:
+            fcs.append((t, v))
+    if n_extra_iijk:
+        pool = _iijk_pool(M)
+        chosen = set()
+        while len(chosen) < n_extra_iijk:
+            chosen.add(rng.randint(len(pool)))
+        for idx in sorted(chosen):
+            fcs.append((pool[idx], rng.logmag(*quartic)))
+    return freqs, fcs
+
+
+def config_input(c, path=None, **override):
+    """Input text (and file when path is given) of synthetic workload c."""
+    seed = BASE_SEED + c
+    if c == 0:
+        M = 12
+        freqs, fcs = field(M, 700.0, 3300.0, seed)
+        args = dict(eps1=0.0, nstate=50, pt2=False, eps2=0.0, ntot=4, caps=[4] * M)
+    elif c == 1:
+        M = 15
+        freqs, fcs = field(M, 700.0, 3300.0, seed, n_extra_iijk=270)
+        args = dict(eps1=1.0, nstate=100, pt2=True, eps2=0.01, ntot=2, caps=[6] * M)
+    elif c == 2:
+        M = 48
+        freqs, fcs = field(M, 150.0, 3200.0, seed, cubic=(0.05, 300.0), quartic=(0.05, 60.0), cutoff=0.5,
+                           quartic_pool=_semidiag_quartic(M) + _iijk_pool(M)[::8])
+        args = dict(eps1=1.0, nstate=128, pt2=True, eps2=0.2, ntot=2, caps=[6] * M)
+    else:
+        raise ValueError("unknown workload %r" % c)
+    args.update(override)
+    caps = args.pop("caps")
+    return write_input(path, "synthetic workload c%d seed %#x" % (c, seed), freqs=freqs, caps=caps, fcs=fcs,
+                       **args)
+
+
+if __name__ == "__main__":
+    import sys
+    c = int(sys.argv[1])
+    out = sys.argv[2]
+    config_input(c, out)
