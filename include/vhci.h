@@ -133,6 +133,22 @@ int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out);
 int vhci_solve(const vhci_problem* p, int form, int device, double tol, double* E, int* nE, int64_t* nvar,
                int64_t* ns2, double* timings);
 
+/* Same loop on an existing context whose basis is the problem's initial
+ * basis (restart must be 0: resuming from a checkpoint is done by the
+ * `vhci` program, which also holds the checkpoint's energies); the final
+ * basis, H and Ritz block stay resident on the device.  E receives the
+ * final energies (after PT2), Evar (nullable) the variational ones, *nE
+ * their count.  info (nullable, 4 entries): final basis size, PT2 space
+ * size, heat-bath iterations, basis checksum (sum over the int32 quanta of
+ * the final basis, row-major, of q_t * 1099511628211^(t+1) mod 2^64).
+ * timings (nullable, 8 entries) as in vhci_solve. */
+int vhci_run_ctx(vhci_ctx* ctx, double tol, int restart, double* E, double* Evar, int* nE, int64_t* info,
+                 double* timings);
+
+/* Basis size after each heat-bath iteration of the last vhci_run_ctx on
+ * this context (the initial size first); returns the count written. */
+int vhci_iteration_sizes(const vhci_ctx* ctx, int64_t* sizes, int max);
+
 /* The drop-in program: same flags, input, freqs.dat and checkpoint formats
  * and log lines as the reference's `vhci -n -i -o -c`. argv as in main. */
 int vhci_main(int argc, char** argv);

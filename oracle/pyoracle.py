@@ -53,6 +53,8 @@ def lib():
             "orc_cmax": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int, ctypes.c_int, _dp, _dp]),
             "orc_pt2": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _dp, ctypes.c_int, _dp, ctypes.c_int,
                                        ctypes.c_double, _dp, _i64p]),
+            "orc_sample_rates": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, ctypes.c_int64, _i32p, ctypes.c_int64,
+                                                ctypes.c_int, _dp]),
             "orc_run": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
         }
@@ -140,6 +142,15 @@ class OProblem:
         lib().orc_hb_select(self.h, _p(q, _i32p), q.shape[0], _p(cmax, _dp), eps, ctypes.byref(nn),
                             _p(out, _i32p))
         return out
+
+    def sample_rates(self, qV, rows, qA, nopt):
+        """(H pairs/s, PT2 pairs/s, details) on a bounded sample."""
+        qV = np.ascontiguousarray(qV, dtype=np.int32)
+        qA = np.ascontiguousarray(qA, dtype=np.int32)
+        out = np.zeros(4)
+        lib().orc_sample_rates(self.h, _p(qV, _i32p), qV.shape[0], rows, _p(qA, _i32p), qA.shape[0], nopt,
+                               _p(out, _dp))
+        return out[0] / out[1], out[2] / out[3], out
 
     def pt2(self, q, C, E, nopt, eps2):
         q = np.ascontiguousarray(q, dtype=np.int32)

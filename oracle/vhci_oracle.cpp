@@ -1130,6 +1130,52 @@ int orc_pt2(const orc_problem* p, const int32_t* q, int64_t n, const double* C, 
     return 0;
 }
 
+int orc_sample_rates(const orc_problem* p, const int32_t* qV, int64_t nV, int64_t rows, const int32_t* qA,
+                     int64_t na, int nopt, double* out) {
+    using clk = std::chrono::steady_clock;
+    const Problem& P = p->P;
+    const int M = P.M;
+    int nt = g_threads > 0 ? g_threads : omp_get_max_threads();
+    rows = std::min(rows, nV);
+    // H pair loop (no triplet storage: the sums are reduced into a checksum
+    // so the work is not optimised away)
+    double chk = 0;
+    auto t0 = clk::now();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) reduction(+ : chk)
+    for (int64_t i = 0; i < rows; i++) {
+        std::vector<int> dv(M);
+        for (int64_t j = i; j < nV; j++) {
+            double v;
+            if (pair_element(P, qV + i * M, qV + j * M, dv.data(), v)) chk += v;
+        }
+    }
+    double th = std::chrono::duration<double>(clk::now() - t0).count();
+    double hpairs = 0;
+    for (int64_t i = 0; i < rows; i++) hpairs += (double)(nV - i);
+    // PT2 gather with unit coefficients
+    std::vector<double> C((size_t)nV * nopt, 1e-3);
+    t0 = clk::now();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt) reduction(+ : chk)
+    for (int64_t a = 0; a < na; a++) {
+        std::vector<int> dv(M);
+        std::vector<double> sum(nopt, 0.0);
+        const int* qa = qA + a * M;
+        for (int64_t i = 0; i < nV; i++) {
+            double v;
+            if (pair_element(P, qa, qV + i * M, dv.data(), v))
+                for (int s2 = 0; s2 < nopt; s2++) sum[s2] += v * C[i * nopt + s2];
+        }
+        for (int s2 = 0; s2 < nopt; s2++) chk += sum[s2] * sum[s2];
+    }
+    double tp = std::chrono::duration<double>(clk::now() - t0).count();
+    out[0] = hpairs;
+    out[1] = th;
+    out[2] = (double)na * (double)nV;
+    out[3] = tp;
+    if (chk == 12345.6789) out[0] += 1;  // keeps chk live
+    return 0;
+}
+
 /* Perform_HCI + DoPT2 + PrintFreqs + checkpoint policy (VHCI.cpp:5-185,
  * HB.cpp:176-265). */
 int orc_run(const char* input, const char* freqs_out, const char* ckpt, int nthreads, const char* json_out,

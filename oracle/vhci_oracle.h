@@ -90,6 +90,14 @@ int orc_pt2(const orc_problem* p, const int32_t* q, int64_t n, const double* C,
             int ncol, const double* E, int nopt, double eps2, double* dE,
             int64_t* ns2);
 
+/* Pair-evaluation rates of the reference algorithms on a bounded sample,
+ * for the CPU baseline: the all-pairs H loop (Ham.cpp:237-324) over rows
+ * [0, rows) of qV against every j >= i, and the PT2 gather (PT2.cpp:66-149)
+ * of the na external states qA against all of qV.  out[4] = H pairs, H
+ * seconds, PT2 pairs, PT2 seconds (steady clock, current thread count). */
+int orc_sample_rates(const orc_problem* p, const int32_t* qV, int64_t nV, int64_t rows, const int32_t* qA,
+                     int64_t na, int nopt, double* out);
+
 /* Whole program (VHCI.cpp:5-185): writes freqs file, optional checkpoint and
  * a JSON summary (basis size per iteration, PT2 space, energies as %.17g,
  * structural nnz per build).  basis_out (optional) receives the final

@@ -58,6 +58,8 @@ _SIGS = {
     "vhci_pt2": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _dp, _i64p]),
     "vhci_time_spmm": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "vhci_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _i64p, _i64p, _dp]),
+    "vhci_run_ctx": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _ip, _i64p, _dp]),
+    "vhci_iteration_sizes": (ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
     "vhci_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
 }
 
@@ -254,10 +256,39 @@ class Context:
         self._check(lib().vhci_pt2(self.h, _ptr(E, _dp), nopt, float(eps2), _ptr(dE, _dp), ctypes.byref(ns2)))
         return dE, ns2.value
 
+    def run(self, tol=1e-12, restart=False):
+        """The whole select -> build -> diagonalize -> PT2 loop on this context."""
+        nmax = max(self.problem.neig, self.n, 1)
+        E = np.zeros(nmax)
+        Ev = np.zeros(nmax)
+        nE = ctypes.c_int()
+        info = np.zeros(4, dtype=np.int64)
+        t = np.zeros(8)
+        self._check(lib().vhci_run_ctx(self.h, tol, int(restart), _ptr(E, _dp), _ptr(Ev, _dp), ctypes.byref(nE),
+                                       _ptr(info, _i64p), _ptr(t, _dp)))
+        keys = ["total_ms", "build_ms", "eig_ms", "hb_ms", "pt2_select_ms", "pt2_sum_ms", "iterations",
+                "spmm_vectors"]
+        return {"E": E[: nE.value].copy(), "E_var": Ev[: nE.value].copy(), "nvar": int(info[0]),
+                "ns2": int(info[1]), "hb_iterations": int(info[2]), "checksum": int(info[3]) & ((1 << 64) - 1),
+                "timings": dict(zip(keys, t.tolist()))}
+
+    def iteration_sizes(self):
+        out = np.zeros(64, dtype=np.int64)
+        k = lib().vhci_iteration_sizes(self.h, _ptr(out, _i64p), 64)
+        return [int(x) for x in out[:k]]
+
     def time_spmm(self, b, reps=20, fp32=False):
         out = np.zeros(2)
         self._check(lib().vhci_time_spmm(self.h, b, reps, int(fp32), _ptr(out, _dp)))
         return float(out[0]), float(out[1])
+
+
+def basis_checksum(q):
+    """Checksum of a basis (int32 rows) as vhci_run_ctx reports it."""
+    v = np.ascontiguousarray(q, dtype=np.int32).ravel().astype(np.int64).astype(np.uint64)
+    with np.errstate(over="ignore"):
+        pw = np.cumprod(np.full(v.size, 1099511628211, dtype=np.uint64), dtype=np.uint64)
+        return int(np.sum(v * pw, dtype=np.uint64))
 
 
 def solve(problem, form=FP64, device=0, tol=1e-12):

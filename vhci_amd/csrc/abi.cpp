@@ -32,6 +32,7 @@ struct vhci_problem {
 struct vhci_ctx {
     std::unique_ptr<Engine> eng;
     std::string err;
+    std::vector<int64_t> sizes;
 };
 
 namespace {
@@ -80,7 +81,9 @@ struct RunResult {
     int64_t ns2 = 0;
     double t_build = 0, t_eig = 0, t_hb = 0, t_pt2sel = 0, t_pt2sum = 0;
     int iters = 0;
+    int hb_iters = 0;
     int64_t spmm_vectors = 0;
+    std::vector<int64_t> sizes;
 };
 
 struct Logger {
@@ -124,8 +127,8 @@ void save_checkpoint(Engine& eng, const Problem& P, const std::vector<double>& E
 }
 
 void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Logger& log) {
-    using clk = std::chrono::steady_clock;
     std::vector<double> E;
+    R.sizes.push_back(eng.n());
     if (P.eps1 == 0.) {
         // one-shot VCI (VHCI.cpp:49-76)
         if (!o.restart) {
@@ -170,12 +173,14 @@ void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Lo
             const int nopt = (int)std::min<int64_t>(P.neig, n);
             auto hb = eng.hb_select(nullptr, nopt, P.eps1);
             R.t_hb += hb.ms;
+            R.hb_iters++;
             if ((double)eng.new_count() / (double)n <= 0.01) {
                 log(" Basis converged after " + std::to_string(iter - 1) + " iterations and contains " +
                     std::to_string(n) + " product states.\n");
                 break;
             }
             eng.append_new_states();
+            R.sizes.push_back(eng.n());
             log(" Basis size after iteration " + std::to_string(iter) + " is: " + std::to_string(eng.n()) + "\n");
             ++iter;
             log(" Constructing Hamiltonian...");
@@ -493,6 +498,65 @@ int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
     });
 }
 
+static uint64_t basis_checksum(Engine& eng) {
+    const int64_t n = eng.n();
+    const int M = eng.problem().M;
+    std::vector<int32_t> q((size_t)n * M);
+    eng.get_basis(q.data());
+    const uint64_t P = 1099511628211ull;
+    uint64_t h = 0, pw = 1;
+    for (int32_t v : q) {
+        pw *= P;
+        h += (uint64_t)(int64_t)v * pw;
+    }
+    return h;
+}
+
+static void fill_timings(double* t, const RunResult& R, double total) {
+    if (!t) return;
+    t[0] = total;
+    t[1] = R.t_build;
+    t[2] = R.t_eig;
+    t[3] = R.t_hb;
+    t[4] = R.t_pt2sel;
+    t[5] = R.t_pt2sum;
+    t[6] = R.iters;
+    t[7] = (double)R.spmm_vectors;
+}
+
+int vhci_run_ctx(vhci_ctx* ctx, double tol, int restart, double* E, double* Evar, int* nE, int64_t* info,
+                 double* timings) {
+    return guard(ctx, [&] {
+        if (restart) throw std::runtime_error("vhci_run_ctx: restart is handled by the vhci program");
+        auto t0 = std::chrono::steady_clock::now();
+        Engine& eng = *ctx->eng;
+        RunOpts o;
+        if (tol > 0) o.tol = tol;
+        o.restart = restart != 0;
+        RunResult R;
+        run(eng, eng.problem(), o, R, Logger{false});
+        ctx->sizes = R.sizes;
+        if (E) std::memcpy(E, R.E.data(), R.E.size() * sizeof(double));
+        if (Evar) std::memcpy(Evar, R.Evar.data(), R.Evar.size() * sizeof(double));
+        if (nE) *nE = (int)R.E.size();
+        if (info) {
+            info[0] = R.nvar;
+            info[1] = R.ns2;
+            info[2] = R.hb_iters;
+            info[3] = (int64_t)basis_checksum(eng);
+        }
+        fill_timings(timings, R,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
+    });
+}
+
+int vhci_iteration_sizes(const vhci_ctx* ctx, int64_t* sizes, int max) {
+    if (!ctx) return VHCI_E_ARG;
+    int n = std::min<int>(max, (int)ctx->sizes.size());
+    for (int i = 0; i < n; i++) sizes[i] = ctx->sizes[i];
+    return n;
+}
+
 int vhci_solve(const vhci_problem* p, int form, int device, double tol, double* E, int* nE, int64_t* nvar,
                int64_t* ns2, double* timings) {
     if (!p) return VHCI_E_ARG;
@@ -508,16 +572,8 @@ int vhci_solve(const vhci_problem* p, int form, int device, double tol, double* 
         if (nE) *nE = (int)R.E.size();
         if (nvar) *nvar = R.nvar;
         if (ns2) *ns2 = R.ns2;
-        if (timings) {
-            timings[0] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-            timings[1] = R.t_build;
-            timings[2] = R.t_eig;
-            timings[3] = R.t_hb;
-            timings[4] = R.t_pt2sel;
-            timings[5] = R.t_pt2sum;
-            timings[6] = R.iters;
-            timings[7] = (double)R.spmm_vectors;
-        }
+        fill_timings(timings, R,
+                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
     } catch (const std::exception& e) {
         g_problem_err = e.what();
         return classify(g_problem_err);

@@ -494,6 +494,206 @@ __global__ void k_colmajor_to_rows(const double* __restrict__ A, int64_t lda, in
 }
 
 // ------------------------------------------------------------------------
+// column-major block helpers of the eigensolver
+// ------------------------------------------------------------------------
+// B (row-major, ldb) <- A (column-major, lda) for an n x k block; 32x32 LDS
+// tiles so both sides are read and written along contiguous addresses.
+template <class TO>
+__global__ __launch_bounds__(256) void k_cm_to_rm(const double* __restrict__ A, int64_t lda, int64_t n, int k,
+                                                  TO* __restrict__ B, int ldb) {
+    __shared__ double tile[32][33];
+    const int64_t r0 = (int64_t)blockIdx.x * 32;
+    const int c0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r = r0 + tx;
+        const int c = c0 + j;
+        tile[j][tx] = (r < n && c < k) ? A[r + (int64_t)c * lda] : 0.0;
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r = r0 + j;
+        const int c = c0 + tx;
+        if (r < n && c < k) B[r * ldb + c] = (TO)tile[tx][j];
+    }
+}
+
+// A (column-major, lda) <- B (row-major, ldb)
+__global__ __launch_bounds__(256) void k_rm_to_cm(const double* __restrict__ B, int ldb, int64_t n, int k,
+                                                  double* __restrict__ A, int64_t lda) {
+    __shared__ double tile[32][33];
+    const int64_t r0 = (int64_t)blockIdx.x * 32;
+    const int c0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r = r0 + j;
+        const int c = c0 + tx;
+        tile[j][tx] = (r < n && c < k) ? B[r * ldb + c] : 0.0;
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        const int64_t r = r0 + tx;
+        const int c = c0 + j;
+        if (r < n && c < k) A[r + (int64_t)c * lda] = tile[tx][j];
+    }
+}
+
+// partial sums over row chunks of (HX - theta X)^2 for column blockIdx.y
+__global__ __launch_bounds__(256) void k_resid_cm(int64_t n, const double* __restrict__ HX,
+                                                  const double* __restrict__ X, int64_t ld,
+                                                  const double* __restrict__ theta, int64_t rows_per_blk,
+                                                  double* __restrict__ part) {
+    __shared__ double red[256];
+    const int c = blockIdx.y;
+    const int64_t r0 = blockIdx.x * rows_per_blk;
+    const int64_t r1 = min(n, r0 + rows_per_blk);
+    const double th = theta[c];
+    double s = 0.0;
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
+        const double x = HX[r + c * ld] - th * X[r + c * ld];
+        s = fma(x, x, s);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(int64_t)c * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ void k_sum_parts(const double* __restrict__ part, int nblk, int b, double* __restrict__ out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= b) return;
+    double s = 0.0;
+    for (int i = 0; i < nblk; i++) s += part[(int64_t)c * nblk + i];
+    out[c] = s;
+}
+
+// T[:, jj] = (HX[:, J[jj]] - theta_J X[:, J[jj]]) / (theta_J - D), column-major
+__global__ void k_precond_cm(int64_t n, int k, const int* __restrict__ J, const double* __restrict__ HX,
+                             const double* __restrict__ X, int64_t ldx, const double* __restrict__ theta,
+                             const double* __restrict__ D, double* __restrict__ T, int64_t ldt) {
+    const int jj = blockIdx.y;
+    const int j = J[jj];
+    const double th = theta[j];
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const double res = HX[r + j * ldx] - th * X[r + j * ldx];
+        double den = th - D[r];
+        if (fabs(den) < 1e-6) den = den < 0 ? -1e-6 : 1e-6;
+        T[r + jj * ldt] = res / den;
+    }
+}
+
+// G (column-major, ld) = diag(theta[0..m)) on an m x m block
+__global__ void k_diag_block(double* G, int m, int ld, const double* __restrict__ theta) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i < m && j < m) G[i + (int64_t)j * ld] = (i == j) ? theta[i] : 0.0;
+}
+
+// start block: columns [0, k0) from the resident row-major Ritz block (rows
+// beyond xrows are zero), columns [k0, b) unit vectors on idx[]
+__global__ void k_start_block(int64_t n, int b, int k0, const double* __restrict__ Xr, int64_t xrows, int xcols,
+                              const int64_t* __restrict__ idx, double* __restrict__ V, int64_t ldv) {
+    const int c = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double v = 0.0;
+        if (c < k0)
+            v = r < xrows ? Xr[r * xcols + c] : 0.0;
+        else
+            v = (idx[c - k0] == r) ? 1.0 : 0.0;
+        V[r + c * ldv] = v;
+    }
+}
+
+// ------------------------------------------------------------------------
+// tall-skinny products of the Krylov basis (n >> p, q)
+// ------------------------------------------------------------------------
+// partial C = A^T B over one row chunk: A n x p, B n x q, column-major.
+// Block: 64 columns of A x 16 columns of B; 32-row tiles staged in LDS with
+// loads along the rows (coalesced), 4 outputs per thread.
+// part layout: [chunk][c][a] (c < q, a < p)
+__global__ __launch_bounds__(256) void k_tsm_tn(int64_t n, int p, int q, const double* __restrict__ A, int64_t lda,
+                                                const double* __restrict__ B, int64_t ldb, int64_t rows_per_chunk,
+                                                double* __restrict__ part) {
+    __shared__ double As[32][65];
+    __shared__ double Bs[32][17];
+    const int t = threadIdx.x;
+    const int a0 = blockIdx.y * 64, c0 = blockIdx.z * 16;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = min(n, r0 + rows_per_chunk);
+    const int aa = t & 63, cg = t >> 6;  // outputs (aa, 4*cg .. 4*cg+3)
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t rt = r0; rt < r1; rt += 32) {
+        for (int idx = t; idx < 32 * 64; idx += 256) {
+            const int i = idx & 31, j = idx >> 5;
+            const int64_t r = rt + i;
+            As[i][j] = (r < r1 && a0 + j < p) ? A[r + (int64_t)(a0 + j) * lda] : 0.0;
+        }
+        for (int idx = t; idx < 32 * 16; idx += 256) {
+            const int i = idx & 31, j = idx >> 5;
+            const int64_t r = rt + i;
+            Bs[i][j] = (r < r1 && c0 + j < q) ? B[r + (int64_t)(c0 + j) * ldb] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int i = 0; i < 32; i++) {
+            const double x = As[i][aa];
+#pragma unroll
+            for (int u = 0; u < 4; u++) acc[u] = fma(x, Bs[i][4 * cg + u], acc[u]);
+        }
+        __syncthreads();
+    }
+    if (a0 + aa < p)
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int c = c0 + 4 * cg + u;
+            if (c < q) part[((int64_t)blockIdx.x * q + c) * p + a0 + aa] = acc[u];
+        }
+}
+
+// C[a + c*ldc] = sum over chunks (fixed order)
+__global__ void k_tsm_reduce(const double* __restrict__ part, int nchunk, int p, int q, double* __restrict__ C,
+                             int ldc) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)p * q) return;
+    const int c = (int)(t / p), a = (int)(t - (int64_t)c * p);
+    double s = 0.0;
+    for (int k = 0; k < nchunk; k++) s += part[((int64_t)k * q + c) * p + a];
+    C[a + (int64_t)c * ldc] = s;
+}
+
+// Y = beta Y + alpha A S: A n x p (lda), S p x q (lds, col-major) staged in
+// LDS (p*q <= 4096), one row per thread, q <= 16 per launch tile.
+__global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double alpha, const double* __restrict__ A,
+                                                int64_t lda, const double* __restrict__ S, int lds, double beta,
+                                                double* __restrict__ Y, int64_t ldy) {
+    __shared__ double Ss[4096];
+    const int c0 = blockIdx.y * 16;
+    const int qq = min(16, q - c0);
+    for (int idx = threadIdx.x; idx < p * qq; idx += blockDim.x) {
+        const int a = idx / qq, c = idx - a * qq;
+        Ss[a * 16 + c] = S[a + (int64_t)(c0 + c) * lds];
+    }
+    __syncthreads();
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double acc[16];
+#pragma unroll
+        for (int c = 0; c < 16; c++) acc[c] = 0.0;
+        for (int a = 0; a < p; a++) {
+            const double x = A[r + (int64_t)a * lda];
+#pragma unroll
+            for (int c = 0; c < 16; c++) acc[c] = fma(x, Ss[a * 16 + c], acc[c]);
+        }
+        for (int c = 0; c < qq; c++) {
+            double* y = Y + r + (int64_t)(c0 + c) * ldy;
+            *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * acc[c];
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // heat-bath selection
 // ------------------------------------------------------------------------
 // Cmax (HB.cpp:204-214): signed entry of largest |C| over the first nopt
@@ -1164,30 +1364,232 @@ EigStats Engine::dense_all(std::vector<double>& E) {
     return st;
 }
 
-EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E) {
-    HIPCHK(hipSetDevice(device_));
-    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
-    auto t0 = clk::now();
-    EigStats st;
-    const int64_t n = n_;
-    if (nev < 1) throw std::runtime_error("nev must be positive");
-    if (nev > n) nev = (int)n;
-    if (nev > 256) throw std::runtime_error("at most 256 eigenpairs are supported");
-    const int b = (int)std::min<int64_t>(n, std::min(256, nev + std::max(4, nev / 5)));
-    if (n <= std::max<int64_t>(4 * b, 1024)) {
-        dense_solve(nev, E, st);
-        st.ms = ms_since(t0);
-        return st;
+// C (p x q, col-major, ldc) = A^T B for column-major A (n x p), B (n x q)
+static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const double* A, int64_t lda,
+                   const double* B, int64_t ldb, double* C, int ldc) {
+    const int64_t rpc = std::max<int64_t>(2048, ((n + 255) / 256 + 31) / 32 * 32);
+    const int nchunk = (int)((n + rpc - 1) / rpc);
+    part.ensure((size_t)nchunk * p * q * sizeof(double));
+    dim3 g(nchunk, (p + 63) / 64, (q + 15) / 16);
+    k_tsm_tn<<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
+    k_tsm_reduce<<<grid_for((int64_t)p * q, 256), 256, 0, s>>>(part.as<double>(), nchunk, p, q, C, ldc);
+    HIPCHK(hipGetLastError());
+}
+
+// Y = beta Y + alpha A S (A n x p, S p x q col-major with lds)
+static void tsm_nn(hipStream_t s, int64_t n, int p, int q, double alpha, const double* A, int64_t lda,
+                   const double* S, int lds, double beta, double* Y, int64_t ldy) {
+    for (int a0 = 0; a0 < p; a0 += 256) {
+        const int pp = std::min(256, p - a0);
+        dim3 g(std::min<unsigned>(grid_for(n, 256), 2048), (q + 15) / 16);
+        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy);
     }
-    const int mmax = (int)std::min<int64_t>(n, 3 * b);
-    const int ld = mmax;
-    V_.ensure((size_t)n * ld * sizeof(double));
-    W_.ensure((size_t)n * ld * sizeof(double));
+    HIPCHK(hipGetLastError());
+}
+
+// Symmetric eigendecomposition of a small k x k matrix (k <= 32) by cyclic
+// Jacobi rotations; lam ascending, U column-major.
+static void small_eigh(int k, std::vector<double> A, std::vector<double>& lam, std::vector<double>& U) {
+    U.assign((size_t)k * k, 0.0);
+    for (int i = 0; i < k; i++) U[(size_t)i * k + i] = 1.0;
+    for (int sweep = 0; sweep < 60; sweep++) {
+        double off = 0, tot = 0;
+        for (int j = 0; j < k; j++)
+            for (int i = 0; i < k; i++) {
+                const double v = A[(size_t)j * k + i] * A[(size_t)j * k + i];
+                tot += v;
+                if (i != j) off += v;
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < k - 1; p++)
+            for (int q = p + 1; q < k; q++) {
+                const double apq = A[(size_t)q * k + p];
+                if (apq == 0.0) continue;
+                const double app = A[(size_t)p * k + p], aqq = A[(size_t)q * k + q];
+                const double tau = (aqq - app) / (2 * apq);
+                const double tt = (tau >= 0 ? 1.0 : -1.0) / (std::abs(tau) + std::sqrt(1 + tau * tau));
+                const double c = 1 / std::sqrt(1 + tt * tt), sn = tt * c;
+                for (int r = 0; r < k; r++) {  // columns p, q
+                    const double x = A[(size_t)p * k + r], y = A[(size_t)q * k + r];
+                    A[(size_t)p * k + r] = c * x - sn * y;
+                    A[(size_t)q * k + r] = sn * x + c * y;
+                }
+                for (int r = 0; r < k; r++) {  // rows p, q
+                    const double x = A[(size_t)r * k + p], y = A[(size_t)r * k + q];
+                    A[(size_t)r * k + p] = c * x - sn * y;
+                    A[(size_t)r * k + q] = sn * x + c * y;
+                }
+                for (int r = 0; r < k; r++) {
+                    const double x = U[(size_t)p * k + r], y = U[(size_t)q * k + r];
+                    U[(size_t)p * k + r] = c * x - sn * y;
+                    U[(size_t)q * k + r] = sn * x + c * y;
+                }
+            }
+    }
+    std::vector<int> idx(k);
+    std::iota(idx.begin(), idx.end(), 0);
+    std::sort(idx.begin(), idx.end(), [&](int x, int y) { return A[(size_t)x * k + x] < A[(size_t)y * k + y]; });
+    lam.resize(k);
+    std::vector<double> Us((size_t)k * k);
+    for (int c = 0; c < k; c++) {
+        lam[c] = A[(size_t)idx[c] * k + idx[c]];
+        for (int r = 0; r < k; r++) Us[(size_t)c * k + r] = U[(size_t)idx[c] * k + r];
+    }
+    U.swap(Us);
+}
+
+// Y = H X for the k column-major vectors of X (ld), through row-major
+// staging in chunks of at most 64 columns.  low selects the mixed form's
+// FP32 product.
+void Engine::apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st) {
+    const int64_t n = n_;
+    const bool f32 = low && form_ == kMixed;
+    for (int c = 0; c < k; c += 64) {
+        const int kk = std::min(64, k - c);
+        stage_in_.ensure((size_t)n * kk * sizeof(double));
+        stage_out_.ensure((size_t)n * kk * sizeof(double));
+        dim3 tg(grid_for(n, 32), (kk + 31) / 32);
+        k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X + c * ld, ld, n, kk, stage_in_.as<double>(), kk);
+        if (f32) {
+            stage_in32_.ensure((size_t)n * kk * sizeof(float));
+            k_cm_to_rm<float><<<tg, 256, 0, stream_>>>(X + c * ld, ld, n, kk, stage_in32_.as<float>(), kk);
+            const unsigned grid = grid_for(n, 4);
+            auto go = [&](auto tag) {
+                constexpr int B = decltype(tag)::value;
+                k_spmm32<B><<<grid, 256, 0, stream_>>>(n, rowptr_.as<int64_t>(), col_.as<int32_t>(), hi_.as<float>(),
+                                                       D_.as<double>(), stage_in32_.as<float>(), kk,
+                                                       stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk);
+            };
+            if (kk <= 1) go(std::integral_constant<int, 1>());
+            else if (kk <= 2) go(std::integral_constant<int, 2>());
+            else if (kk <= 4) go(std::integral_constant<int, 4>());
+            else if (kk <= 8) go(std::integral_constant<int, 8>());
+            else if (kk <= 16) go(std::integral_constant<int, 16>());
+            else if (kk <= 32) go(std::integral_constant<int, 32>());
+            else go(std::integral_constant<int, 64>());
+        } else {
+            spmm(stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk, false);
+        }
+        k_rm_to_cm<<<tg, 256, 0, stream_>>>(stage_out_.as<double>(), kk, n, kk, Y + c * ld, ld);
+        HIPCHK(hipGetLastError());
+        st.spmm_calls++;
+    }
+    st.spmm_vectors += k;
+    if (!f32) st.fp64_spmm_vectors += k;
+}
+
+// pseudo-random column-major block, reproducible from the seed
+__global__ void k_random_cm(double* V, int64_t ld, int64_t n, int k, uint64_t seed) {
+    const int c = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t z = seed ^ ((uint64_t)c * 0x9E3779B97F4A7C15ULL) ^ ((uint64_t)r * 0xD1B54A32D192ED03ULL);
+        z = fmix64(z);
+        V[r + c * ld] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+}
+
+void Engine::random_cm(double* V, int64_t ld, int k, uint64_t seed) {
+    dim3 g(std::min<unsigned>(grid_for(n_, 256), 1024), k);
+    k_random_cm<<<g, 256, 0, stream_>>>(V, ld, n_, k, seed);
+    HIPCHK(hipGetLastError());
+}
+
+// Orthonormalise V[:, m0:m0+k] against V[:, :m0] (two passes of block
+// classical Gram-Schmidt) and within itself (SVQB: eigendecomposition of the
+// scaled Gram matrix, near-dependent directions dropped); returns the number
+// of columns kept, packed at m0.
+int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
+    const int64_t n = n_;
+    const int ni = (int)n;
+    const double one = 1.0, zero = 0.0, mone = -1.0;
+    double* T = V + m0 * ld;
+    S_.ensure((size_t)std::max(1, m0) * k * sizeof(double));
+    gram_.ensure((size_t)k * k * sizeof(double));
+    otmp_.ensure((size_t)n * k * sizeof(double));
+    work_.ensure((size_t)std::max<size_t>(work_.cap / sizeof(double), 2 * (size_t)k) * sizeof(double));
+    info_.ensure(sizeof(int) * 4);
+    const bool small = k <= 32;
+    for (int pass = 0; pass < 2; pass++) {
+        if (m0 > 0) {
+            if (small) {
+                tsm_tn(stream_, tsm_part_, n, m0, k, V, ld, T, ld, S_.as<double>(), m0);
+                tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), m0, 1.0, T, ld);
+            } else {
+                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, m0, k, ni, &one, V,
+                                      ld, T, ld, &zero, S_.as<double>(), m0));
+                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, k, m0, &mone, V, ld,
+                                      S_.as<double>(), m0, &one, T, ld));
+            }
+        }
+        if (small)
+            tsm_tn(stream_, tsm_part_, n, k, k, T, ld, T, ld, gram_.as<double>(), k);
+        else
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, k, k, ni, &one, T, ld, T,
+                                  ld, &zero, gram_.as<double>(), k));
+        std::vector<double> g((size_t)k * k);
+        HIPCHK(hipMemcpyAsync(g.data(), gram_.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        std::vector<double> dsc(k);
+        for (int i = 0; i < k; i++) dsc[i] = g[(size_t)i * k + i] > 0 ? 1.0 / std::sqrt(g[(size_t)i * k + i]) : 0.0;
+        for (int i = 0; i < k; i++)
+            for (int j = 0; j < k; j++) g[(size_t)j * k + i] *= dsc[i] * dsc[j];
+        std::vector<double> lam(k), U((size_t)k * k);
+        if (small) {
+            small_eigh(k, g, lam, U);
+        } else {
+            HIPCHK(hipMemcpyAsync(gram_.p, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+            BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, k, gram_.as<double>(), k,
+                                     work_.as<double>() + k, work_.as<double>(), info_.as<rocblas_int>()));
+            HIPCHK(hipMemcpyAsync(lam.data(), work_.as<double>() + k, k * sizeof(double), hipMemcpyDeviceToHost,
+                                  stream_));
+            HIPCHK(hipMemcpyAsync(U.data(), gram_.p, U.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+        }
+        const double lmax = lam[k - 1];
+        std::vector<int> keep;
+        for (int c = k - 1; c >= 0; c--)
+            if (lam[c] > 1e-10 * std::max(1.0, lmax)) keep.push_back(c);
+        const int kk = (int)keep.size();
+        if (kk == 0) return 0;
+        std::vector<double> Z((size_t)k * kk);
+        for (int c = 0; c < kk; c++) {
+            const int src = keep[c];
+            const double sc = 1.0 / std::sqrt(lam[src]);
+            for (int i = 0; i < k; i++) Z[(size_t)c * k + i] = dsc[i] * U[(size_t)src * k + i] * sc;
+        }
+        S_.ensure((size_t)std::max(std::max(1, m0) * k, k * kk) * sizeof(double));
+        HIPCHK(hipMemcpyAsync(S_.p, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        if (small)
+            tsm_nn(stream_, n, k, kk, 1.0, T, ld, S_.as<double>(), k, 0.0, otmp_.as<double>(), n);
+        else
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, kk, k, &one, T, ld,
+                                  S_.as<double>(), k, &zero, otmp_.as<double>(), ni));
+        HIPCHK(hipMemcpy2DAsync(T, ld * sizeof(double), otmp_.p, n * sizeof(double), n * sizeof(double), kk,
+                                hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        k = kk;
+    }
+    return k;
+}
+
+// Block Davidson (Davidson-Liu): search space V and W = H V column-major
+// n x m, projected matrix grown by new columns only, corrections = the
+// diagonally preconditioned residuals of the unconverged wanted pairs; the
+// b = nev + buffer Ritz block restarts the space at mmax.
+void Engine::davidson(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st) {
+    const int64_t n = n_;
+    const int ni = (int)n;
+    const int b = (int)std::min<int64_t>(n, std::min(256, nev + std::max(4, nev / 5)));
+    int mfac = 5;
+    if (const char* e = std::getenv("VHCI_DAVIDSON_MFAC")) mfac = std::max(2, std::atoi(e));
+    const int mmax = (int)std::min<int64_t>(n, (int64_t)mfac * b);
+    const int64_t ld = n;
+    V_.ensure((size_t)n * mmax * sizeof(double));
+    W_.ensure((size_t)n * mmax * sizeof(double));
     HX_.ensure((size_t)n * b * sizeof(double));
     Xw_.ensure((size_t)n * b * sizeof(double));
-    T_b_.ensure((size_t)n * std::max(b, mmax) * sizeof(double));
     G_.ensure((size_t)mmax * mmax * sizeof(double));
-    S_.ensure((size_t)mmax * std::max(b, mmax) * sizeof(double));
+    Gw_.ensure((size_t)mmax * mmax * sizeof(double));
     theta_.ensure((size_t)mmax * sizeof(double));
     work_.ensure((size_t)mmax * sizeof(double) * 2);
     info_.ensure(sizeof(int) * 4);
@@ -1195,103 +1597,42 @@ EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E
     double* W = W_.as<double>();
     double* HX = HX_.as<double>();
     double* X = Xw_.as<double>();
-    double* Tt = T_b_.as<double>();
     double* G = G_.as<double>();
-    double* S = S_.as<double>();
+    double* Gw = Gw_.as<double>();
     double* th = theta_.as<double>();
-    const double one = 1.0, zero = 0.0, mone = -1.0;
-
-    // host copy of the diagonal for start vectors
+    const double one = 1.0, zero = 0.0;
     std::vector<double> Dh(n);
     HIPCHK(hipMemcpyAsync(Dh.data(), D_.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-
-    // SVQB orthonormalisation of V[:, m0:m0+k] after Gram-Schmidt against V[:, :m0]
-    auto ortho = [&](int m0, int k) -> int {
-        for (int pass = 0; pass < 2; pass++) {
-            if (m0 > 0) {
-                // S (m0 x k) = Vc[0:m0] * Tc^T ; Tc -= S^T Vc
-                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, m0, k, (int)n, &one,
-                                      V, ld, V + m0, ld, &zero, S, m0));
-                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, k, (int)n, m0,
-                                      &mone, S, m0, V, ld, &one, V + m0, ld));
-            }
-            // Gram matrix of the block
-            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, k, k, (int)n, &one,
-                                  V + m0, ld, V + m0, ld, &zero, G, k));
-            std::vector<double> g((size_t)k * k);
-            HIPCHK(hipMemcpyAsync(g.data(), G, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-            HIPCHK(hipStreamSynchronize(stream_));
-            std::vector<double> dsc(k);
-            for (int i = 0; i < k; i++) dsc[i] = g[(size_t)i * k + i] > 0 ? 1.0 / std::sqrt(g[(size_t)i * k + i]) : 0.0;
-            for (int i = 0; i < k; i++)
-                for (int j = 0; j < k; j++) g[(size_t)j * k + i] *= dsc[i] * dsc[j];
-            HIPCHK(hipMemcpyAsync(G, g.data(), g.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
-            BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, k, G, k, th,
-                                     work_.as<double>(), info_.as<rocblas_int>()));
-            std::vector<double> lam(k), U((size_t)k * k);
-            HIPCHK(hipMemcpyAsync(lam.data(), th, k * sizeof(double), hipMemcpyDeviceToHost, stream_));
-            HIPCHK(hipMemcpyAsync(U.data(), G, U.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-            HIPCHK(hipStreamSynchronize(stream_));
-            const double lmax = lam[k - 1];
-            std::vector<int> keep;
-            for (int c = k - 1; c >= 0; c--)
-                if (lam[c] > 1e-10 * std::max(1.0, lmax)) keep.push_back(c);
-            const int kk = (int)keep.size();
-            if (kk == 0) return 0;
-            // Z (k x kk) col-major = diag(dsc) U[:, keep] / sqrt(lam)
-            std::vector<double> Z((size_t)k * kk);
-            for (int c = 0; c < kk; c++) {
-                const int src = keep[c];
-                const double s = 1.0 / std::sqrt(lam[src]);
-                for (int i = 0; i < k; i++) Z[(size_t)c * k + i] = dsc[i] * U[(size_t)src * k + i] * s;
-            }
-            HIPCHK(hipMemcpyAsync(S, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
-            // Tt (kk x n, ld kk) = Z^T Tc ; then copy back into V columns m0..
-            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, kk, (int)n, k, &one, S, k,
-                                  V + m0, ld, &zero, Tt, kk));
-            k_copy2d<<<grid_for(n * kk, 256), 256, 0, stream_>>>(Tt, kk, V + m0, ld, n, kk);
-            HIPCHK(hipGetLastError());
-            k = kk;
-        }
-        return k;
-    };
-
-    // ---- start block
     int k0 = 0;
-    std::vector<double> start((size_t)n * b, 0.0);
-    std::vector<char> used(n, 0);
-    if (xcols_ > 0 && xrows_ > 0 && xrows_ <= n) {
-        k0 = std::min(xcols_, b);
-        std::vector<double> xo((size_t)xrows_ * xcols_);
-        HIPCHK(hipMemcpyAsync(xo.data(), X_.p, xo.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-        HIPCHK(hipStreamSynchronize(stream_));
-        for (int64_t i = 0; i < xrows_; i++)
-            for (int c = 0; c < k0; c++) start[i * b + c] = xo[i * xcols_ + c];
-    }
+    if (xcols_ > 0 && xrows_ > 0 && xrows_ <= n) k0 = std::min(xcols_, b);
     {
-        // unit vectors on the lowest diagonal entries, new states first
         std::vector<int64_t> idx(n);
         std::iota(idx.begin(), idx.end(), 0);
         const int64_t first = (k0 > 0) ? xrows_ : 0;
-        auto cmp = [&](int64_t a, int64_t c) { return Dh[a] < Dh[c] || (Dh[a] == Dh[c] && a < c); };
         std::vector<int64_t> cand(idx.begin() + first, idx.end());
         const int need = b - k0;
         if ((int64_t)cand.size() < need) cand = idx;
+        auto cmp = [&](int64_t a, int64_t c) { return Dh[a] < Dh[c] || (Dh[a] == Dh[c] && a < c); };
         std::partial_sort(cand.begin(), cand.begin() + std::min<int64_t>(need, cand.size()), cand.end(), cmp);
-        for (int c = 0; c < need && c < (int)cand.size(); c++) start[cand[c] * b + k0 + c] = 1.0;
+        cand.resize(std::max(need, 1));
+        DevBuf di;
+        di.ensure(cand.size() * sizeof(int64_t));
+        HIPCHK(hipMemcpyAsync(di.p, cand.data(), cand.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream_));
+        dim3 g(grid_for(n, 256), b);
+        k_start_block<<<g, 256, 0, stream_>>>(n, b, k0, k0 ? X_.as<double>() : nullptr, xrows_, xcols_,
+                                             di.as<int64_t>(), V, ld);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream_));
     }
-    HIPCHK(hipMemcpyAsync(Tt, start.data(), start.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
-    k_copy2d<<<grid_for(n * b, 256), 256, 0, stream_>>>(Tt, b, V, ld, n, b);
-    int m = ortho(0, b);
+    int m = ortho_cm(V, ld, 0, b);
     if (m == 0) throw std::runtime_error("Davidson: empty start block");
-    const bool mixed = form_ == kMixed;
-    bool low = mixed;
-    spmm(V, ld, W, ld, m, low);
-    st.spmm_calls++;
-    st.spmm_vectors += m;
+    bool low = form_ == kMixed;
+    apply_cm(V, W, ld, m, low, st);
+    BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, m, m, ni, &one, V, ld, W, ld,
+                          &zero, G, mmax));
     const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
-    const int rpb = 256;
+    const int rpb = 8192;
     const int nblk = (int)((n + rpb - 1) / rpb);
     part_.ensure((size_t)nblk * b * sizeof(double));
     std::vector<double> thh(mmax), rn(b);
@@ -1301,21 +1642,18 @@ EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E
     int stall = 0;
     for (int it = 0; it < maxit; it++) {
         st.iterations = it + 1;
-        // Rayleigh-Ritz
-        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_transpose, m, m, (int)n, &one, V, ld, W,
-                              ld, &zero, G, m));
-        dim3 sg(grid_for(m, 64), m);
-        k_symmetrize<<<sg, 64, 0, stream_>>>(G, m, m);
-        BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, m, G, m, th, work_.as<double>(),
+        HIPCHK(hipMemcpy2DAsync(Gw, m * sizeof(double), G, mmax * sizeof(double), m * sizeof(double), m,
+                                hipMemcpyDeviceToDevice, stream_));
+        BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, m, Gw, m, th, work_.as<double>(),
                                  info_.as<rocblas_int>()));
         const int nb = std::min(b, m);
-        // X = V Y[:, :nb], HX = W Y[:, :nb]  (row-major n x nb, ld b)
-        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, nb, (int)n, m, &one, G, m, V,
-                              ld, &zero, X, b));
-        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, nb, (int)n, m, &one, G, m, W,
-                              ld, &zero, HX, b));
-        k_resid_partial<<<nblk, 256, 0, stream_>>>(n, nb, HX, X, b, th, rpb, part_.as<double>());
-        k_reduce_cols<<<grid_for(nb, 64), 64, 0, stream_>>>(part_.as<double>(), nblk, nb, work_.as<double>());
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, nb, m, &one, V, ld, Gw, m,
+                              &zero, X, ld));
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, nb, m, &one, W, ld, Gw, m,
+                              &zero, HX, ld));
+        dim3 rg(nblk, nb);
+        k_resid_cm<<<rg, 256, 0, stream_>>>(n, HX, X, ld, th, rpb, part_.as<double>());
+        k_sum_parts<<<grid_for(nb, 64), 64, 0, stream_>>>(part_.as<double>(), nblk, nb, work_.as<double>());
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(thh.data(), th, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipMemcpyAsync(rn.data(), work_.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -1323,69 +1661,254 @@ EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E
         std::vector<int> J;
         double maxrel = 0;
         bool all = true;
-        for (int j = 0; j < nb; j++) {
+        for (int j = 0; j < std::min(nev, nb); j++) {
             const double r = std::sqrt(rn[j]);
-            const double lim = tol * std::max(eps23, std::abs(thh[j]));
-            if (j < nev) maxrel = std::max(maxrel, r / std::max(eps23, std::abs(thh[j])));
-            if (r >= lim) {
+            maxrel = std::max(maxrel, r / std::max(eps23, std::abs(thh[j])));
+            if (r >= tol * std::max(eps23, std::abs(thh[j]))) {
                 J.push_back(j);
-                if (j < nev) all = false;
+                all = false;
             }
         }
         st.max_resid = maxrel;
-        if (all && !low) break;
+        if (std::getenv("VHCI_DEBUG"))
+            std::fprintf(stderr, "[davidson] it %d m %d maxrel %.3e unconverged %zu %s\n", it, m, maxrel, J.size(),
+                         low ? "fp32" : "fp64");
+        if (all && !low) {
+            E.assign(thh.begin(), thh.begin() + nev);
+            X_.ensure((size_t)n * nev * sizeof(double));
+            dim3 tg(grid_for(n, 32), (nev + 31) / 32);
+            k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X, ld, n, nev, X_.as<double>(), nev);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream_));
+            xrows_ = n;
+            xcols_ = nev;
+            return;
+        }
         if (low) {
-            // FP32 phase ends once the FP32 SpMM no longer improves the
-            // residuals; the rest runs with the FP64 (hi+lo) product.
-            const bool floor_hit = maxrel < 2e-7 || (maxrel > 0.5 * prev_max && ++stall >= 2) || all;
+            const bool floor_hit = maxrel < 2e-7 || all || (maxrel > 0.5 * prev_max && ++stall >= 2);
+            prev_max = std::min(prev_max, maxrel);
             if (floor_hit) {
                 low = false;
-                k_copy2d<<<grid_for(n * nb, 256), 256, 0, stream_>>>(X, b, V, ld, n, nb);
-                spmm(V, ld, W, ld, nb, false);
-                st.spmm_calls++;
-                st.spmm_vectors += nb;
-                st.fp64_spmm_vectors += nb;
+                HIPCHK(hipMemcpyAsync(V, X, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+                apply_cm(V, W, ld, nb, false, st);
+                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, nb, nb, ni, &one, V,
+                                      ld, W, ld, &zero, G, mmax));
                 m = nb;
-                prev_max = 1e300;
-                stall = 0;
                 continue;
             }
         }
-        if (!low) {
-            if (maxrel > 0.9 * prev_max) stall++;
-            else stall = 0;
-        }
-        prev_max = std::min(prev_max, maxrel);
-        if (J.empty()) J.push_back(0);
         const int k = (int)J.size();
         if (m + k > mmax) {
-            k_copy2d<<<grid_for(n * nb, 256), 256, 0, stream_>>>(X, b, V, ld, n, nb);
-            k_copy2d<<<grid_for(n * nb, 256), 256, 0, stream_>>>(HX, b, W, ld, n, nb);
+            HIPCHK(hipMemcpyAsync(V, X, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            HIPCHK(hipMemcpyAsync(W, HX, (size_t)n * nb * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            dim3 dg(grid_for(nb, 64), nb);
+            k_diag_block<<<dg, 64, 0, stream_>>>(G, nb, mmax, th);
             m = nb;
         }
         HIPCHK(hipMemcpyAsync(Jd.p, J.data(), k * sizeof(int), hipMemcpyHostToDevice, stream_));
-        k_precond<<<grid_for(n * k, 256), 256, 0, stream_>>>(n, k, Jd.as<int>(), HX, X, b, th, D_.as<double>(), V + m,
-                                                           ld);
+        dim3 pg(std::min<unsigned>(grid_for(n, 256), 1024), k);
+        k_precond_cm<<<pg, 256, 0, stream_>>>(n, k, Jd.as<int>(), HX, X, ld, th, D_.as<double>(), V + m * ld, ld);
         HIPCHK(hipGetLastError());
-        const int kept = ortho(m, k);
-        if (kept == 0) {
-            if (all) break;
-            throw std::runtime_error("Error: Eigenvalues did not converge.");
-        }
-        spmm(V + m, ld, W + m, ld, kept, low);
-        st.spmm_calls++;
-        st.spmm_vectors += kept;
-        if (!low) st.fp64_spmm_vectors += kept;
+        const int kept = ortho_cm(V, ld, m, k);
+        if (kept == 0) throw std::runtime_error("Error: Eigenvalues did not converge.");
+        apply_cm(V + m * ld, W + m * ld, ld, kept, low, st);
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, m + kept, kept, ni, &one, V,
+                              ld, W + m * ld, ld, &zero, G + (int64_t)m * mmax, mmax));
         m += kept;
-        if (it + 1 == maxit) throw std::runtime_error("Error: Eigenvalues did not converge.");
     }
-    E.assign(thh.begin(), thh.begin() + nev);
-    X_.ensure((size_t)n * nev * sizeof(double));
-    k_copy2d<<<grid_for(n * nev, 256), 256, 0, stream_>>>(X, b, X_.as<double>(), nev, n, nev);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipStreamSynchronize(stream_));
-    xrows_ = n;
-    xcols_ = nev;
+    throw std::runtime_error("Error: Eigenvalues did not converge.");
+}
+
+// Block Krylov-Schur (thick-restart block Lanczos with full
+// re-orthogonalisation).  V holds m + bs column-major basis vectors, W = H V
+// for the first m; after each cycle the Rayleigh-Ritz pairs of G = V^T W are
+// tested (||H x - theta x|| < tol max(eps^(2/3), |theta|) for the nev
+// lowest), and the space restarts from the keep lowest Ritz vectors plus the
+// last residual block, which is orthogonal to all of them.  The start block
+// mixes the resident Ritz vectors of the previous basis (zero-padded) with a
+// small random component.  Mixed form: the cycles run with the FP32 product
+// until the residuals stop improving; then W of the kept Ritz vectors is
+// recomputed in FP64 and the cycles continue in FP64.
+void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st) {
+    const int64_t n = n_;
+    const int ni = (int)n;
+    int bs = 4;
+    if (const char* e = std::getenv("VHCI_KS_BLOCK")) bs = std::max(1, std::min(32, std::atoi(e)));
+    int m = ((2 * nev + bs + bs - 1) / bs) * bs;
+    if (const char* e = std::getenv("VHCI_KS_DIM")) m = std::max(m, (std::atoi(e) / bs) * bs);
+    while (m + bs > n) m -= bs;
+    if (m < nev + 2 * bs) throw std::runtime_error("Krylov-Schur: basis too small for the requested states");
+    const int keep = m - std::max(1, (m - nev) / (2 * bs)) * bs;
+    const int64_t ld = n;
+    V_.ensure((size_t)n * (m + bs) * sizeof(double));
+    W_.ensure((size_t)n * m * sizeof(double));
+    Xw_.ensure((size_t)n * keep * sizeof(double));
+    HX_.ensure((size_t)n * keep * sizeof(double));
+    G_.ensure((size_t)m * m * sizeof(double));
+    Gw_.ensure((size_t)m * m * sizeof(double));
+    theta_.ensure((size_t)m * sizeof(double));
+    work_.ensure((size_t)std::max(m, 2 * bs) * 2 * sizeof(double));
+    info_.ensure(sizeof(int) * 4);
+    double* V = V_.as<double>();
+    double* W = W_.as<double>();
+    double* X = Xw_.as<double>();
+    double* HX = HX_.as<double>();
+    double* G = G_.as<double>();
+    double* Gw = Gw_.as<double>();
+    double* th = theta_.as<double>();
+    const double one = 1.0, zero = 0.0;
+    // start block
+    uint64_t seed = 0x6B72796C6F76ull + (uint64_t)n;
+    random_cm(V, ld, bs, seed);
+    if (xcols_ > 0 && xrows_ > 0 && xrows_ <= n) {
+        // V[:xrows, :bs] = 1e3 * Xprev R (R: xcols x bs, fixed pseudo-random) + the random block
+        std::vector<double> R((size_t)xcols_ * bs);
+        uint64_t z = seed;
+        for (auto& r : R) {
+            z += 0x9E3779B97F4A7C15ull;
+            uint64_t y = z;
+            y = (y ^ (y >> 30)) * 0xBF58476D1CE4E5B9ull;
+            y = (y ^ (y >> 27)) * 0x94D049BB133111EBull;
+            y ^= y >> 31;
+            r = (double)(y >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+        }
+        DevBuf Rd;
+        Rd.ensure(R.size() * sizeof(double));
+        HIPCHK(hipMemcpyAsync(Rd.p, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        const double big = 1e3;
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, (int)xrows_, bs, xcols_,
+                              &big, X_.as<double>(), xcols_, Rd.as<double>(), xcols_, &one, V, ld));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    int got = ortho_cm(V, ld, 0, bs);
+    if (got < bs) throw std::runtime_error("Krylov-Schur: degenerate start block");
+    bool low = form_ == kMixed;
+    const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
+    const int rpb = 8192;
+    const int nblk = (int)((n + rpb - 1) / rpb);
+    part_.ensure((size_t)nblk * keep * sizeof(double));
+    std::vector<double> thh(m), rn(nev);
+    int k = 0;
+    double prev_max = 1e300;
+    int stall = 0;
+    for (int it = 0; it < maxit; it++) {
+        st.iterations = it + 1;
+        for (int j = k; j < m; j += bs) {
+            apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st);
+            HIPCHK(hipMemcpyAsync(V + (int64_t)(j + bs) * ld, W + (int64_t)j * ld, (size_t)n * bs * sizeof(double),
+                                  hipMemcpyDeviceToDevice, stream_));
+            int kept = ortho_cm(V, ld, j + bs, bs);
+            for (int tries = 0; kept < bs && tries < 4; tries++) {
+                // invariant subspace reached for some directions: refill
+                random_cm(V + (int64_t)(j + bs + kept) * ld, ld, bs - kept, seed + 977 * (it * m + j + tries + 1));
+                int more = ortho_cm(V, ld, j + bs + kept, bs - kept);
+                kept += more;
+            }
+            if (kept < bs) throw std::runtime_error("Krylov-Schur: could not extend the basis");
+        }
+        // G[:, k:m] = V[:, :m]^T W[:, k:m] (upper triangle of the new columns)
+        tsm_tn(stream_, tsm_part_, n, m, m - k, V, ld, W + (int64_t)k * ld, ld, G + (int64_t)k * m, m);
+        HIPCHK(hipMemcpyAsync(Gw, G, (size_t)m * m * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, m, Gw, m, th, work_.as<double>(),
+                                 info_.as<rocblas_int>()));
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, V, ld, Gw, m,
+                              &zero, X, ld));
+        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, W, ld, Gw, m,
+                              &zero, HX, ld));
+        dim3 rg(nblk, nev);
+        k_resid_cm<<<rg, 256, 0, stream_>>>(n, HX, X, ld, th, rpb, part_.as<double>());
+        k_sum_parts<<<grid_for(nev, 64), 64, 0, stream_>>>(part_.as<double>(), nblk, nev, work_.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(thh.data(), th, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(rn.data(), work_.p, nev * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        double maxrel = 0;
+        int unconv = 0;
+        for (int j = 0; j < nev; j++) {
+            const double r = std::sqrt(rn[j]);
+            maxrel = std::max(maxrel, r / std::max(eps23, std::abs(thh[j])));
+            if (r >= tol * std::max(eps23, std::abs(thh[j]))) unconv++;
+        }
+        st.max_resid = maxrel;
+        if (std::getenv("VHCI_DEBUG"))
+            std::fprintf(stderr, "[krylov-schur] cycle %d m %d keep %d bs %d maxrel %.3e unconverged %d %s\n", it, m,
+                         keep, bs, maxrel, unconv, low ? "fp32" : "fp64");
+        if (unconv == 0 && !low) {
+            E.assign(thh.begin(), thh.begin() + nev);
+            X_.ensure((size_t)n * nev * sizeof(double));
+            dim3 tg(grid_for(n, 32), (nev + 31) / 32);
+            k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X, ld, n, nev, X_.as<double>(), nev);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream_));
+            xrows_ = n;
+            xcols_ = nev;
+            return;
+        }
+        // thick restart
+        HIPCHK(hipMemcpyAsync(V, X, (size_t)n * keep * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        HIPCHK(hipMemcpyAsync(V + (int64_t)keep * ld, V + (int64_t)m * ld, (size_t)n * bs * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream_));
+        bool switched = false;
+        if (low) {
+            const bool floor_hit = maxrel < 1e-7 || unconv == 0 || (maxrel > 0.5 * prev_max && ++stall >= 2);
+            prev_max = std::min(prev_max, maxrel);
+            if (floor_hit) {
+                low = false;
+                switched = true;
+            }
+        }
+        if (switched) {
+            // The FP32 cycles' Krylov relation does not hold for the FP64
+            // product, so the FP64 cycles start afresh from a block of random
+            // combinations of the kept Ritz vectors.
+            std::vector<double> R((size_t)keep * bs);
+            uint64_t z = seed ^ 0x5A5A5A5Aull;
+            for (auto& r : R) {
+                z += 0x9E3779B97F4A7C15ull;
+                uint64_t y = z;
+                y = (y ^ (y >> 30)) * 0xBF58476D1CE4E5B9ull;
+                y = (y ^ (y >> 27)) * 0x94D049BB133111EBull;
+                y ^= y >> 31;
+                r = (double)(y >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+            }
+            DevBuf Rd;
+            Rd.ensure(R.size() * sizeof(double));
+            HIPCHK(hipMemcpyAsync(Rd.p, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, bs, keep, &one, X, ld,
+                                  Rd.as<double>(), keep, &zero, V, ld));
+            HIPCHK(hipStreamSynchronize(stream_));
+            if (ortho_cm(V, ld, 0, bs) < bs) throw std::runtime_error("Krylov-Schur: degenerate restart block");
+            k = 0;
+            continue;
+        } else {
+            HIPCHK(hipMemcpyAsync(W, HX, (size_t)n * keep * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            dim3 dg(grid_for(keep, 64), keep);
+            k_diag_block<<<dg, 64, 0, stream_>>>(G, keep, m, th);
+        }
+        HIPCHK(hipGetLastError());
+        k = keep;
+    }
+    throw std::runtime_error("Error: Eigenvalues did not converge.");
+}
+
+EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E) {
+    HIPCHK(hipSetDevice(device_));
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    auto t0 = clk::now();
+    EigStats st;
+    const int64_t n = n_;
+    if (nev < 1) throw std::runtime_error("nev must be positive");
+    if (nev > n) nev = (int)n;
+    if (nev > 256) throw std::runtime_error("at most 256 eigenpairs are supported");
+    if (n <= std::max<int64_t>(4 * (nev + std::max(4, nev / 5)), 1024)) {
+        dense_solve(nev, E, st);
+    } else {
+        const char* alg = std::getenv("VHCI_EIGSOLVER");
+        if (alg && std::string(alg) == "davidson")
+            davidson(nev, tol, maxit, E, st);
+        else
+            krylov_schur(nev, tol, maxit, E, st);
+    }
     st.ms = ms_since(t0);
     return st;
 }

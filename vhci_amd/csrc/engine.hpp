@@ -113,6 +113,12 @@ class Engine {
     void rebuild_hash(int64_t cap_hint);
     void spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low);
     void dense_solve(int nev, std::vector<double>& E, EigStats& st);
+    // column-major block helpers of the iterative eigensolvers
+    void apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st);
+    int ortho_cm(double* V, int64_t ld, int m0, int k);
+    void random_cm(double* V, int64_t ld, int k, uint64_t seed);
+    void davidson(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st);
+    void krylov_schur(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st);
     template <int NW>
     void hb_select_t(double eps, HbStats& st);
     template <int NW>
@@ -146,6 +152,7 @@ class Engine {
     int xcols_ = 0;
     // eigensolver work
     DevBuf V_, W_, HX_, Xw_, T_b_, G_, S_, theta_, work_, part_, X32_, info_;
+    DevBuf stage_in_, stage_in32_, stage_out_, gram_, otmp_, Gw_, tsm_part_;
     // heat-bath work
     DevBuf cmax_, candcnt_, candoff_, nslots_, pool_keys_, pool_rank_, counters_, col_rank_, col_idx_,
         srt_rank_, srt_idx_, newkeys_, scan_tmp_;
