@@ -78,10 +78,10 @@ def cpu_baseline(ctx, r, problem_text, threads):
         os.unlink(path)
     q = ctx.basis()
     ns2 = r["ns2"]
-    nA = min(ns2, 600)
+    nA = min(ns2, 12000)
     qA = ctx.new_states(ns2)[:nA] if nA else np.zeros((0, q.shape[1]), dtype=np.int32)
     n = q.shape[0]
-    rows = max(1, int(4e8 / n))
+    rows = max(1, int(6e9 / n))
     nopt = min(o.neig, n)
     rate_h, rate_p, raw = o.sample_rates(q, rows, qA, nopt)
     sizes = ctx.iteration_sizes()
@@ -145,7 +145,7 @@ def main():
         same = e64["checksum"] == e32["checksum"] and e64["nvar"] == e32["nvar"] and e64["ns2"] == e32["ns2"]
         out["mixed_vs_fp64"] = {"max_abs_err_cm": err, "max_abs_err_var_cm": err_var, "basis_identical": same,
                                 "speedup": forms_out["mixed"]["solves_per_s"] / forms_out["fp64"]["solves_per_s"]}
-        if not (err <= 0.01 and same):
+        if not (err <= 0.01 and same) or forms_out["mixed"]["solves_per_s"] < forms_out["fp64"]["solves_per_s"]:
             head = "fp64"
     lat, (ctx, r) = res[head]
     out["value"] = forms_out[head]["solves_per_s"]
@@ -156,15 +156,18 @@ def main():
     out["p50_latency_s"] = forms_out[head]["p50_s"]
     out["p99_latency_s"] = forms_out[head]["p99_s"]
 
-    # SpMV / SpMM on the final Hamiltonian (FP64 context): HIP events
+    # SpMV / block SpMM on the final Hamiltonian (FP64 context), HIP events
+    # on the context's stream.  Algorithmic bytes per launch:
+    # nnz*(8+4) + 8*(N+1) + 2*N*b*8 + 8*N (values, columns, row pointers,
+    # X read once and Y written once, diagonal).
     ctx64 = res["fp64"][1][0] if "fp64" in res else ctx
     ms1, b1 = ctx64.time_spmm(1, args.spmm_reps)
-    bs = 64
+    bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
     msb, bb = ctx64.time_spmm(bs, args.spmm_reps)
     out["spmv"] = {"ms": ms1, "bytes": b1, "GBs": b1 / ms1 / 1e6, "frac_of_hbm": b1 / ms1 / 1e6 / HBM_PEAK_GBS,
                    "n": ctx64.n}
-    out["roofline"] = {"kernel": "k_spmm64<64> (block SpMM, %d vectors, FP64 CSR)" % bs, "bound": "hbm",
-                       "achieved": bb / msb / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+    out["roofline"] = {"kernel": "k_spmm64<%d> (eigensolver block SpMM, %d vectors, FP64 CSR)" % (bs, bs),
+                       "bound": "hbm", "achieved": bb / msb / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": bb / msb / 1e6 / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": msb,
                        "algorithmic_bytes_per_launch": bb}
     if "mixed" in res:

@@ -1,0 +1,51 @@
+"""Full-size parity of the 15-mode heat-bath workload against the oracle's
+20-minute CPU run (tests/golden/c1_oracle.json): the basis size after every
+heat-bath iteration, the final basis (checksum over the quanta in order) and
+the 100 variational energies; the mixed form must select the identical
+basis and agree within 0.01 cm^-1."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN
+
+import vhci_amd
+from vhci_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ref():
+    with open(os.path.join(GOLDEN, "c1_oracle.json")) as f:
+        return json.load(f)
+
+
+@pytest.fixture(scope="module")
+def fp64_run():
+    p = vhci_amd.Problem(text=synth.config_input(1, pt2=False))
+    ctx = vhci_amd.Context(p, vhci_amd.FP64)
+    r = ctx.run()
+    r["sizes"] = ctx.iteration_sizes()
+    ctx.close()
+    return r
+
+
+def test_c1_fp64_matches_oracle(ref, fp64_run):
+    print("c1 fp64 sizes", fp64_run["sizes"], flush=True)
+    assert fp64_run["sizes"] == ref["sizes"]
+    assert fp64_run["checksum"] == int(ref["checksum"])
+    err = np.max(np.abs(fp64_run["E_var"] - np.array(ref["E_var"])))
+    print("max |E - oracle| = %.3e cm^-1" % err, flush=True)
+    assert err < 1e-6
+
+
+def test_c1_mixed_matches_fp64(ref, fp64_run):
+    p = vhci_amd.Problem(text=synth.config_input(1, pt2=False))
+    ctx = vhci_amd.Context(p, vhci_amd.MIXED)
+    r = ctx.run()
+    assert ctx.iteration_sizes() == ref["sizes"]
+    assert r["checksum"] == fp64_run["checksum"]
+    assert np.max(np.abs(r["E_var"] - fp64_run["E_var"])) < 1e-2

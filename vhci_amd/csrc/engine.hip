@@ -7,6 +7,7 @@
 // selection thresholds) is written as separate multiplies and adds; fused
 // multiply-adds are requested explicitly with fma() where rounding is free.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_segmented_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 #include <rocsolver/rocsolver.h>
 
@@ -333,84 +334,168 @@ __global__ void k_diag(int64_t n, int M, const uint4* __restrict__ keys, const d
 
 // ------------------------------------------------------------------------
 // SpMM: Y = (D + H_off) X, X/Y row-major; one wave per row, lanes split into
-// 64/B nonzero groups x B columns.
+// 64/B nonzero groups x B columns.  Columns are sorted within each row and
+// pp holds per-row offsets of column panels: with pp, one launch covers one
+// panel of columns (its slice of X stays in L2) and accumulates into Y
+// (the first panel also adds the diagonal term).
+// MODE 0: FP64 values; 1: FP32 values and FP32 X (FP32 accumulation, the
+// diagonal term in FP64); 2: FP64 value = hi + lo.
 // ------------------------------------------------------------------------
-template <int B>
-__global__ __launch_bounds__(256) void k_spmm64(int64_t n, const int64_t* __restrict__ rowptr,
-                                                const int32_t* __restrict__ col, const double* __restrict__ val,
-                                                const double* __restrict__ D, const double* __restrict__ X, int ldx,
-                                                double* __restrict__ Y, int ldy, int b) {
+template <int MODE>
+struct SpAcc {
+    using T = double;
+};
+template <>
+struct SpAcc<1> {
+    using T = float;
+};
+
+// value of entry k in FP64 (MODE 0, 2) or FP32 (MODE 1); the matrix arrays
+// are streamed once per product and loaded non-temporally so the gathered
+// vector stays in cache
+template <int MODE>
+__device__ __forceinline__ typename SpAcc<MODE>::T sp_val(const double* __restrict__ val,
+                                                          const float* __restrict__ hi,
+                                                          const float* __restrict__ lo, int64_t k) {
+    if (MODE == 0) return __builtin_nontemporal_load(val + k);
+    if (MODE == 1) return __builtin_nontemporal_load(hi + k);
+    return (double)__builtin_nontemporal_load(hi + k) + (double)__builtin_nontemporal_load(lo + k);
+}
+
+__device__ __forceinline__ float sp_fma(float a, float b, float c) { return fmaf(a, b, c); }
+__device__ __forceinline__ double sp_fma(double a, double b, double c) { return fma(a, b, c); }
+
+template <int MODE>
+__device__ __forceinline__ typename SpAcc<MODE>::T sp_x(const double* __restrict__ X,
+                                                        const float* __restrict__ X32, int64_t idx32, int64_t idx) {
+    if (MODE == 1) return X32[idx32];
+    return X[idx];
+}
+
+template <int B, int MODE>
+__global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ pp, int P1, int panel, int first,
+                                              const int32_t* __restrict__ col, const double* __restrict__ val,
+                                              const float* __restrict__ hi, const float* __restrict__ lo,
+                                              const double* __restrict__ D, const float* __restrict__ X32, int ldx32,
+                                              const double* __restrict__ X, int ldx, double* __restrict__ Y, int ldy,
+                                              int b) {
+    using AT = typename SpAcc<MODE>::T;
     constexpr int G = kWave / B;
     const int lane = threadIdx.x & 63;
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
     const int grp = lane / B, c = lane % B;
-    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    if (pp) {
+        const int32_t* q = pp + row * P1 + panel;
+        k1 = k0 + q[1];
+        k0 = k0 + q[0];
+    }
     const double d = D[row];
     for (int c0 = 0; c0 < b; c0 += B) {
         const int cc = c0 + c;
         const bool act = cc < b;
-        double acc = 0.0;
-        if (act)
-            for (int64_t k = k0 + grp; k < k1; k += G) acc = fma(val[k], X[(int64_t)col[k] * ldx + cc], acc);
+        AT a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+        if (act) {
+            int64_t k = k0 + grp;
+            for (; k + 3 * G < k1; k += 4 * G) {
+                const int64_t j0 = __builtin_nontemporal_load(col + k);
+                const int64_t j1 = __builtin_nontemporal_load(col + k + G);
+                const int64_t j2 = __builtin_nontemporal_load(col + k + 2 * G);
+                const int64_t j3 = __builtin_nontemporal_load(col + k + 3 * G);
+                const AT v0 = sp_val<MODE>(val, hi, lo, k), v1 = sp_val<MODE>(val, hi, lo, k + G);
+                const AT v2 = sp_val<MODE>(val, hi, lo, k + 2 * G), v3 = sp_val<MODE>(val, hi, lo, k + 3 * G);
+                a0 = sp_fma(v0, sp_x<MODE>(X, X32, j0 * ldx32 + cc, j0 * ldx + cc), a0);
+                a1 = sp_fma(v1, sp_x<MODE>(X, X32, j1 * ldx32 + cc, j1 * ldx + cc), a1);
+                a2 = sp_fma(v2, sp_x<MODE>(X, X32, j2 * ldx32 + cc, j2 * ldx + cc), a2);
+                a3 = sp_fma(v3, sp_x<MODE>(X, X32, j3 * ldx32 + cc, j3 * ldx + cc), a3);
+            }
+            for (; k < k1; k += G) {
+                const int64_t j0 = __builtin_nontemporal_load(col + k);
+                a0 = sp_fma(sp_val<MODE>(val, hi, lo, k), sp_x<MODE>(X, X32, j0 * ldx32 + cc, j0 * ldx + cc), a0);
+            }
+        }
+        AT acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
         for (int off = B; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
-        if (grp == 0 && act) Y[row * ldy + cc] = fma(d, X[row * ldx + cc], acc);
+        if (grp == 0 && act) {
+            const double base = first ? d * X[row * ldx + cc] : Y[row * ldy + cc];
+            Y[row * ldy + cc] = base + (double)acc;
+        }
     }
 }
 
-// mixed form: FP32 values and FP32 X for the off-diagonal sum (FP32
-// accumulation), diagonal term in FP64
-template <int B>
-__global__ __launch_bounds__(256) void k_spmm32(int64_t n, const int64_t* __restrict__ rowptr,
-                                                const int32_t* __restrict__ col, const float* __restrict__ hi,
-                                                const double* __restrict__ D, const float* __restrict__ X32, int ldx32,
-                                                const double* __restrict__ X, int ldx, double* __restrict__ Y, int ldy,
-                                                int b) {
-    constexpr int G = kWave / B;
-    const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
-    const int grp = lane / B, c = lane % B;
-    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    const double d = D[row];
-    for (int c0 = 0; c0 < b; c0 += B) {
-        const int cc = c0 + c;
-        const bool act = cc < b;
-        float acc = 0.0f;
-        if (act)
-            for (int64_t k = k0 + grp; k < k1; k += G) acc = fmaf(hi[k], X32[(int64_t)col[k] * ldx32 + cc], acc);
+// single vector: 16 lanes per row, 16 rows per 256-thread block
+template <int MODE>
+__global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t* __restrict__ rowptr,
+                                              const int32_t* __restrict__ col, const double* __restrict__ val,
+                                              const float* __restrict__ hi, const float* __restrict__ lo,
+                                              const double* __restrict__ D, const float* __restrict__ x32,
+                                              const double* __restrict__ x, double* __restrict__ y) {
+    using AT = typename SpAcc<MODE>::T;
+    const int sub = threadIdx.x & 15;
+    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const bool live = row < n;
+    AT a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    if (live) {
+        const int64_t k1 = rowptr[row + 1];
+        int64_t k = rowptr[row] + sub;
+        for (; k + 48 < k1; k += 64) {
+            const int64_t j0 = __builtin_nontemporal_load(col + k);
+            const int64_t j1 = __builtin_nontemporal_load(col + k + 16);
+            const int64_t j2 = __builtin_nontemporal_load(col + k + 32);
+            const int64_t j3 = __builtin_nontemporal_load(col + k + 48);
+            a0 = sp_fma(sp_val<MODE>(val, hi, lo, k), sp_x<MODE>(x, x32, j0, j0), a0);
+            a1 = sp_fma(sp_val<MODE>(val, hi, lo, k + 16), sp_x<MODE>(x, x32, j1, j1), a1);
+            a2 = sp_fma(sp_val<MODE>(val, hi, lo, k + 32), sp_x<MODE>(x, x32, j2, j2), a2);
+            a3 = sp_fma(sp_val<MODE>(val, hi, lo, k + 48), sp_x<MODE>(x, x32, j3, j3), a3);
+        }
+        for (; k < k1; k += 16) {
+            const int64_t j0 = __builtin_nontemporal_load(col + k);
+            a0 = sp_fma(sp_val<MODE>(val, hi, lo, k), sp_x<MODE>(x, x32, j0, j0), a0);
+        }
+    }
+    AT acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
-        for (int off = B; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
-        if (grp == 0 && act) Y[row * ldy + cc] = fma(d, X[row * ldx + cc], (double)acc);
+    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    if (live && sub == 0) y[row] = D[row] * x[row] + (double)acc;
+}
+
+// pack / unpack the mixed form's hi, lo pair into one 64-bit word (sort payload)
+__global__ void k_pack_hilo(int64_t nnz, const float* __restrict__ hi, const float* __restrict__ lo,
+                            uint64_t* __restrict__ out) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x)
+        out[k] = (uint64_t)__float_as_uint(hi[k]) | ((uint64_t)__float_as_uint(lo[k]) << 32);
+}
+__global__ void k_unpack_hilo(int64_t nnz, const uint64_t* __restrict__ in, float* __restrict__ hi,
+                              float* __restrict__ lo) {
+    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < nnz; k += (int64_t)gridDim.x * blockDim.x) {
+        hi[k] = __uint_as_float((uint32_t)in[k]);
+        lo[k] = __uint_as_float((uint32_t)(in[k] >> 32));
     }
 }
 
-// mixed form, refinement: value = hi + lo in FP64
-template <int B>
-__global__ __launch_bounds__(256) void k_spmm_hilo(int64_t n, const int64_t* __restrict__ rowptr,
-                                                   const int32_t* __restrict__ col, const float* __restrict__ hi,
-                                                   const float* __restrict__ lo, const double* __restrict__ D,
-                                                   const double* __restrict__ X, int ldx, double* __restrict__ Y,
-                                                   int ldy, int b) {
-    constexpr int G = kWave / B;
-    const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (row >= n) return;
-    const int grp = lane / B, c = lane % B;
-    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    const double d = D[row];
-    for (int c0 = 0; c0 < b; c0 += B) {
-        const int cc = c0 + c;
-        const bool act = cc < b;
-        double acc = 0.0;
-        if (act)
-            for (int64_t k = k0 + grp; k < k1; k += G)
-                acc = fma((double)hi[k] + (double)lo[k], X[(int64_t)col[k] * ldx + cc], acc);
-#pragma unroll
-        for (int off = B; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
-        if (grp == 0 && act) Y[row * ldy + cc] = fma(d, X[row * ldx + cc], acc);
+// per-row offsets of the column panels [p*w, (p+1)*w) in a sorted row
+__global__ void k_panel_ptr(int64_t n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col, int P,
+                            int w, int32_t* __restrict__ pp) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t k0 = rowptr[i], k1 = rowptr[i + 1];
+    pp[i * (P + 1)] = 0;
+    for (int p = 1; p < P; p++) {
+        const int32_t bound = p * w;
+        int64_t lo = k0, hi = k1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (col[mid] < bound)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        pp[i * (P + 1) + p] = (int32_t)(lo - k0);
     }
+    pp[i * (P + 1) + P] = (int32_t)(k1 - k0);
 }
 
 // ------------------------------------------------------------------------
@@ -624,26 +709,41 @@ __global__ __launch_bounds__(256) void k_tsm_tn(int64_t n, int p, int q, const d
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
     const int64_t r1 = min(n, r0 + rows_per_chunk);
     const int aa = t & 63, cg = t >> 6;  // outputs (aa, 4*cg .. 4*cg+3)
+    // each thread stages 8 A values and 2 B values per 32-row tile; the next
+    // tile is fetched into registers while the current one is consumed
+    const int ai = t & 31, aj = t >> 5;  // A: rows ai, columns aj + 8 u
+    const int bi = t & 31, bj = t >> 5;  // B: rows bi, columns bj, bj + 8
+    double ra[8], rb[2];
+    auto fetch = [&](int64_t rt) {
+        const int64_t r = rt + ai;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = aj + 8 * u;
+            ra[u] = (r < r1 && a0 + j < p) ? A[r + (int64_t)(a0 + j) * lda] : 0.0;
+        }
+        const int64_t rbb = rt + bi;
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int j = bj + 8 * u;
+            rb[u] = (rbb < r1 && c0 + j < q) ? B[rbb + (int64_t)(c0 + j) * ldb] : 0.0;
+        }
+    };
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    if (r0 < r1) fetch(r0);
     for (int64_t rt = r0; rt < r1; rt += 32) {
-        for (int idx = t; idx < 32 * 64; idx += 256) {
-            const int i = idx & 31, j = idx >> 5;
-            const int64_t r = rt + i;
-            As[i][j] = (r < r1 && a0 + j < p) ? A[r + (int64_t)(a0 + j) * lda] : 0.0;
-        }
-        for (int idx = t; idx < 32 * 16; idx += 256) {
-            const int i = idx & 31, j = idx >> 5;
-            const int64_t r = rt + i;
-            Bs[i][j] = (r < r1 && c0 + j < q) ? B[r + (int64_t)(c0 + j) * ldb] : 0.0;
-        }
         __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; u++) As[ai][aj + 8 * u] = ra[u];
+#pragma unroll
+        for (int u = 0; u < 2; u++) Bs[bi][bj + 8 * u] = rb[u];
+        __syncthreads();
+        if (rt + 32 < r1) fetch(rt + 32);
 #pragma unroll 8
         for (int i = 0; i < 32; i++) {
             const double x = As[i][aa];
 #pragma unroll
             for (int u = 0; u < 4; u++) acc[u] = fma(x, Bs[i][4 * cg + u], acc[u]);
         }
-        __syncthreads();
     }
     if (a0 + aa < p)
 #pragma unroll
@@ -653,15 +753,29 @@ __global__ __launch_bounds__(256) void k_tsm_tn(int64_t n, int p, int q, const d
         }
 }
 
-// C[a + c*ldc] = sum over chunks (fixed order)
-__global__ void k_tsm_reduce(const double* __restrict__ part, int nchunk, int p, int q, double* __restrict__ C,
-                             int ldc) {
-    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (t >= (int64_t)p * q) return;
-    const int c = (int)(t / p), a = (int)(t - (int64_t)c * p);
-    double s = 0.0;
-    for (int k = 0; k < nchunk; k++) s += part[((int64_t)k * q + c) * p + a];
-    C[a + (int64_t)c * ldc] = s;
+// C[a + c*ldc] = sum over chunks: 64 outputs per block, the chunks split
+// over 4 waves, combined in a fixed order (deterministic)
+__global__ __launch_bounds__(256) void k_tsm_reduce(const double* __restrict__ part, int nchunk, int p, int q,
+                                                    double* __restrict__ C, int ldc) {
+    __shared__ double red[4][64];
+    const int o = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int64_t t = (int64_t)blockIdx.x * 64 + o;
+    const int64_t pq = (int64_t)p * q;
+    double s0 = 0.0, s1 = 0.0;
+    if (t < pq) {
+        int k = w;
+        for (; k + 4 < nchunk; k += 8) {
+            s0 += part[(int64_t)k * pq + t];
+            s1 += part[(int64_t)(k + 4) * pq + t];
+        }
+        if (k < nchunk) s0 += part[(int64_t)k * pq + t];
+    }
+    red[w][o] = s0 + s1;
+    __syncthreads();
+    if (w == 0 && t < pq) {
+        const int c = (int)(t / p), a = (int)(t - (int64_t)c * p);
+        C[a + (int64_t)c * ldc] = (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
+    }
 }
 
 // Y = beta Y + alpha A S: A n x p (lda), S p x q (lds, col-major) staged in
@@ -1034,6 +1148,227 @@ __global__ void k_pt2_reduce(const double* __restrict__ part, int nw, int nopt, 
     out[c] = s + cmp;
 }
 
+// ------------------------------------------------------------------------
+// PT2, push form: every variational state i probes its connection patterns
+// in the external-space hash; each hit emits (a, i, pattern) at offsets fixed
+// by a prefix sum over i and the lane order, so after a stable sort by a the
+// pairs of every external state come in ascending i.
+// ------------------------------------------------------------------------
+struct Pt2PushArgs {
+    int64_t n;
+    int M;
+    int P;
+    const uint4* vkeys;
+    const uint64_t* aslots;
+    uint64_t amask;
+    const uint4* apool;
+    const uint32_t* apos;
+    const DeltaEnt* deltas;
+    int64_t* cnt;
+    const int64_t* off;
+    uint32_t* pkey;
+    uint64_t* pval;
+};
+
+template <int NW, bool FILL>
+__global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
+    __shared__ uint8_t sq[64];
+    const int lane = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    uint32_t w[NW];
+    key_load<NW>(a.vkeys, row, w);
+    if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+    __syncthreads();
+    int64_t base = FILL ? a.off[row] : 0;
+    int64_t cnt = 0;
+    for (int p0 = 0; p0 < a.P; p0 += kWave) {
+        const int p = p0 + lane;
+        bool hit = false;
+        int64_t idx = -1;
+        if (p < a.P) {
+            const DeltaEnt e = a.deltas[p];
+            const int nz = (int)(e.meta >> 28);
+            uint32_t t[NW];
+#pragma unroll
+            for (int i = 0; i < NW; i++) t[i] = w[i];
+            bool ok = true;
+            for (int s = 0; s < nz; s++) {
+                const int nqv = (int)sq[e.mode[s]] + (int)e.delta[s];
+                ok = ok && nqv >= 0 && nqv <= 255;
+                key_add<NW>(t, e.mode[s], e.delta[s]);
+            }
+            if (ok) {
+                idx = hash_find<NW>(a.aslots, a.amask, a.apool, t);
+                hit = idx >= 0;
+            }
+        }
+        const uint64_t bal = __ballot(hit);
+        if (!FILL) {
+            cnt += __popcll(bal);
+        } else {
+            if (hit) {
+                const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
+                a.pkey[pos] = a.apos[idx];
+                a.pval[pos] = ((uint64_t)(uint32_t)row << 32) | (uint32_t)p;
+            }
+            base += __popcll(bal);
+        }
+    }
+    if (!FILL && lane == 0) a.cnt[row] = cnt;
+}
+
+// canonical position of every pool entry (inverse of the sort by rank)
+__global__ void k_pool_pos(const uint32_t* __restrict__ sorted_idx, int64_t u, uint32_t* __restrict__ pos) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < u) pos[sorted_idx[t]] = (uint32_t)t;
+}
+
+// first pair of every external state in the sorted pair list
+__global__ void k_seg_start(const uint32_t* __restrict__ skey, int64_t npairs, int64_t ns2, int64_t* __restrict__ seg) {
+    const int64_t a = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (a > ns2) return;
+    int64_t lo = 0, hi = npairs;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if ((int64_t)skey[mid] < a)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    seg[a] = lo;
+}
+
+struct Pt2NumArgs {
+    int64_t ns2;
+    int M;
+    int nq;
+    int nopt;
+    const uint4* akeys;
+    const int64_t* seg;
+    const uint64_t* sval;
+    const DeltaEnt* deltas;
+    const GroupEnt* groups;
+    const uint32_t* fclist;
+    const ElemFC* elem;
+    const double* ladder;
+    const uint32_t* dfc;
+    int ndfc;
+    const double* freq;
+    const double* C;
+    const float* C32;
+    int ldc;
+    const double* E;
+    double* part;
+};
+
+// one wave per external state: numerators sum_i H_ai C_in over its pairs in
+// ascending i (FP64: product then sum, as the reference's SumHaiCi), E_a,
+// then Kahan sums of num^2 / (E_n - E_a) per column
+template <int NW, bool MIXED>
+__global__ __launch_bounds__(64) void k_pt2_num(Pt2NumArgs a) {
+    __shared__ uint8_t sq[64];
+    __shared__ int32_t hidx[64];
+    __shared__ double hval[64];
+    __shared__ double sdiag[64];
+    const int lane = threadIdx.x;
+    constexpr int R = 4;
+    double ks[R], kc[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) ks[r] = kc[r] = 0.0;
+    for (int64_t ia = blockIdx.x; ia < a.ns2; ia += gridDim.x) {
+        uint32_t w[NW];
+        key_load<NW>(a.akeys, ia, w);
+        if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+        __syncthreads();
+        double num[R];
+        float num32[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            num[r] = 0.0;
+            num32[r] = 0.0f;
+        }
+        const int64_t s0 = a.seg[ia], s1 = a.seg[ia + 1];
+        for (int64_t h0 = s0; h0 < s1; h0 += kWave) {
+            const int64_t h = h0 + lane;
+            if (h < s1) {
+                const uint64_t v = a.sval[h];
+                DeltaEnt e = a.deltas[(uint32_t)v];
+                const int nz = (int)(e.meta >> 28);
+                for (int s = 0; s < nz; s++) e.delta[s] = (int8_t)(-e.delta[s]);  // i = a - delta
+                const GroupEnt ge = a.groups[e.meta & 0x03ffffffu];
+                hidx[lane] = (int32_t)(v >> 32);
+                hval[lane] = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, true);
+            }
+            __syncthreads();
+            const int lim = (int)min((int64_t)kWave, s1 - h0);
+            for (int l = 0; l < lim; l++) {
+                const int64_t i = hidx[l];
+                const double hv = hval[l];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int c = lane + 64 * r;
+                    if (c < a.nopt) {
+                        if (MIXED)
+                            num32[r] = fmaf((float)hv, a.C32[i * a.ldc + c], num32[r]);
+                        else
+                            num[r] = num[r] + hv * a.C[i * a.ldc + c];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        double ea = 0.;
+        for (int f0 = 0; f0 < a.ndfc || f0 == 0; f0 += kWave) {
+            const int f = f0 + lane;
+            double x = 0.0;
+            if (f < a.ndfc) {
+                const ElemFC fc = a.elem[a.dfc[f]];
+                x = fc.fc;
+                for (int t = 0; t < fc.d; t++) {
+                    const int q = sq[fc.mode[t]];
+                    x = x * a.ladder[((size_t)fc.pow[t] * a.nq + q) * kLadderW + fc.pow[t]];
+                }
+            }
+            sdiag[lane] = x;
+            __syncthreads();
+            if (lane == 0) {
+                if (f0 == 0)
+                    for (int m = 0; m < a.M; m++) {
+                        double ej = 0.5;
+                        ej = ej + (double)sq[m];
+                        ej = ej * a.freq[m];
+                        ea = ea + ej;
+                    }
+                const int lim = min(kWave, a.ndfc - f0);
+                for (int t = 0; t < lim; t++) ea = ea + sdiag[t];
+            }
+            __syncthreads();
+            if (a.ndfc == 0) break;
+        }
+        ea = __shfl(ea, 0);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int c = lane + 64 * r;
+            if (c < a.nopt) {
+                const double nm = MIXED ? (double)num32[r] : num[r];
+                const double term = (nm * nm) / (a.E[c] - ea);
+                const double y = term - kc[r];
+                const double t = ks[r] + y;
+                kc[r] = (t - ks[r]) - y;
+                ks[r] = t;
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int c = lane + 64 * r;
+        if (c < a.nopt) {
+            a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 0] = ks[r];
+            a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 1] = kc[r];
+        }
+    }
+}
+
 }  // namespace
 
 // ==========================================================================
@@ -1206,6 +1541,48 @@ void Engine::build_h_t(HStats& st) {
         k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     }
+    // columns ascending within each row; per-row offsets of 32k-column panels
+    if (nnz > 0) {
+        unsigned endbit = 1;
+        while (endbit < 31 && ((uint64_t)(n - 1) >> endbit) != 0) endbit++;
+        DevBuf ctmp, vin, vout;
+        ctmp.ensure(nnz * sizeof(int32_t));
+        vout.ensure(nnz * sizeof(uint64_t));
+        uint64_t* vals_in;
+        if (form_ == kFP64) {
+            vals_in = val_.as<uint64_t>();
+        } else {
+            vin.ensure(nnz * sizeof(uint64_t));
+            k_pack_hilo<<<grid_for(nnz, 256), 256, 0, stream_>>>(nnz, hi_.as<float>(), lo_.as<float>(),
+                                                                 vin.as<uint64_t>());
+            vals_in = vin.as<uint64_t>();
+        }
+        size_t tmp = 0;
+        const int64_t* rp = rowptr_.as<int64_t>();
+        HIPCHK(rocprim::segmented_radix_sort_pairs(nullptr, tmp, col_.as<int32_t>(), ctmp.as<int32_t>(), vals_in,
+                                                   vout.as<uint64_t>(), (size_t)nnz, (unsigned)n, rp, rp + 1, 0,
+                                                   endbit, stream_));
+        scan_tmp_.ensure(tmp);
+        HIPCHK(rocprim::segmented_radix_sort_pairs(scan_tmp_.p, tmp, col_.as<int32_t>(), ctmp.as<int32_t>(), vals_in,
+                                                   vout.as<uint64_t>(), (size_t)nnz, (unsigned)n, rp, rp + 1, 0,
+                                                   endbit, stream_));
+        HIPCHK(hipMemcpyAsync(col_.p, ctmp.p, nnz * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+        if (form_ == kFP64)
+            HIPCHK(hipMemcpyAsync(val_.p, vout.p, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+        else
+            k_unpack_hilo<<<grid_for(nnz, 256), 256, 0, stream_>>>(nnz, vout.as<uint64_t>(), hi_.as<float>(),
+                                                                   lo_.as<float>());
+        npanel_ = (int)((n + kPanel - 1) / kPanel);
+        if (npanel_ > 1) {
+            pp_.ensure((size_t)n * (npanel_ + 1) * sizeof(int32_t));
+            k_panel_ptr<<<grid_for(n, 256), 256, 0, stream_>>>(n, rp, col_.as<int32_t>(), npanel_, kPanel,
+                                                               pp_.as<int32_t>());
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream_));
+    } else {
+        npanel_ = 1;
+    }
     D_.ensure(n * sizeof(double));
     k_diag<NW><<<grid_for(n, 128), 128, 0, stream_>>>(n, P_.M, a.keys, d_freq_.as<double>(), d_diag_.as<uint32_t>(),
                                                       (int)T_.diag_fcs.size(), a.elem, a.ladder, 256,
@@ -1258,41 +1635,56 @@ void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std:
 int64_t Engine::structural_nnz() { return -1; }
 
 // ---------------------------------------------------------------- SpMM
-void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low) {
+int Engine::spmm_launches(int b) const {
+    static const bool panels = std::getenv("VHCI_SPMM_PANELS") != nullptr;
+    return (panels && b >= 2 && npanel_ > 1) ? npanel_ : 1;
+}
+
+void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32, int ldx32) {
     const int64_t n = n_;
-    const unsigned grid = grid_for(n, 4);
     const int64_t* rp = rowptr_.as<int64_t>();
     const int32_t* cl = col_.as<int32_t>();
     const double* D = D_.as<double>();
-    auto launch64 = [&](auto tag) {
-        constexpr int B = decltype(tag)::value;
-        k_spmm64<B><<<grid, 256, 0, stream_>>>(n, rp, cl, val_.as<double>(), D, X, ldx, Y, ldy, b);
-    };
-    auto launch32 = [&](auto tag, const float* X32, int ld32) {
-        constexpr int B = decltype(tag)::value;
-        k_spmm32<B><<<grid, 256, 0, stream_>>>(n, rp, cl, hi_.as<float>(), D, X32, ld32, X, ldx, Y, ldy, b);
-    };
-    auto launchhl = [&](auto tag) {
-        constexpr int B = decltype(tag)::value;
-        k_spmm_hilo<B><<<grid, 256, 0, stream_>>>(n, rp, cl, hi_.as<float>(), lo_.as<float>(), D, X, ldx, Y, ldy, b);
-    };
-    auto dispatch = [&](auto fn) {
-        if (b <= 1) fn(std::integral_constant<int, 1>());
-        else if (b <= 2) fn(std::integral_constant<int, 2>());
-        else if (b <= 4) fn(std::integral_constant<int, 4>());
-        else if (b <= 8) fn(std::integral_constant<int, 8>());
-        else if (b <= 16) fn(std::integral_constant<int, 16>());
-        else if (b <= 32) fn(std::integral_constant<int, 32>());
-        else fn(std::integral_constant<int, 64>());
-    };
-    if (form_ == kFP64) {
-        dispatch(launch64);
-    } else if (low) {
+    const int mode = form_ == kFP64 ? 0 : (low ? 1 : 2);
+    if (mode == 1 && !X32) {
         X32_.ensure((size_t)n * b * sizeof(float));
         k_to_f32<<<grid_for(n * b, 256), 256, 0, stream_>>>(X, ldx, X32_.as<float>(), b, n, b);
-        dispatch([&](auto tag) { launch32(tag, X32_.as<float>(), b); });
-    } else {
-        dispatch(launchhl);
+        X32 = X32_.as<float>();
+        ldx32 = b;
+    }
+    const double* vl = val_.as<double>();
+    const float* h = hi_.as<float>();
+    const float* l = lo_.as<float>();
+    if (b == 1) {
+        const unsigned grid = grid_for(n, 16);
+        if (mode == 0) k_spmv<0><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
+        else if (mode == 1) k_spmv<1><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
+        else k_spmv<2><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
+        HIPCHK(hipGetLastError());
+        return;
+    }
+    const unsigned grid = grid_for(n, 4);
+    const int P = spmm_launches(b);
+    const int32_t* pp = P > 1 ? pp_.as<int32_t>() : nullptr;
+    for (int p = 0; p < P; p++) {
+        auto go = [&](auto tag) {
+            constexpr int B = decltype(tag)::value;
+            if (mode == 0)
+                k_spmm<B, 0><<<grid, 256, 0, stream_>>>(n, rp, pp, npanel_ + 1, p, p == 0, cl, vl, h, l, D, X32,
+                                                        ldx32, X, ldx, Y, ldy, b);
+            else if (mode == 1)
+                k_spmm<B, 1><<<grid, 256, 0, stream_>>>(n, rp, pp, npanel_ + 1, p, p == 0, cl, vl, h, l, D, X32,
+                                                        ldx32, X, ldx, Y, ldy, b);
+            else
+                k_spmm<B, 2><<<grid, 256, 0, stream_>>>(n, rp, pp, npanel_ + 1, p, p == 0, cl, vl, h, l, D, X32,
+                                                        ldx32, X, ldx, Y, ldy, b);
+        };
+        if (b <= 2) go(std::integral_constant<int, 2>());
+        else if (b <= 4) go(std::integral_constant<int, 4>());
+        else if (b <= 8) go(std::integral_constant<int, 8>());
+        else if (b <= 16) go(std::integral_constant<int, 16>());
+        else if (b <= 32) go(std::integral_constant<int, 32>());
+        else go(std::integral_constant<int, 64>());
     }
     HIPCHK(hipGetLastError());
 }
@@ -1317,7 +1709,7 @@ double Engine::time_spmm(int b, int reps, bool fp32, double* bytes_per_call) {
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const double sv = low ? 4.0 : (form_ == kMixed ? 8.0 : 8.0);
+    const double sv = low ? 4.0 : 8.0;
     const double sx = low ? 4.0 : 8.0;
     if (bytes_per_call) *bytes_per_call = (double)nnz_ * (sv + 4) + 8.0 * (n + 1) + 2.0 * n * b * sx + 8.0 * n;
     return ms / reps;
@@ -1367,12 +1759,14 @@ EigStats Engine::dense_all(std::vector<double>& E) {
 // C (p x q, col-major, ldc) = A^T B for column-major A (n x p), B (n x q)
 static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const double* A, int64_t lda,
                    const double* B, int64_t ldb, double* C, int ldc) {
-    const int64_t rpc = std::max<int64_t>(2048, ((n + 255) / 256 + 31) / 32 * 32);
+    // about 384 chunks of rows (>= 256 rows each): enough blocks for the CUs,
+    // few partial sums to combine
+    const int64_t rpc = std::max<int64_t>(256, ((n + 383) / 384 + 31) / 32 * 32);
     const int nchunk = (int)((n + rpc - 1) / rpc);
     part.ensure((size_t)nchunk * p * q * sizeof(double));
     dim3 g(nchunk, (p + 63) / 64, (q + 15) / 16);
     k_tsm_tn<<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
-    k_tsm_reduce<<<grid_for((int64_t)p * q, 256), 256, 0, s>>>(part.as<double>(), nchunk, p, q, C, ldc);
+    k_tsm_reduce<<<grid_for((int64_t)p * q, 64), 256, 0, s>>>(part.as<double>(), nchunk, p, q, C, ldc);
     HIPCHK(hipGetLastError());
 }
 
@@ -1453,20 +1847,7 @@ void Engine::apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, E
         if (f32) {
             stage_in32_.ensure((size_t)n * kk * sizeof(float));
             k_cm_to_rm<float><<<tg, 256, 0, stream_>>>(X + c * ld, ld, n, kk, stage_in32_.as<float>(), kk);
-            const unsigned grid = grid_for(n, 4);
-            auto go = [&](auto tag) {
-                constexpr int B = decltype(tag)::value;
-                k_spmm32<B><<<grid, 256, 0, stream_>>>(n, rowptr_.as<int64_t>(), col_.as<int32_t>(), hi_.as<float>(),
-                                                       D_.as<double>(), stage_in32_.as<float>(), kk,
-                                                       stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk);
-            };
-            if (kk <= 1) go(std::integral_constant<int, 1>());
-            else if (kk <= 2) go(std::integral_constant<int, 2>());
-            else if (kk <= 4) go(std::integral_constant<int, 4>());
-            else if (kk <= 8) go(std::integral_constant<int, 8>());
-            else if (kk <= 16) go(std::integral_constant<int, 16>());
-            else if (kk <= 32) go(std::integral_constant<int, 32>());
-            else go(std::integral_constant<int, 64>());
+            spmm(stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk, true, stage_in32_.as<float>(), kk);
         } else {
             spmm(stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk, false);
         }
@@ -1732,9 +2113,11 @@ void Engine::davidson(int nev, double tol, int maxit, std::vector<double>& E, Ei
 void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st) {
     const int64_t n = n_;
     const int ni = (int)n;
-    int bs = 4;
+    int bs = 8;
     if (const char* e = std::getenv("VHCI_KS_BLOCK")) bs = std::max(1, std::min(32, std::atoi(e)));
-    int m = ((2 * nev + bs + bs - 1) / bs) * bs;
+    // basis of m vectors (a multiple of bs, at least 2 nev and nev + 6 blocks);
+    // each cycle adds about half of m - nev new vectors
+    int m = ((std::max(2 * nev, nev + 6 * bs) + bs + bs - 1) / bs) * bs;
     if (const char* e = std::getenv("VHCI_KS_DIM")) m = std::max(m, (std::atoi(e) / bs) * bs);
     while (m + bs > n) m -= bs;
     if (m < nev + 2 * bs) throw std::runtime_error("Krylov-Schur: basis too small for the requested states");
@@ -2009,6 +2392,9 @@ void Engine::hb_select_t(double eps, HbStats& st) {
         HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, stmp, col_rank_.as<uint64_t>(), srt_rank_.as<uint64_t>(),
                                          col_idx_.as<uint32_t>(), srt_idx_.as<uint32_t>(), (size_t)u, 0, endbit,
                                          stream_));
+        nmask_ = cap - 1;
+        pool_pos_.ensure((size_t)pool_cap * sizeof(uint32_t));
+        k_pool_pos<<<grid_for((int64_t)u, 256), 256, 0, stream_>>>(srt_idx_.as<uint32_t>(), u, pool_pos_.as<uint32_t>());
         newkeys_.ensure((size_t)std::max<uint32_t>(u, 1) * KW_ * 16);
         k_gather_keys<<<grid_for((int64_t)u * KW_, 256), 256, 0, stream_>>>(
             pool_keys_.as<uint4>(), srt_idx_.as<uint32_t>(), u, KW_, newkeys_.as<uint4>());
@@ -2090,6 +2476,94 @@ void Engine::append_new_states() {
 }
 
 // ---------------------------------------------------------------- PT2
+// Push form; returns false (nothing launched) when the pair list would not
+// fit in a quarter of the free device memory.
+template <int NW>
+bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st) {
+    const int64_t n = n_, ns2 = nnew_;
+    Pt2PushArgs a{};
+    a.n = n;
+    a.M = P_.M;
+    a.P = P_deltas_;
+    a.vkeys = keys_.as<uint4>();
+    a.aslots = nslots_.as<uint64_t>();
+    a.amask = nmask_;
+    a.apool = pool_keys_.as<uint4>();
+    a.apos = pool_pos_.as<uint32_t>();
+    a.deltas = d_deltas_.as<DeltaEnt>();
+    rowcnt_.ensure((n + 1) * sizeof(int64_t));
+    candoff_.ensure((n + 1) * sizeof(int64_t));
+    a.cnt = rowcnt_.as<int64_t>();
+    if (a.P == 0) return false;
+    k_pt2_push<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
+    k_fill<int64_t><<<1, 1, 0, stream_>>>(a.cnt + n, 1, 0);
+    size_t tmp = 0;
+    HIPCHK(rocprim::exclusive_scan(nullptr, tmp, a.cnt, candoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                   rocprim::plus<int64_t>(), stream_));
+    scan_tmp_.ensure(tmp);
+    HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, a.cnt, candoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                   rocprim::plus<int64_t>(), stream_));
+    int64_t npairs = 0;
+    HIPCHK(hipMemcpyAsync(&npairs, candoff_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    st.pairs = npairs;
+    size_t freeb = 0, totb = 0;
+    HIPCHK(hipMemGetInfo(&freeb, &totb));
+    if ((double)npairs * 32.0 > 0.25 * (double)freeb) return false;
+    DevBuf pk, pv, sk, sv, seg;
+    pk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
+    pv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
+    sk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
+    sv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
+    seg.ensure((ns2 + 1) * sizeof(int64_t));
+    a.off = candoff_.as<int64_t>();
+    a.pkey = pk.as<uint32_t>();
+    a.pval = pv.as<uint64_t>();
+    if (npairs > 0) {
+        k_pt2_push<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
+        HIPCHK(hipGetLastError());
+        unsigned endbit = 1;
+        while (endbit < 32 && ((uint64_t)ns2 >> endbit) != 0) endbit++;
+        size_t stmp = 0;
+        HIPCHK(rocprim::radix_sort_pairs(nullptr, stmp, pk.as<uint32_t>(), sk.as<uint32_t>(), pv.as<uint64_t>(),
+                                         sv.as<uint64_t>(), (size_t)npairs, 0, endbit, stream_));
+        scan_tmp_.ensure(stmp);
+        HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, stmp, pk.as<uint32_t>(), sk.as<uint32_t>(), pv.as<uint64_t>(),
+                                         sv.as<uint64_t>(), (size_t)npairs, 0, endbit, stream_));
+    }
+    k_seg_start<<<grid_for(ns2 + 1, 256), 256, 0, stream_>>>(sk.as<uint32_t>(), npairs, ns2, seg.as<int64_t>());
+    HIPCHK(hipGetLastError());
+    const bool mixed = form_ == kMixed;
+    Pt2NumArgs b{};
+    b.ns2 = ns2;
+    b.M = P_.M;
+    b.nq = 256;
+    b.nopt = nopt;
+    b.akeys = newkeys_.as<uint4>();
+    b.seg = seg.as<int64_t>();
+    b.sval = sv.as<uint64_t>();
+    b.deltas = d_deltas_.as<DeltaEnt>();
+    b.groups = d_groups_.as<GroupEnt>();
+    b.fclist = d_fclist_.as<uint32_t>();
+    b.elem = d_elem_.as<ElemFC>();
+    b.ladder = d_ladder_.as<double>();
+    b.dfc = d_diag_.as<uint32_t>();
+    b.ndfc = (int)T_.diag_fcs.size();
+    b.freq = d_freq_.as<double>();
+    b.C = X_.as<double>();
+    b.C32 = C32_.as<float>();
+    b.ldc = mixed ? nopt : xcols_;
+    b.E = Ed;
+    b.part = pt2_part_.as<double>();
+    if (mixed)
+        k_pt2_num<NW, true><<<nwaves, 64, 0, stream_>>>(b);
+    else
+        k_pt2_num<NW, false><<<nwaves, 64, 0, stream_>>>(b);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+    return true;
+}
+
 template <int NW>
 void Engine::pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::vector<double>& dE) {
     const int64_t ns2 = nnew_;
@@ -2109,6 +2583,14 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::ve
     }
     int nwaves = (int)std::min<int64_t>(ns2, 256 * 32);
     pt2_part_.ensure((size_t)nwaves * nopt * 2 * sizeof(double));
+    if (!std::getenv("VHCI_PT2_PULL") && pt2_push_t<NW>(Ed.as<double>(), nopt, nwaves, st)) {
+        k_pt2_reduce<<<grid_for(nopt, 64), 64, 0, stream_>>>(pt2_part_.as<double>(), nwaves, nopt, out.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dE.data(), out.p, nopt * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        st.sum_ms = ms_since(t0);
+        return;
+    }
     Pt2Args a{};
     a.ns2 = ns2;
     a.M = P_.M;

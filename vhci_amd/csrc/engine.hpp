@@ -51,6 +51,7 @@ struct EigStats {
 
 struct Pt2Stats {
     int64_t ns2 = 0;
+    int64_t pairs = 0;  // (external, variational) pairs of the push form
     int64_t candidates = 0;
     double select_ms = 0, sum_ms = 0;
 };
@@ -107,11 +108,15 @@ class Engine {
     double last_spmm_ms() const { return last_spmm_ms_; }
     // time one SpMM of b vectors, averaged over reps (for the roofline figure)
     double time_spmm(int b, int reps, bool fp32, double* bytes_per_call);
+    // kernel launches per SpMM of b vectors (one per column panel)
+    int spmm_launches(int b) const;
+    static constexpr int kPanel = 32768;  // columns per panel of the block SpMM
 
   private:
     void upload_tables();
     void rebuild_hash(int64_t cap_hint);
-    void spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low);
+    void spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32 = nullptr,
+              int ldx32 = 0);
     void dense_solve(int nev, std::vector<double>& E, EigStats& st);
     // column-major block helpers of the iterative eigensolvers
     void apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st);
@@ -125,6 +130,8 @@ class Engine {
     void build_h_t(HStats& st);
     template <int NW>
     void pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::vector<double>& dE);
+    template <int NW>
+    bool pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st);
 
     Problem P_;
     Tables T_;
@@ -145,7 +152,8 @@ class Engine {
     // Hamiltonian
     bool h_valid_ = false;
     int64_t nnz_ = 0;
-    DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_;
+    DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_, pp_;
+    int npanel_ = 1;
     // Ritz block (row-major n x xcols_, ld = xcols_)
     DevBuf X_;
     int64_t xrows_ = 0;
@@ -157,6 +165,8 @@ class Engine {
     DevBuf cmax_, candcnt_, candoff_, nslots_, pool_keys_, pool_rank_, counters_, col_rank_, col_idx_,
         srt_rank_, srt_idx_, newkeys_, scan_tmp_;
     int64_t nnew_ = 0;
+    uint64_t nmask_ = 0;  // mask of the new-state hash of the last selection
+    DevBuf pool_pos_;     // pool entry -> position in first-discovery order
     // PT2 work
     DevBuf s2keys_, pt2_part_, C32_;
     double last_spmm_ms_ = 0;
