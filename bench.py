@@ -162,7 +162,7 @@ def main():
     # X read once and Y written once, diagonal).
     ctx64 = res["fp64"][1][0] if "fp64" in res else ctx
     ms1, b1 = ctx64.time_spmm(1, args.spmm_reps)
-    bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
+    bs = int(os.environ.get("VHCI_KS_BLOCK", "4"))
     msb, bb = ctx64.time_spmm(bs, args.spmm_reps)
     out["spmv"] = {"ms": ms1, "bytes": b1, "GBs": b1 / ms1 / 1e6, "frac_of_hbm": b1 / ms1 / 1e6 / HBM_PEAK_GBS,
                    "n": ctx64.n}

@@ -88,6 +88,14 @@ int vhci_get_basis(vhci_ctx* ctx, int32_t* q);
  * nonzeros, structural nonzeros of the reference's triplet list (-1 if not
  * computed), build time in microseconds. */
 int vhci_build_h(vhci_ctx* ctx, int64_t* stats);
+/* Structural nonzeros of the reference's triplet list for the current
+ * basis: n plus every ordered pair whose quanta differ by an even L1
+ * distance <= the highest FC order or an odd one <= that order - 1
+ * (src/Ham.cpp:247,283: explicit zeros included, the count behind the
+ * "The Hamiltonian is X% sparse." line, src/Ham.cpp:341-342).  Returns -1
+ * when the displacement ball exceeds 500000 vectors (then the driver prints
+ * the line from the stored entries), or a negative error code. */
+int64_t vhci_structural_nnz(vhci_ctx* ctx);
 /* Copy H to the host: rowptr[n+1], col/val[nnz] (off-diagonal entries in
  * stored order), diag[n].  Pass rowptr=NULL to query nnz via *nnz. */
 int vhci_get_h(vhci_ctx* ctx, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val, double* diag);

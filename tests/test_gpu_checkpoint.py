@@ -1,0 +1,92 @@
+"""Checkpoint / restart of the drop-in `vhci` program (checkpoint format of
+Core_functions.cpp:94-167, written every heat-bath iteration once the basis
+exceeds 1e5 states).  A run stopped right after a checkpoint (the
+VHCI_STOP_AFTER_ITERATION hook stands in for a killed job) and resumed with
+the same -c file must end with the uninterrupted run's basis and energies;
+damaged checkpoints must fail with the reference's messages."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, absolute, read_freqs
+
+import vhci_amd
+from vhci_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _vhci(args, env=None, timeout=300):
+    e = dict(os.environ)
+    e.update(env or {})
+    return subprocess.run([vhci_amd.CLI_PATH] + args, capture_output=True, text=True, env=e, timeout=timeout)
+
+
+@pytest.fixture(scope="module")
+def c1(tmp_path_factory):
+    d = tmp_path_factory.mktemp("ck")
+    inp = str(d / "c1.inp")
+    synth.config_input(1, inp, pt2=False)
+    full = _vhci(["-n", "2", "-i", inp, "-o", str(d / "full.freqs")])
+    assert full.returncode == 0, full.stdout[-2000:]
+    return d, inp, full.stdout
+
+
+def _sizes(log):
+    return [int(x) for x in re.findall(r"Basis size after iteration \d+ is: (\d+)", log)]
+
+
+def test_checkpoint_written_and_readable(c1):
+    d, inp, full_log = c1
+    ck = str(d / "a.ck")
+    r = _vhci(["-n", "2", "-i", inp, "-o", str(d / "a.freqs"), "-c", ck],
+              env={"VHCI_STOP_AFTER_ITERATION": "2"})
+    assert r.returncode != 0 and "stopped after heat-bath iteration 2" in r.stdout
+    assert "Writing checkpoint file...done." in r.stdout
+    with open(ck) as f:
+        head = f.readline().split()
+        freqs = f.readline().split()
+    n2 = _sizes(full_log)[1]
+    assert head == [str(n2), "15", "100"]
+    assert len(freqs) == 15 + 100
+    assert all(re.fullmatch(r"-?\d+\.\d{17}", x) for x in freqs)
+
+
+def test_resume_matches_uninterrupted_run(c1):
+    d, inp, full_log = c1
+    ck = str(d / "b.ck")
+    r = _vhci(["-n", "2", "-i", inp, "-o", str(d / "b.freqs"), "-c", ck],
+              env={"VHCI_STOP_AFTER_ITERATION": "2"})
+    assert r.returncode != 0
+    r2 = _vhci(["-n", "2", "-i", inp, "-o", str(d / "b.freqs"), "-c", ck])
+    assert r2.returncode == 0, r2.stdout[-2000:]
+    assert "Checkpoint file found. Calculation will be resumed." in r2.stdout
+    m_full = re.search(r"contains (\d+) product states", full_log)
+    m_res = re.search(r"contains (\d+) product states", r2.stdout)
+    assert m_full.group(1) == m_res.group(1)
+    e, _ = read_freqs(str(d / "b.freqs"))
+    ef, _ = read_freqs(str(d / "full.freqs"))
+    # the resumed run starts from 17-decimal vectors; energies agree far
+    # below the reporting precision
+    assert np.max(np.abs(absolute(e) - absolute(ef))) < 1e-6
+
+
+def test_damaged_checkpoints(tmp_path):
+    inp = os.path.join(GOLDEN, "clo2_vhci.inp")
+    bad = tmp_path / "bad.ck"
+    bad.write_text("10 4 20\n")
+    r = _vhci(["-n", "1", "-i", inp, "-o", str(tmp_path / "x.freqs"), "-c", str(bad)])
+    assert "Error: Checkpoint file contains wrong number of fundamental modes!" in r.stdout
+    bad.write_text("10 3 7\n")
+    r = _vhci(["-n", "1", "-i", inp, "-o", str(tmp_path / "x.freqs"), "-c", str(bad)])
+    assert "Error: Checkpoint file contains wrong number of eigenstates!" in r.stdout
+    bad.write_text("10 3 20\n455.6 960.149798 1127.824246\n")
+    r = _vhci(["-n", "1", "-i", inp, "-o", str(tmp_path / "x.freqs"), "-c", str(bad)])
+    assert "Error: Checkpoint file contains wrong fundamental frequencies!" in r.stdout
+    empty = tmp_path / "empty.ck"
+    empty.write_text("")
+    r = _vhci(["-n", "1", "-i", inp, "-o", str(tmp_path / "y.freqs"), "-c", str(empty)])
+    assert r.returncode == 0 and "No checkpoint file found. Calculation will start from the beginning." in r.stdout

@@ -96,6 +96,9 @@ def test_h_build_c1_grown(c1_iter1):
     ctx.set_basis(q)
     ctx.build_h()
     _compare_h(ctx, pyoracle.OProblem(path), q)
+    # structural count of the reference's triplet list (explicit zeros)
+    orp, _, _ = pyoracle.OProblem(path).build_h(q)
+    assert ctx.structural_nnz() == orp[-1]
 
 
 def test_hb_select_matches_oracle_order(c1_iter1):
@@ -206,6 +209,9 @@ def test_cli_clo2_fixtures(tmp_path, name):
     m = re.search(r"Perturbative space contains (\d+) states", log)
     if "pt2_space" in pins:
         assert int(m.group(1)) == pins["pt2_space"]
+    # the structural "% sparse" lines (explicit zeros counted)
+    pct = [float(x) for x in re.findall(r"The Hamiltonian is ([\d.]+)% sparse", log)]
+    assert pct == pins["sparse_pct"]
 
 
 def test_cli_mixed_matches_fp64(tmp_path):

@@ -46,6 +46,7 @@ _SIGS = {
     "vhci_basis_size": (ctypes.c_int64, [_vp]),
     "vhci_get_basis": (ctypes.c_int, [_vp, _i32p]),
     "vhci_build_h": (ctypes.c_int, [_vp, _i64p]),
+    "vhci_structural_nnz": (ctypes.c_int64, [_vp]),
     "vhci_get_h": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _dp, _dp]),
     "vhci_eigsolve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "vhci_dense_diag": (ctypes.c_int, [_vp, _dp, _dp]),
@@ -194,6 +195,12 @@ class Context:
         st = np.zeros(4, dtype=np.int64)
         self._check(lib().vhci_build_h(self.h, _ptr(st, _i64p)))
         return {"n": int(st[0]), "nnz_offdiag": int(st[1]), "nnz_struct": int(st[2]), "us": int(st[3])}
+
+    def structural_nnz(self):
+        r = int(lib().vhci_structural_nnz(self.h))
+        if r < -1:
+            self._check(r)
+        return r
 
     def get_h(self):
         L = lib()

@@ -72,6 +72,7 @@ struct RunOpts {
     std::string cpname = "none";
     bool restart = false;
     bool log = false;
+    int stop_after = 0;  // failure injection: end the run after this HCI iteration
 };
 
 struct RunResult {
@@ -135,6 +136,7 @@ void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Lo
             log("Constructing the Vibrational CI Hamiltonian...");
             auto hs = eng.build_h();
             R.t_build += hs.build_ms;
+            if (log.on) hs.nnz_struct = eng.structural_nnz();
             log(sparse_line(hs));
             if (P.neig == 1)
                 log("done.\n\nSolving for the ground state...\n\n");
@@ -186,6 +188,7 @@ void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Lo
             log(" Constructing Hamiltonian...");
             auto hs = eng.build_h();
             R.t_build += hs.build_ms;
+            if (log.on) hs.nnz_struct = eng.structural_nnz();
             log(sparse_line(hs));
             if (P.neig == 1)
                 log(" Finding the ground state...\n");
@@ -201,6 +204,9 @@ void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Lo
                 save_checkpoint(eng, P, E, o, log);
                 log("done.\n\n");
             }
+            if (o.stop_after > 0 && iter - 1 >= o.stop_after)
+                throw std::runtime_error("stopped after heat-bath iteration " + std::to_string(iter - 1) +
+                                         " (VHCI_STOP_AFTER_ITERATION)");
         }
     }
     R.Evar = E;
@@ -420,6 +426,13 @@ int vhci_build_h(vhci_ctx* ctx, int64_t* stats) {
             stats[3] = (int64_t)(h.build_ms * 1000);
         }
     });
+}
+
+int64_t vhci_structural_nnz(vhci_ctx* ctx) {
+    if (!ctx || !ctx->eng) return VHCI_E_ARG;
+    int64_t r = -1;
+    int rc = guard(ctx, [&] { r = ctx->eng->structural_nnz(); });
+    return rc ? rc : r;
 }
 
 int vhci_get_h(vhci_ctx* ctx, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val, double* diag) {
@@ -698,6 +711,7 @@ int vhci_main(int argc, char** argv) {
     if (const char* d = std::getenv("VHCI_DEVICE")) device = std::atoi(d);
     RunOpts o;
     if (const char* t = std::getenv("VHCI_TOL")) o.tol = std::atof(t);
+    if (const char* t = std::getenv("VHCI_STOP_AFTER_ITERATION")) o.stop_after = std::atoi(t);
     o.cpname = cpname;
     o.restart = restart;
     o.log = true;

@@ -1632,7 +1632,38 @@ void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std:
     HIPCHK(hipStreamSynchronize(stream_));
 }
 
-int64_t Engine::structural_nnz() { return -1; }
+// Structural nonzeros of the reference's triplet list (Ham.cpp:237-326):
+// every pair whose quanta differ by an even L1 distance <= fcmax or an odd
+// one <= fcmax-1 gets a triplet, zero or not; after H + H^T that is n plus
+// twice the off-diagonal pairs.  Counted by probing the L1 ball.
+int64_t Engine::structural_nnz() {
+    if (T_.ball.empty() || n_ == 0) return -1;
+    HIPCHK(hipSetDevice(device_));
+    if (!d_ball_.p) upload(d_ball_, T_.ball, stream_);
+    HBuildArgs a{};
+    a.n = n_;
+    a.M = P_.M;
+    a.P = (int)T_.ball.size();
+    a.keys = keys_.as<uint4>();
+    a.slots = slots_.as<uint64_t>();
+    a.mask = hmask_;
+    a.deltas = d_ball_.as<DeltaEnt>();
+    rowcnt_.ensure((n_ + 1) * sizeof(int64_t));
+    a.rowcnt = rowcnt_.as<int64_t>();
+    switch (KW_) {
+        case 1: k_hbuild<4, false><<<(unsigned)n_, 64, 0, stream_>>>(a); break;
+        case 2: k_hbuild<8, false><<<(unsigned)n_, 64, 0, stream_>>>(a); break;
+        case 3: k_hbuild<12, false><<<(unsigned)n_, 64, 0, stream_>>>(a); break;
+        default: k_hbuild<16, false><<<(unsigned)n_, 64, 0, stream_>>>(a); break;
+    }
+    HIPCHK(hipGetLastError());
+    std::vector<int64_t> cnt(n_);
+    HIPCHK(hipMemcpyAsync(cnt.data(), rowcnt_.p, n_ * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    int64_t tot = n_;
+    for (int64_t c : cnt) tot += c;
+    return tot;
+}
 
 // ---------------------------------------------------------------- SpMM
 int Engine::spmm_launches(int b) const {
@@ -1890,26 +1921,52 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
     work_.ensure((size_t)std::max<size_t>(work_.cap / sizeof(double), 2 * (size_t)k) * sizeof(double));
     info_.ensure(sizeof(int) * 4);
     const bool small = k <= 32;
-    for (int pass = 0; pass < 2; pass++) {
-        if (m0 > 0) {
-            if (small) {
-                tsm_tn(stream_, tsm_part_, n, m0, k, V, ld, T, ld, S_.as<double>(), m0);
-                tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), m0, 1.0, T, ld);
-            } else {
-                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, m0, k, ni, &one, V,
-                                      ld, T, ld, &zero, S_.as<double>(), m0));
-                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, k, m0, &mone, V, ld,
-                                      S_.as<double>(), m0, &one, T, ld));
-            }
-        }
+    auto gram_host = [&](int kk, std::vector<double>& g) {
         if (small)
-            tsm_tn(stream_, tsm_part_, n, k, k, T, ld, T, ld, gram_.as<double>(), k);
+            tsm_tn(stream_, tsm_part_, n, kk, kk, T, ld, T, ld, gram_.as<double>(), kk);
         else
-            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, k, k, ni, &one, T, ld, T,
-                                  ld, &zero, gram_.as<double>(), k));
-        std::vector<double> g((size_t)k * k);
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, kk, kk, ni, &one, T, ld,
+                                  T, ld, &zero, gram_.as<double>(), kk));
+        g.resize((size_t)kk * kk);
         HIPCHK(hipMemcpyAsync(g.data(), gram_.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
+    };
+    auto cgs = [&]() {
+        if (small) {
+            tsm_tn(stream_, tsm_part_, n, m0, k, V, ld, T, ld, S_.as<double>(), m0);
+            tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), m0, 1.0, T, ld);
+        } else {
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, m0, k, ni, &one, V, ld,
+                                  T, ld, &zero, S_.as<double>(), m0));
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, k, m0, &mone, V, ld,
+                                  S_.as<double>(), m0, &one, T, ld));
+        }
+    };
+    // squared column norms before projection (DGKS test)
+    std::vector<double> g, before;
+    if (m0 > 0) {
+        gram_host(k, g);
+        before.resize(k);
+        for (int i = 0; i < k; i++) before[i] = g[(size_t)i * k + i];
+    }
+    int passes = 0;
+    for (int round = 0; round < 4; round++) {
+        if (m0 > 0) {
+            cgs();
+            passes++;
+        }
+        gram_host(k, g);
+        if (m0 > 0 && passes == 1 && round == 0) {
+            // a column that lost more than half its squared norm to the
+            // projection gets a second Gram-Schmidt pass (Daniel, Gragg,
+            // Kaufman & Stewart, Math. Comp. 30 (1976))
+            bool cancel = false;
+            for (int i = 0; i < k; i++) cancel = cancel || g[(size_t)i * k + i] < 0.5 * before[i];
+            if (cancel) {
+                for (int i = 0; i < k; i++) before[i] = g[(size_t)i * k + i];
+                continue;
+            }
+        }
         std::vector<double> dsc(k);
         for (int i = 0; i < k; i++) dsc[i] = g[(size_t)i * k + i] > 0 ? 1.0 / std::sqrt(g[(size_t)i * k + i]) : 0.0;
         for (int i = 0; i < k; i++)
@@ -1932,6 +1989,7 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
             if (lam[c] > 1e-10 * std::max(1.0, lmax)) keep.push_back(c);
         const int kk = (int)keep.size();
         if (kk == 0) return 0;
+        const double lmin = lam[keep.back()];
         std::vector<double> Z((size_t)k * kk);
         for (int c = 0; c < kk; c++) {
             const int src = keep[c];
@@ -1949,6 +2007,13 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
                                 hipMemcpyDeviceToDevice, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
         k = kk;
+        // a well-conditioned block is done; otherwise repeat projection and
+        // orthonormalisation once more on the new block
+        if (lmin > 1e-4 * std::max(1.0, lmax) || round >= 2) return k;
+        if (m0 > 0) {
+            before.assign(k, 1.0);
+            passes = 1;  // the next round's projection is a second pass
+        }
     }
     return k;
 }
@@ -2117,7 +2182,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     if (const char* e = std::getenv("VHCI_KS_BLOCK")) bs = std::max(1, std::min(32, std::atoi(e)));
     // basis of m vectors (a multiple of bs, at least 2 nev and nev + 6 blocks);
     // each cycle adds about half of m - nev new vectors
-    int m = ((std::max(2 * nev, nev + 6 * bs) + bs + bs - 1) / bs) * bs;
+    int m = ((std::max(std::max(3 * nev, 2 * nev + bs), nev + 6 * bs) + bs - 1) / bs) * bs;
     if (const char* e = std::getenv("VHCI_KS_DIM")) m = std::max(m, (std::atoi(e) / bs) * bs);
     while (m + bs > n) m -= bs;
     if (m < nev + 2 * bs) throw std::runtime_error("Krylov-Schur: basis too small for the requested states");

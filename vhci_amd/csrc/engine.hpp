@@ -147,7 +147,7 @@ class Engine {
     uint64_t hmask_ = 0;
     // tables
     DevBuf d_ladder_, d_elem_, d_deltas_, d_groups_, d_fclist_, d_diag_, d_freq_;
-    DevBuf d_hbfc_, d_hbvaroff_, d_hbvars_;
+    DevBuf d_hbfc_, d_hbvaroff_, d_hbvars_, d_ball_;
     int P_deltas_ = 0;
     // Hamiltonian
     bool h_valid_ = false;

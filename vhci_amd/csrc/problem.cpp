@@ -3,6 +3,8 @@
 #include "problem.hpp"
 
 #include <algorithm>
+#include <functional>
+#include <numeric>
 #include <cmath>
 #include <fstream>
 #include <iomanip>
@@ -307,6 +309,53 @@ void build_tables(const Problem& P, Tables& T) {
             T.deltas.push_back(de);
         }
         gid++;
+    }
+    // L1 ball of the pair prefilter, enumerated as (sorted modes, signed
+    // deltas); counted first so an oversized ball is skipped
+    {
+        const int R = P.fcmax;
+        const int M = P.M;
+        // number of vectors with exactly k nonzero modes and L1 norm s:
+        // C(M,k) * C(s-1,k-1) * 2^k
+        auto binom = [](int nn, int kk) {
+            if (kk < 0 || kk > nn) return 0.0;
+            double r = 1;
+            for (int i = 1; i <= kk; i++) r = r * (nn - kk + i) / i;
+            return r;
+        };
+        double total = 0;
+        for (int sN = 1; sN <= R; sN++) {
+            if ((sN & 1) && sN > R - 1) continue;
+            for (int k = 1; k <= std::min(sN, 6); k++) total += binom(M, k) * binom(sN - 1, k - 1) * std::pow(2.0, k);
+        }
+        if (R >= 1 && total <= (double)kMaxBall) {
+            std::vector<int> modes, mags;
+            std::function<void(int, int)> rec = [&](int start, int left) {
+                const int sN = std::accumulate(mags.begin(), mags.end(), 0);
+                if (!modes.empty() && (!(sN & 1) || sN <= R - 1)) {
+                    const int nz = (int)modes.size();
+                    for (int sg = 0; sg < (1 << nz); sg++) {
+                        DeltaEnt de{};
+                        for (int t = 0; t < nz; t++) {
+                            de.mode[t] = (uint8_t)modes[t];
+                            de.delta[t] = (int8_t)(((sg >> t) & 1) ? -mags[t] : mags[t]);
+                        }
+                        de.meta = (uint32_t)nz << 28;
+                        T.ball.push_back(de);
+                    }
+                }
+                if ((int)modes.size() == 6) return;
+                for (int m = start; m < M; m++)
+                    for (int v = 1; v <= left; v++) {
+                        modes.push_back(m);
+                        mags.push_back(v);
+                        rec(m + 1, left - v);
+                        modes.pop_back();
+                        mags.pop_back();
+                    }
+            };
+            rec(0, R);
+        }
     }
     // heat-bath variants in sorted-FC order (HB.cpp:27-164): delta_t runs
     // -k..k in steps of 2 with the last mode fastest; all-zero is skipped.

@@ -100,10 +100,15 @@ struct Tables {
     std::vector<int64_t> hb_varoff;    // prefix of variant counts, size nhb+1
     std::vector<HbVar> hb_vars;
     int max_hb_delta = 0;              // largest |delta| of any HB variant
+    // every nonzero displacement with even L1 norm <= fcmax or odd L1 norm
+    // <= fcmax-1 (the reference's pair prefilter); empty when it would
+    // exceed kMaxBall entries
+    std::vector<DeltaEnt> ball;
     int max_delta = 0;                 // largest |delta| of any pattern
 };
 
 void build_tables(const Problem& P, Tables& T);
+constexpr size_t kMaxBall = 500000;
 
 // Ladder table T[k][qs][qe-qs+k], k = 0..6, qs = 0..qmax, 13 entries per row.
 void build_ladder(int qmax, std::vector<double>& L);
