@@ -111,6 +111,9 @@ def main():
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--profile-spmm", action="store_true",
+                    help="only build the final H of the workload (FP64), then time --spmm-reps block SpMMs and "
+                         "one 1 GiB calibration copy (for rocprofv3 kernel and counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -118,6 +121,16 @@ def main():
         sys.exit(2)
     text = synth.config_input(1)
     problem = vhci_amd.Problem(text=text)
+    if args.profile_spmm:
+        ctx = vhci_amd.Context(vhci_amd.Problem(text=synth.config_input(1, pt2=False)), vhci_amd.FP64)
+        ctx.run()
+        bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
+        ms, by, nl = ctx.time_spmm(bs, args.spmm_reps)
+        cms = ctx.copy_probe(1 << 30)
+        print(json.dumps({"n": ctx.n, "b": bs, "ms_per_product": ms, "launches_per_product": nl,
+                          "algorithmic_bytes_per_product": by, "GBs": by / ms / 1e6, "copy_bytes": 1 << 30,
+                          "copy_ms": cms, "copy_GBs": 2.0 * (1 << 30) / cms / 1e6}), flush=True)
+        return
     forms = {"fp64": vhci_amd.FP64, "mixed": vhci_amd.MIXED}
     if args.forms != "both":
         forms = {args.forms: forms[args.forms]}
@@ -161,18 +174,18 @@ def main():
     # nnz*(8+4) + 8*(N+1) + 2*N*b*8 + 8*N (values, columns, row pointers,
     # X read once and Y written once, diagonal).
     ctx64 = res["fp64"][1][0] if "fp64" in res else ctx
-    ms1, b1 = ctx64.time_spmm(1, args.spmm_reps)
-    bs = int(os.environ.get("VHCI_KS_BLOCK", "4"))
-    msb, bb = ctx64.time_spmm(bs, args.spmm_reps)
+    ms1, b1, _ = ctx64.time_spmm(1, args.spmm_reps)
+    bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
+    msb, bb, nl = ctx64.time_spmm(bs, args.spmm_reps)
     out["spmv"] = {"ms": ms1, "bytes": b1, "GBs": b1 / ms1 / 1e6, "frac_of_hbm": b1 / ms1 / 1e6 / HBM_PEAK_GBS,
                    "n": ctx64.n}
     out["roofline"] = {"kernel": "k_spmm64<%d> (eigensolver block SpMM, %d vectors, FP64 CSR)" % (bs, bs),
                        "bound": "hbm", "achieved": bb / msb / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": bb / msb / 1e6 / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": msb,
-                       "algorithmic_bytes_per_launch": bb}
+                       "frac": bb / msb / 1e6 / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": msb / nl,
+                       "launches_per_product": nl, "algorithmic_bytes_per_launch": bb / nl}
     if "mixed" in res:
         ctx32 = res["mixed"][1][0]
-        ms32, b32 = ctx32.time_spmm(1, args.spmm_reps, fp32=True)
+        ms32, b32, _ = ctx32.time_spmm(1, args.spmm_reps, fp32=True)
         out["spmv_fp32"] = {"ms": ms32, "bytes": b32, "GBs": b32 / ms32 / 1e6}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,

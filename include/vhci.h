@@ -127,10 +127,16 @@ int vhci_append_new_states(vhci_ctx* ctx);
  * for n < nopt.  *ns2 (nullable) receives the external-space size. */
 int vhci_pt2(vhci_ctx* ctx, const double* E, int nopt, double eps2, double* dE, int64_t* ns2);
 
-/* Time b-vector SpMMs with the resident H (reps launches, HIP events on
- * the context stream).  out[0] = ms per launch, out[1] = algorithmic bytes
- * per launch. fp32 != 0 selects the mixed form's FP32 product. */
+/* Time b-vector SpMMs with the resident H (reps products, HIP events on
+ * the context stream).  out[0] = ms per product, out[1] = algorithmic
+ * bytes per product, out[2] = kernel launches per product.  fp32 != 0
+ * selects the mixed form's FP32 product.  Measurement support; no
+ * counterpart in the reference. */
 int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out);
+/* One plain device copy of `bytes` bytes on the context stream (the
+ * calibration kernel for memory-counter readings); *ms its duration.
+ * Measurement support; no counterpart in the reference. */
+int vhci_copy_probe(vhci_ctx* ctx, int64_t bytes, double* ms);
 
 /* ---- whole solve (Perform_HCI + DoPT2 of one input) --------------------
  * Runs the reference's control flow (src/VHCI.cpp:39-114) on a device

@@ -58,6 +58,7 @@ _SIGS = {
     "vhci_append_new_states": (ctypes.c_int, [_vp]),
     "vhci_pt2": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _dp, _i64p]),
     "vhci_time_spmm": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
+    "vhci_copy_probe": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
     "vhci_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _i64p, _i64p, _dp]),
     "vhci_run_ctx": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _ip, _i64p, _dp]),
     "vhci_iteration_sizes": (ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
@@ -285,9 +286,15 @@ class Context:
         return [int(x) for x in out[:k]]
 
     def time_spmm(self, b, reps=20, fp32=False):
-        out = np.zeros(2)
+        """(ms per product, algorithmic bytes per product, launches per product)"""
+        out = np.zeros(3)
         self._check(lib().vhci_time_spmm(self.h, b, reps, int(fp32), _ptr(out, _dp)))
-        return float(out[0]), float(out[1])
+        return float(out[0]), float(out[1]), int(out[2])
+
+    def copy_probe(self, nbytes):
+        ms = ctypes.c_double()
+        self._check(lib().vhci_copy_probe(self.h, int(nbytes), ctypes.byref(ms)))
+        return ms.value
 
 
 def basis_checksum(q):

@@ -503,11 +503,16 @@ int vhci_pt2(vhci_ctx* ctx, const double* E, int nopt, double eps2, double* dE, 
     });
 }
 
+int vhci_copy_probe(vhci_ctx* ctx, int64_t bytes, double* ms) {
+    return guard(ctx, [&] { *ms = ctx->eng->copy_probe((size_t)bytes); });
+}
+
 int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
     return guard(ctx, [&] {
         double bytes = 0;
         out[0] = ctx->eng->time_spmm(b, reps, fp32 != 0, &bytes);
         out[1] = bytes;
+        out[2] = ctx->eng->spmm_launches(b);
     });
 }
 

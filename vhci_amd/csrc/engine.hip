@@ -1720,6 +1720,34 @@ void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low,
     HIPCHK(hipGetLastError());
 }
 
+// plain device copy of `bytes` bytes (16-byte words), timed with HIP events:
+// the calibration kernel for memory-counter readings
+__global__ void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+double Engine::copy_probe(size_t bytes) {
+    HIPCHK(hipSetDevice(device_));
+    const int64_t nw = (int64_t)(bytes / 16);
+    DevBuf a, b;
+    a.ensure(nw * 16);
+    b.ensure(nw * 16);
+    k_fill<uint64_t><<<grid_for(2 * nw, 256), 256, 0, stream_>>>(a.as<uint64_t>(), 2 * nw, 1ull);
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipEventRecord(e0, stream_));
+    k_copy16<<<std::min<unsigned>(grid_for(nw, 256), 8192), 256, 0, stream_>>>(a.as<uint4>(), b.as<uint4>(), nw);
+    HIPCHK(hipEventRecord(e1, stream_));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
 double Engine::time_spmm(int b, int reps, bool fp32, double* bytes_per_call) {
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
     const int64_t n = n_;

@@ -108,6 +108,8 @@ class Engine {
     double last_spmm_ms() const { return last_spmm_ms_; }
     // time one SpMM of b vectors, averaged over reps (for the roofline figure)
     double time_spmm(int b, int reps, bool fp32, double* bytes_per_call);
+    // a plain copy of `bytes` bytes on the context stream; returns ms
+    double copy_probe(size_t bytes);
     // kernel launches per SpMM of b vectors (one per column panel)
     int spmm_launches(int b) const;
     static constexpr int kPanel = 32768;  // columns per panel of the block SpMM
