@@ -13,10 +13,13 @@ refinement, FP64 PT2 denominators and Kahan sums).  The headline value is
 the mixed form's solves/s when every energy is within 0.01 cm^-1 of FP64
 and the selected basis is identical, else the FP64 figure.
 
-The roofline entry is for the block SpMM of the eigensolver on the final H,
-timed with HIP events on the context's stream; the CPU baseline is the
-oracle's pair-evaluation rate on a bounded sample, extrapolated to the
-whole solve with the run's exact pair counts.
+The roofline entry is for the FP64 eigensolver's block SpMM (k_spmm_lane),
+every launch of the timed solves bracketed by HIP events on the context's
+stream; with --traffic its HBM bytes per launch come from two rocprofv3
+counter passes (FETCH_SIZE, WRITE_SIZE) scaled by the same counters' reading
+of a 1 GiB copy.  The CPU baseline is the oracle's pair-evaluation rate on
+a bounded sample, extrapolated to the whole solve with the run's exact pair
+counts.
 """
 import argparse
 import json
@@ -40,24 +43,114 @@ def percentile(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
 
-def run_form(problem, form, steps, warmup):
+def run_form(problem, form, steps, warmup, bs):
+    """Warmup solves, then `steps` timed solves.  The solver's bs-column
+    SpMM launches of every timed solve are bracketed by HIP events on the
+    context stream; returns (latencies, (last ctx, last result), spmm)."""
     for _ in range(warmup):
         ctx = vhci_amd.Context(problem, form)
         ctx.run()
         ctx.close()
     lat = []
     last = None
+    sp = [0.0, 0.0, 0]
     for _ in range(steps):
         t0 = time.perf_counter()
         ctx = vhci_amd.Context(problem, form)
+        ctx.spmm_profile(bs)
         r = ctx.run()
         lat.append(time.perf_counter() - t0)
+        ms, by, nl = ctx.spmm_profile_read()
+        ctx.spmm_profile(0)
+        sp = [sp[0] + ms, sp[1] + by, sp[2] + nl]
         if last is not None:
             last[0].close()
         last = (ctx, r)
         print("[bench] form=%s solve %.3f s nvar=%d ns2=%d" % (form, lat[-1], r["nvar"], r["ns2"]),
               file=sys.stderr, flush=True)
-    return lat, last
+    return lat, last, {"ms": sp[0], "bytes": sp[1], "launches": sp[2]}
+
+
+def spmm_kernel_name(bs, mode):
+    return "k_spmm_lane<%d, %d>" % (bs, mode)
+
+
+def traffic_pass(counters, kernel, timeout_s=170):
+    """One rocprofv3 counter pass (a single FP64 solve of this workload plus
+    the 1 GiB calibration copy) in a child process; returns, per counter, the
+    mean value per dispatch of `kernel` and of the copy kernel k_copy16, and
+    the number of `kernel` dispatches."""
+    import csv
+    import glob
+    import signal
+    import subprocess
+    import tempfile
+
+    d = tempfile.mkdtemp(prefix="vhci_pmc_")
+    cmd = ["rocprofv3", "--pmc"] + counters + ["--output-format", "csv", "-d", d, "-o", "pmc", "--",
+                                               sys.executable, "-u", os.path.abspath(__file__), "--steps", "1",
+                                               "--warmup", "0", "--forms", "fp64", "--no-cpu-baseline"]
+    print("[bench] counter pass: %s" % " ".join(cmd), file=sys.stderr, flush=True)
+    log = os.path.join(d, "pass.log")
+    with open(log, "w") as lf:
+        p = subprocess.Popen(cmd, stdout=lf, stderr=subprocess.STDOUT, start_new_session=True)
+        try:
+            rc = p.wait(timeout=timeout_s)
+        except subprocess.TimeoutExpired:
+            os.killpg(p.pid, signal.SIGKILL)
+            p.wait()
+            raise RuntimeError("counter pass %s timed out (log %s)" % (counters, log))
+    print("[bench] counter pass %s ended with status %d" % (counters, rc), file=sys.stderr, flush=True)
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise RuntimeError("counter pass %s wrote no counter file (exit %d)" % (counters, rc))
+    vals = {c: {"kernel": [], "copy": []} for c in counters}
+    with open(files[0]) as f:
+        for r in csv.DictReader(f):
+            c = r["Counter_Name"]
+            if c not in vals:
+                continue
+            name = r["Kernel_Name"]
+            if kernel in name:
+                vals[c]["kernel"].append(float(r["Counter_Value"]))
+            elif "k_copy16" in name:
+                vals[c]["copy"].append(float(r["Counter_Value"]))
+    out = {}
+    nk = 0
+    for c, v in vals.items():
+        if not v["kernel"] or not v["copy"]:
+            raise RuntimeError("counter pass %s: kernel or copy dispatches missing" % c)
+        out[c] = (sum(v["kernel"]) / len(v["kernel"]), sum(v["copy"]) / len(v["copy"]))
+        nk = len(v["kernel"])
+    return out, nk
+
+
+def measure_traffic(kernel):
+    """HBM bytes per launch of `kernel`.  traffic: a FETCH_SIZE pass and a
+    WRITE_SIZE pass (KiB per dispatch), each scaled by what the same counter
+    reads for the 1 GiB copy (1 GiB fetched, 1 GiB written).  FETCH_SIZE
+    counts a 128-byte read request as 64 bytes on this GPU (the copy reads
+    half its size), while 32- and 64-byte requests count in full, so the
+    copy-scaled figure overstates a kernel whose reads are 64-byte gathers;
+    a third pass counts the read requests by size (32, 64, 128 bytes) and
+    gives the read bytes directly."""
+    f, nk = traffic_pass(["FETCH_SIZE"], kernel)
+    w, _ = traffic_pass(["WRITE_SIZE"], kernel)
+    sizes = ["TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum"]
+    r, _ = traffic_pass(sizes, kernel)
+    gib_kib = float(1 << 20)
+    fk, fc = f["FETCH_SIZE"]
+    wk, wc = w["WRITE_SIZE"]
+    ff, fw = gib_kib / fc, gib_kib / wc
+    traffic = 1024.0 * (fk * ff + wk * fw)
+    read_bytes = 32.0 * r[sizes[0]][0] + 64.0 * r[sizes[1]][0] + 128.0 * r[sizes[2]][0]
+    copy_read = 32.0 * r[sizes[0]][1] + 64.0 * r[sizes[1]][1] + 128.0 * r[sizes[2]][1]
+    return traffic, {"fetch_kib": fk, "write_kib": wk, "dispatches": nk, "copy_fetch_kib": fc,
+                     "copy_write_kib": wc, "fetch_scale": ff, "write_scale": fw,
+                     "read_requests_32_64_128_all": [r[c][0] for c in sizes],
+                     "read_bytes_by_request_size": read_bytes,
+                     "copy_read_bytes_by_request_size": copy_read,
+                     "traffic_by_request_size": read_bytes + 1024.0 * wk * fw}
 
 
 def cpu_baseline(ctx, r, problem_text, threads):
@@ -78,10 +171,10 @@ def cpu_baseline(ctx, r, problem_text, threads):
         os.unlink(path)
     q = ctx.basis()
     ns2 = r["ns2"]
-    nA = min(ns2, 12000)
+    nA = min(ns2, 40000)
     qA = ctx.new_states(ns2)[:nA] if nA else np.zeros((0, q.shape[1]), dtype=np.int32)
     n = q.shape[0]
-    rows = max(1, int(6e9 / n))
+    rows = min(n, max(1, int(2.4e10 / n)))
     nopt = min(o.neig, n)
     rate_h, rate_p, raw = o.sample_rates(q, rows, qA, nopt)
     sizes = ctx.iteration_sizes()
@@ -111,9 +204,12 @@ def main():
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--traffic", action="store_true",
+                    help="also measure the SpMM kernel's HBM traffic with two rocprofv3 counter passes "
+                         "(child processes, run before this process uses the GPU)")
     ap.add_argument("--profile-spmm", action="store_true",
                     help="only build the final H of the workload (FP64), then time --spmm-reps block SpMMs and "
-                         "one 1 GiB calibration copy (for rocprofv3 kernel and counter runs)")
+                         "one 1 GiB calibration copy")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
@@ -121,10 +217,15 @@ def main():
         sys.exit(2)
     text = synth.config_input(1)
     problem = vhci_amd.Problem(text=text)
+    bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
+    kname = spmm_kernel_name(bs, 0)
+    traffic = None
+    if args.traffic:
+        # the counter passes run before this process touches the GPU
+        traffic = measure_traffic(kname)
     if args.profile_spmm:
         ctx = vhci_amd.Context(vhci_amd.Problem(text=synth.config_input(1, pt2=False)), vhci_amd.FP64)
         ctx.run()
-        bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
         ms, by, nl = ctx.time_spmm(bs, args.spmm_reps)
         cms = ctx.copy_probe(1 << 30)
         print(json.dumps({"n": ctx.n, "b": bs, "ms_per_product": ms, "launches_per_product": nl,
@@ -136,7 +237,7 @@ def main():
         forms = {args.forms: forms[args.forms]}
     res = {}
     for name, f in forms.items():
-        res[name] = run_form(problem, f, args.steps, args.warmup)
+        res[name] = run_form(problem, f, args.steps, args.warmup, bs)
 
     out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
@@ -144,11 +245,13 @@ def main():
                                   "Ntot 2, caps 6, eps1 1.0, Nstate 100, PT2 eps2 0.01 (seed 0x5648434901)",
                       "modes": 15, "nstate": 100, "eps1": 1.0, "eps2": 0.01}}
     forms_out = {}
-    for name, (lat, (ctx, r)) in res.items():
+    for name, (lat, (ctx, r), sp) in res.items():
         forms_out[name] = {"solves_per_s": 1.0 / float(np.mean(lat)), "p50_s": percentile(lat, 50),
                            "p99_s": percentile(lat, 99), "nvar": r["nvar"], "ns2": r["ns2"],
                            "hb_iterations": r["hb_iterations"], "timings_ms": r["timings"],
-                           "basis_sizes": ctx.iteration_sizes(), "E0": float(r["E"][0])}
+                           "basis_sizes": ctx.iteration_sizes(), "E0": float(r["E"][0]),
+                           "solver_spmm": {"launches": sp["launches"], "ms": sp["ms"],
+                                           "GBs": sp["bytes"] / sp["ms"] / 1e6 if sp["ms"] > 0 else None}}
     out["forms"] = forms_out
     head = "mixed" if "mixed" in res else "fp64"
     if "mixed" in res and "fp64" in res:
@@ -160,7 +263,7 @@ def main():
                                 "speedup": forms_out["mixed"]["solves_per_s"] / forms_out["fp64"]["solves_per_s"]}
         if not (err <= 0.01 and same) or forms_out["mixed"]["solves_per_s"] < forms_out["fp64"]["solves_per_s"]:
             head = "fp64"
-    lat, (ctx, r) = res[head]
+    lat, (ctx, r), _ = res[head]
     out["value"] = forms_out[head]["solves_per_s"]
     out["unit"] = "solves/s"
     out["ms_per_step"] = 1000.0 * float(np.mean(lat))
@@ -169,24 +272,37 @@ def main():
     out["p50_latency_s"] = forms_out[head]["p50_s"]
     out["p99_latency_s"] = forms_out[head]["p99_s"]
 
-    # SpMV / block SpMM on the final Hamiltonian (FP64 context), HIP events
-    # on the context's stream.  Algorithmic bytes per launch:
-    # nnz*(8+4) + 8*(N+1) + 2*N*b*8 + 8*N (values, columns, row pointers,
-    # X read once and Y written once, diagonal).
+    # Roofline of the dominant kernel: the FP64 eigensolver's bs-column SpMM,
+    # every launch of the timed solves bracketed by HIP events on the
+    # context's stream.  Algorithmic bytes per launch (summed over the
+    # launches, whose H grows with the basis): nnz*(8+4) + 8*(N+1) + 8*N +
+    # 2*N*bs*8 (values and columns, row pointers, diagonal, X read once and
+    # Y written once).
     ctx64 = res["fp64"][1][0] if "fp64" in res else ctx
+    if "fp64" in res:
+        sp = res["fp64"][2]
+        ach = sp["bytes"] / sp["ms"] / 1e6
+        per = sp["bytes"] / sp["launches"]
+        out["roofline"] = {"kernel": "%s (FP64 eigensolver block SpMM, %d vectors, CSR)" % (kname, bs),
+                           "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": sp["ms"] / sp["launches"],
+                           "launches": sp["launches"], "algorithmic_bytes_per_launch": per}
+        if traffic is not None:
+            t, detail = traffic
+            out["roofline"]["traffic"] = t
+            detail["traffic_over_algorithmic"] = t / per
+            detail["traffic_by_request_size_over_algorithmic"] = detail["traffic_by_request_size"] / per
+            out["roofline"]["traffic_detail"] = detail
+    # single-vector SpMV on the final H, and a plain 1 GiB device copy
     ms1, b1, _ = ctx64.time_spmm(1, args.spmm_reps)
-    bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
-    msb, bb, nl = ctx64.time_spmm(bs, args.spmm_reps)
-    out["spmv"] = {"ms": ms1, "bytes": b1, "GBs": b1 / ms1 / 1e6, "frac_of_hbm": b1 / ms1 / 1e6 / HBM_PEAK_GBS,
-                   "n": ctx64.n}
-    out["roofline"] = {"kernel": "k_spmm64<%d> (eigensolver block SpMM, %d vectors, FP64 CSR)" % (bs, bs),
-                       "bound": "hbm", "achieved": bb / msb / 1e6, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": bb / msb / 1e6 / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": msb / nl,
-                       "launches_per_product": nl, "algorithmic_bytes_per_launch": bb / nl}
+    out["spmv"] = {"kernel": "k_spmv<0>", "ms": ms1, "bytes": b1, "GBs": b1 / ms1 / 1e6,
+                   "frac_of_hbm": b1 / ms1 / 1e6 / HBM_PEAK_GBS, "n": ctx64.n}
     if "mixed" in res:
-        ctx32 = res["mixed"][1][0]
-        ms32, b32, _ = ctx32.time_spmm(1, args.spmm_reps, fp32=True)
-        out["spmv_fp32"] = {"ms": ms32, "bytes": b32, "GBs": b32 / ms32 / 1e6}
+        ms32, b32, _ = res["mixed"][1][0].time_spmm(1, args.spmm_reps, fp32=True)
+        out["spmv_fp32"] = {"kernel": "k_spmv<1>", "ms": ms32, "bytes": b32, "GBs": b32 / ms32 / 1e6}
+    cms = ctx64.copy_probe(1 << 30)
+    out["hbm_copy"] = {"kernel": "k_copy16", "bytes_moved": 2 * (1 << 30), "ms": cms,
+                       "GBs": 2.0 * (1 << 30) / cms / 1e6}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,
                                            args.cpu_threads)

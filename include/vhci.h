@@ -137,6 +137,13 @@ int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out);
  * calibration kernel for memory-counter readings); *ms its duration.
  * Measurement support; no counterpart in the reference. */
 int vhci_copy_probe(vhci_ctx* ctx, int64_t bytes, double* ms);
+/* Per-launch timing of the b-column SpMMs the eigensolver issues (HIP
+ * events around each launch on the context stream).  b > 0 starts and
+ * clears collection, b = 0 stops it.  The read gives out[0] = total ms,
+ * out[1] = total algorithmic bytes, out[2] = launches since the start.
+ * Measurement support; no counterpart in the reference. */
+int vhci_spmm_profile(vhci_ctx* ctx, int b);
+int vhci_spmm_profile_read(vhci_ctx* ctx, double* out);
 
 /* ---- whole solve (Perform_HCI + DoPT2 of one input) --------------------
  * Runs the reference's control flow (src/VHCI.cpp:39-114) on a device

@@ -59,6 +59,8 @@ _SIGS = {
     "vhci_pt2": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _dp, _i64p]),
     "vhci_time_spmm": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "vhci_copy_probe": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
+    "vhci_spmm_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "vhci_spmm_profile_read": (ctypes.c_int, [_vp, _dp]),
     "vhci_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _i64p, _i64p, _dp]),
     "vhci_run_ctx": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _ip, _i64p, _dp]),
     "vhci_iteration_sizes": (ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
@@ -289,6 +291,17 @@ class Context:
         """(ms per product, algorithmic bytes per product, launches per product)"""
         out = np.zeros(3)
         self._check(lib().vhci_time_spmm(self.h, b, reps, int(fp32), _ptr(out, _dp)))
+        return float(out[0]), float(out[1]), int(out[2])
+
+    def spmm_profile(self, b):
+        """Start (b > 0, clears) or stop (b = 0) per-launch timing of the
+        solver's b-column SpMMs."""
+        self._check(lib().vhci_spmm_profile(self.h, int(b)))
+
+    def spmm_profile_read(self):
+        """(total ms, total algorithmic bytes, launches) since the start"""
+        out = np.zeros(3)
+        self._check(lib().vhci_spmm_profile_read(self.h, _ptr(out, _dp)))
         return float(out[0]), float(out[1]), int(out[2])
 
     def copy_probe(self, nbytes):

@@ -507,6 +507,17 @@ int vhci_copy_probe(vhci_ctx* ctx, int64_t bytes, double* ms) {
     return guard(ctx, [&] { *ms = ctx->eng->copy_probe((size_t)bytes); });
 }
 
+int vhci_spmm_profile(vhci_ctx* ctx, int b) {
+    return guard(ctx, [&] {
+        if (b < 0) throw std::invalid_argument("vhci_spmm_profile: block width must be >= 0");
+        ctx->eng->spmm_profile(b);
+    });
+}
+
+int vhci_spmm_profile_read(vhci_ctx* ctx, double* out) {
+    return guard(ctx, [&] { ctx->eng->spmm_profile_read(out); });
+}
+
 int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
     return guard(ctx, [&] {
         double bytes = 0;

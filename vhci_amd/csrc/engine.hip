@@ -333,11 +333,9 @@ __global__ void k_diag(int64_t n, int M, const uint4* __restrict__ keys, const d
 }
 
 // ------------------------------------------------------------------------
-// SpMM: Y = (D + H_off) X, X/Y row-major; one wave per row, lanes split into
-// 64/B nonzero groups x B columns.  Columns are sorted within each row and
-// pp holds per-row offsets of column panels: with pp, one launch covers one
-// panel of columns (its slice of X stays in L2) and accumulates into Y
-// (the first panel also adds the diagonal term).
+// SpMM: Y = (D + H_off) X, X/Y row-major.  The general kernel k_spmm takes
+// one wave per row with the lanes split into 64/B entry groups x B columns;
+// k_spmm_lane (below) is the form for exactly 4, 8 or 16 contiguous columns.
 // MODE 0: FP64 values; 1: FP32 values and FP32 X (FP32 accumulation, the
 // diagonal term in FP64); 2: FP64 value = hi + lo.
 // ------------------------------------------------------------------------
@@ -374,7 +372,6 @@ __device__ __forceinline__ typename SpAcc<MODE>::T sp_x(const double* __restrict
 
 template <int B, int MODE>
 __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restrict__ rowptr,
-                                              const int32_t* __restrict__ pp, int P1, int panel, int first,
                                               const int32_t* __restrict__ col, const double* __restrict__ val,
                                               const float* __restrict__ hi, const float* __restrict__ lo,
                                               const double* __restrict__ D, const float* __restrict__ X32, int ldx32,
@@ -386,12 +383,7 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
     const int grp = lane / B, c = lane % B;
-    int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
-    if (pp) {
-        const int32_t* q = pp + row * P1 + panel;
-        k1 = k0 + q[1];
-        k0 = k0 + q[0];
-    }
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
     const double d = D[row];
     for (int c0 = 0; c0 < b; c0 += B) {
         const int cc = c0 + c;
@@ -419,10 +411,113 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
         AT acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
         for (int off = B; off < kWave; off <<= 1) acc += __shfl_xor(acc, off);
-        if (grp == 0 && act) {
-            const double base = first ? d * X[row * ldx + cc] : Y[row * ldy + cc];
-            Y[row * ldy + cc] = base + (double)acc;
+        if (grp == 0 && act) Y[row * ldy + cc] = d * X[row * ldx + cc] + (double)acc;
+    }
+}
+
+// Block product for exactly B contiguous row-major columns (ldx == B, rows
+// 16-byte aligned): one wave per row, each lane owns one stored entry per
+// step, so the column and value streams are read with full-width coalesced
+// loads (256 B of columns and 512 B of FP64 values per wave instruction)
+// and each lane gathers the whole B-wide row of X with 16-byte loads.  Two
+// entries per lane are in flight per step.  The B per-lane partial sums are
+// reduced by a transposing butterfly: at each of the first log2(B) steps a
+// lane keeps half of its columns and trades the other half with its
+// partner, so after 6 steps lane l holds column l / (64 / B).  The
+// summation order is fixed, so products are reproducible bit for bit.
+template <int B, int MODE>
+__device__ __forceinline__ void sp_xrow(const double* __restrict__ X, const float* __restrict__ X32, int64_t j,
+                                        typename SpAcc<MODE>::T (&x)[B]) {
+    if constexpr (MODE == 1) {
+        const float4* p = reinterpret_cast<const float4*>(X32 + j * B);
+#pragma unroll
+        for (int q = 0; q < B / 4; q++) {
+            const float4 t = p[q];
+            x[4 * q] = t.x;
+            x[4 * q + 1] = t.y;
+            x[4 * q + 2] = t.z;
+            x[4 * q + 3] = t.w;
         }
+    } else {
+        const double2* p = reinterpret_cast<const double2*>(X + j * B);
+#pragma unroll
+        for (int q = 0; q < B / 2; q++) {
+            const double2 t = p[q];
+            x[2 * q] = t.x;
+            x[2 * q + 1] = t.y;
+        }
+    }
+}
+
+// dot products of one row segment [k0, k1) with the B columns of X, one
+// stored entry per lane per step (two in flight); returns, in lane l, the
+// full wave sum of column l / (64 / B) after the transposing butterfly
+template <int B, int MODE>
+__device__ __forceinline__ typename SpAcc<MODE>::T sp_lane_dot(int64_t k0, int64_t k1, int lane,
+                                                             const int32_t* __restrict__ col,
+                                                             const double* __restrict__ val,
+                                                             const float* __restrict__ hi,
+                                                             const float* __restrict__ lo,
+                                                             const float* __restrict__ X32,
+                                                             const double* __restrict__ X) {
+    using AT = typename SpAcc<MODE>::T;
+    static_assert(B == 4 || B == 8 || B == 16, "block width");
+    AT acc[B];
+#pragma unroll
+    for (int c = 0; c < B; c++) acc[c] = 0;
+    int64_t k = k0 + lane;
+    for (; k + 64 < k1; k += 128) {
+        const int64_t j0 = __builtin_nontemporal_load(col + k);
+        const int64_t j1 = __builtin_nontemporal_load(col + k + 64);
+        const AT v0 = sp_val<MODE>(val, hi, lo, k), v1 = sp_val<MODE>(val, hi, lo, k + 64);
+        AT x0[B], x1[B];
+        sp_xrow<B, MODE>(X, X32, j0, x0);
+        sp_xrow<B, MODE>(X, X32, j1, x1);
+#pragma unroll
+        for (int c = 0; c < B; c++) acc[c] = sp_fma(v0, x0[c], acc[c]);
+#pragma unroll
+        for (int c = 0; c < B; c++) acc[c] = sp_fma(v1, x1[c], acc[c]);
+    }
+    if (k < k1) {
+        const int64_t j0 = __builtin_nontemporal_load(col + k);
+        const AT v0 = sp_val<MODE>(val, hi, lo, k);
+        AT x0[B];
+        sp_xrow<B, MODE>(X, X32, j0, x0);
+#pragma unroll
+        for (int c = 0; c < B; c++) acc[c] = sp_fma(v0, x0[c], acc[c]);
+    }
+    // transposing butterfly over lane bits 5, 4, ...: keep half, trade half
+#pragma unroll
+    for (int h = B / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int i = 0; i < h; i++) {
+            const AT send = up ? acc[i] : acc[i + h];
+            const AT keep = up ? acc[i + h] : acc[i];
+            acc[i] = keep + __shfl_xor(send, bit);
+        }
+    }
+    constexpr int R = 64 / B;  // lanes that share one column after the butterfly
+    AT s = acc[0];
+#pragma unroll
+    for (int off = R / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+    return s;
+}
+
+template <int B, int MODE>
+__global__ __launch_bounds__(256) void k_spmm_lane(int64_t n, const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                   const float* __restrict__ hi, const float* __restrict__ lo,
+                                                   const double* __restrict__ D, const float* __restrict__ X32,
+                                                   const double* __restrict__ X, double* __restrict__ Y, int ldy) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const auto s = sp_lane_dot<B, MODE>(rowptr[row], rowptr[row + 1], lane, col, val, hi, lo, X32, X);
+    constexpr int R = 64 / B;
+    if ((lane & (R - 1)) == 0) {
+        const int c = lane / R;
+        Y[row * ldy + c] = D[row] * X[row * B + c] + (double)s;
     }
 }
 
@@ -474,28 +569,6 @@ __global__ void k_unpack_hilo(int64_t nnz, const uint64_t* __restrict__ in, floa
         hi[k] = __uint_as_float((uint32_t)in[k]);
         lo[k] = __uint_as_float((uint32_t)(in[k] >> 32));
     }
-}
-
-// per-row offsets of the column panels [p*w, (p+1)*w) in a sorted row
-__global__ void k_panel_ptr(int64_t n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col, int P,
-                            int w, int32_t* __restrict__ pp) {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t k0 = rowptr[i], k1 = rowptr[i + 1];
-    pp[i * (P + 1)] = 0;
-    for (int p = 1; p < P; p++) {
-        const int32_t bound = p * w;
-        int64_t lo = k0, hi = k1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (col[mid] < bound)
-                lo = mid + 1;
-            else
-                hi = mid;
-        }
-        pp[i * (P + 1) + p] = (int32_t)(lo - k0);
-    }
-    pp[i * (P + 1) + P] = (int32_t)(k1 - k0);
 }
 
 // ------------------------------------------------------------------------
@@ -1414,6 +1487,7 @@ Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), dev
 }
 
 Engine::~Engine() {
+    for (hipEvent_t e : prof_.ev) (void)hipEventDestroy(e);
     if (blas_) rocblas_destroy_handle(blas_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1541,7 +1615,7 @@ void Engine::build_h_t(HStats& st) {
         k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     }
-    // columns ascending within each row; per-row offsets of 32k-column panels
+    // columns ascending within each row
     if (nnz > 0) {
         unsigned endbit = 1;
         while (endbit < 31 && ((uint64_t)(n - 1) >> endbit) != 0) endbit++;
@@ -1572,16 +1646,8 @@ void Engine::build_h_t(HStats& st) {
         else
             k_unpack_hilo<<<grid_for(nnz, 256), 256, 0, stream_>>>(nnz, vout.as<uint64_t>(), hi_.as<float>(),
                                                                    lo_.as<float>());
-        npanel_ = (int)((n + kPanel - 1) / kPanel);
-        if (npanel_ > 1) {
-            pp_.ensure((size_t)n * (npanel_ + 1) * sizeof(int32_t));
-            k_panel_ptr<<<grid_for(n, 256), 256, 0, stream_>>>(n, rp, col_.as<int32_t>(), npanel_, kPanel,
-                                                               pp_.as<int32_t>());
-        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream_));
-    } else {
-        npanel_ = 1;
     }
     D_.ensure(n * sizeof(double));
     k_diag<NW><<<grid_for(n, 128), 128, 0, stream_>>>(n, P_.M, a.keys, d_freq_.as<double>(), d_diag_.as<uint32_t>(),
@@ -1666,10 +1732,7 @@ int64_t Engine::structural_nnz() {
 }
 
 // ---------------------------------------------------------------- SpMM
-int Engine::spmm_launches(int b) const {
-    static const bool panels = std::getenv("VHCI_SPMM_PANELS") != nullptr;
-    return (panels && b >= 2 && npanel_ > 1) ? npanel_ : 1;
-}
+int Engine::spmm_launches(int) const { return 1; }
 
 void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32, int ldx32) {
     const int64_t n = n_;
@@ -1695,28 +1758,58 @@ void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low,
         return;
     }
     const unsigned grid = grid_for(n, 4);
-    const int P = spmm_launches(b);
-    const int32_t* pp = P > 1 ? pp_.as<int32_t>() : nullptr;
-    for (int p = 0; p < P; p++) {
+    const bool rec = prof_.b > 0 && b == prof_.b;
+    if (rec) {
+        if (prof_.used + 2 > prof_.ev.size()) {
+            for (int i = 0; i < 64; i++) {
+                hipEvent_t e;
+                HIPCHK(hipEventCreate(&e));
+                prof_.ev.push_back(e);
+            }
+        }
+        HIPCHK(hipEventRecord(prof_.ev[prof_.used], stream_));
+        prof_.bytes += spmm_bytes(b, mode);
+    }
+    struct RecEnd {
+        Engine* e;
+        bool on;
+        ~RecEnd() {
+            if (on) {
+                (void)hipEventRecord(e->prof_.ev[e->prof_.used + 1], e->stream_);
+                e->prof_.used += 2;
+            }
+        }
+    } rec_end{this, rec};
+    const bool dense_x = ldx == b && ((uintptr_t)X & 15) == 0 &&
+                         (mode != 1 || (ldx32 == b && ((uintptr_t)X32 & 15) == 0));
+    if (dense_x && (b == 4 || b == 8 || b == 16)) {
         auto go = [&](auto tag) {
             constexpr int B = decltype(tag)::value;
-            if (mode == 0)
-                k_spmm<B, 0><<<grid, 256, 0, stream_>>>(n, rp, pp, npanel_ + 1, p, p == 0, cl, vl, h, l, D, X32,
-                                                        ldx32, X, ldx, Y, ldy, b);
-            else if (mode == 1)
-                k_spmm<B, 1><<<grid, 256, 0, stream_>>>(n, rp, pp, npanel_ + 1, p, p == 0, cl, vl, h, l, D, X32,
-                                                        ldx32, X, ldx, Y, ldy, b);
-            else
-                k_spmm<B, 2><<<grid, 256, 0, stream_>>>(n, rp, pp, npanel_ + 1, p, p == 0, cl, vl, h, l, D, X32,
-                                                        ldx32, X, ldx, Y, ldy, b);
+            if (mode == 0) k_spmm_lane<B, 0><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
+            else if (mode == 1) k_spmm_lane<B, 1><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
+            else k_spmm_lane<B, 2><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
         };
-        if (b <= 2) go(std::integral_constant<int, 2>());
-        else if (b <= 4) go(std::integral_constant<int, 4>());
-        else if (b <= 8) go(std::integral_constant<int, 8>());
-        else if (b <= 16) go(std::integral_constant<int, 16>());
-        else if (b <= 32) go(std::integral_constant<int, 32>());
-        else go(std::integral_constant<int, 64>());
+        if (b == 4) go(std::integral_constant<int, 4>());
+        else if (b == 8) go(std::integral_constant<int, 8>());
+        else go(std::integral_constant<int, 16>());
+        HIPCHK(hipGetLastError());
+        return;
     }
+    auto go = [&](auto tag) {
+        constexpr int B = decltype(tag)::value;
+        if (mode == 0)
+            k_spmm<B, 0><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, ldx32, X, ldx, Y, ldy, b);
+        else if (mode == 1)
+            k_spmm<B, 1><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, ldx32, X, ldx, Y, ldy, b);
+        else
+            k_spmm<B, 2><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, ldx32, X, ldx, Y, ldy, b);
+    };
+    if (b <= 2) go(std::integral_constant<int, 2>());
+    else if (b <= 4) go(std::integral_constant<int, 4>());
+    else if (b <= 8) go(std::integral_constant<int, 8>());
+    else if (b <= 16) go(std::integral_constant<int, 16>());
+    else if (b <= 32) go(std::integral_constant<int, 32>());
+    else go(std::integral_constant<int, 64>());
     HIPCHK(hipGetLastError());
 }
 
@@ -1768,10 +1861,37 @@ double Engine::time_spmm(int b, int reps, bool fp32, double* bytes_per_call) {
     HIPCHK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    const double sv = low ? 4.0 : 8.0;
-    const double sx = low ? 4.0 : 8.0;
-    if (bytes_per_call) *bytes_per_call = (double)nnz_ * (sv + 4) + 8.0 * (n + 1) + 2.0 * n * b * sx + 8.0 * n;
+    if (bytes_per_call) *bytes_per_call = spmm_bytes(b, form_ == kFP64 ? 0 : (low ? 1 : 2));
     return ms / reps;
+}
+
+double Engine::spmm_bytes(int b, int mode) const {
+    const double n = (double)n_;
+    const double sv = mode == 1 ? 4.0 : 8.0;  // stored value bytes (hi + lo counts 8)
+    const double sx = mode == 1 ? 4.0 : 8.0;  // gathered X element bytes
+    // FP32 products also read the FP64 X row of the diagonal term
+    const double xdiag = mode == 1 ? n * b * 8.0 : 0.0;
+    return (double)nnz_ * (sv + 4) + 8.0 * (n + 1) + 8.0 * n + n * b * sx + n * b * 8.0 + xdiag;
+}
+
+void Engine::spmm_profile(int b) {
+    HIPCHK(hipStreamSynchronize(stream_));
+    prof_.b = b;
+    prof_.used = 0;
+    prof_.bytes = 0;
+}
+
+void Engine::spmm_profile_read(double out[3]) {
+    HIPCHK(hipStreamSynchronize(stream_));
+    double ms = 0;
+    for (size_t i = 0; i + 1 < prof_.used; i += 2) {
+        float t = 0;
+        HIPCHK(hipEventElapsedTime(&t, prof_.ev[i], prof_.ev[i + 1]));
+        ms += t;
+    }
+    out[0] = ms;
+    out[1] = prof_.bytes;
+    out[2] = (double)(prof_.used / 2);
 }
 
 // ---------------------------------------------------------------- eigensolver
@@ -2273,6 +2393,19 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
             apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st);
             HIPCHK(hipMemcpyAsync(V + (int64_t)(j + bs) * ld, W + (int64_t)j * ld, (size_t)n * bs * sizeof(double),
                                   hipMemcpyDeviceToDevice, stream_));
+            if (j > k) {
+                // Lanczos step: H V_j has large components only along V_j and
+                // V_(j-bs) (the block-tridiagonal part of the projected
+                // matrix; the kept Ritz vectors couple to V_k alone).  Taking
+                // those out first leaves the full pass below a block that is
+                // already nearly orthogonal, so one projection suffices.
+                const int lo = std::max(k, j - bs);
+                const int p = j + bs - lo;
+                S_.ensure((size_t)std::max(p * bs, (j + bs) * bs) * sizeof(double));
+                double* T = V + (int64_t)(j + bs) * ld;
+                tsm_tn(stream_, tsm_part_, n, p, bs, V + (int64_t)lo * ld, ld, T, ld, S_.as<double>(), p);
+                tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
+            }
             int kept = ortho_cm(V, ld, j + bs, bs);
             for (int tries = 0; kept < bs && tries < 4; tries++) {
                 // invariant subspace reached for some directions: refill

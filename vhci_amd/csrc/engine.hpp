@@ -110,9 +110,16 @@ class Engine {
     double time_spmm(int b, int reps, bool fp32, double* bytes_per_call);
     // a plain copy of `bytes` bytes on the context stream; returns ms
     double copy_probe(size_t bytes);
-    // kernel launches per SpMM of b vectors (one per column panel)
+    // kernel launches per SpMM of b vectors
     int spmm_launches(int b) const;
-    static constexpr int kPanel = 32768;  // columns per panel of the block SpMM
+    // per-launch timing of the b-column products that the solver issues:
+    // b > 0 starts (and clears) collection, b = 0 stops it.  The read gives
+    // total ms, total algorithmic bytes and the launch count since the start.
+    void spmm_profile(int b);
+    void spmm_profile_read(double out[3]);
+    // algorithmic bytes of one product of b vectors: stored entries (value +
+    // column), row pointers, the diagonal, X read once and Y written once
+    double spmm_bytes(int b, int mode) const;
 
   private:
     void upload_tables();
@@ -154,8 +161,7 @@ class Engine {
     // Hamiltonian
     bool h_valid_ = false;
     int64_t nnz_ = 0;
-    DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_, pp_;
-    int npanel_ = 1;
+    DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_;
     // Ritz block (row-major n x xcols_, ld = xcols_)
     DevBuf X_;
     int64_t xrows_ = 0;
@@ -172,6 +178,12 @@ class Engine {
     // PT2 work
     DevBuf s2keys_, pt2_part_, C32_;
     double last_spmm_ms_ = 0;
+    struct SpmmProf {
+        int b = 0;
+        std::vector<hipEvent_t> ev;  // start, end pairs
+        size_t used = 0;
+        double bytes = 0;
+    } prof_;
 };
 
 }  // namespace vhci
