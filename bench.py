@@ -13,6 +13,13 @@ refinement, FP64 PT2 denominators and Kahan sums).  The headline value is
 the mixed form's solves/s when every energy is within 0.01 cm^-1 of FP64
 and the selected basis is identical, else the FP64 figure.
 
+The serving entry runs a stream of small requests in the style of the
+serving workload (seeded fields of 8-14 modes, eps1 1 or 2, Nstate 20-200,
+eps2 0, 0.01 or 0.1): each request alone first (the solo FP64 rate), then
+Poisson arrivals at 1.2x that rate through a four-slot server, once per
+precision form, with p50/p99 latency from submission to result and a check
+that every result equals its solo run.
+
 The roofline entry is for the FP64 eigensolver's block SpMM (k_spmm_lane),
 every launch of the timed solves bracketed by HIP events on the context's
 stream; with --traffic its HBM bytes per launch come from two rocprofv3
@@ -89,7 +96,8 @@ def traffic_pass(counters, kernel, timeout_s=170):
     d = tempfile.mkdtemp(prefix="vhci_pmc_")
     cmd = ["rocprofv3", "--pmc"] + counters + ["--output-format", "csv", "-d", d, "-o", "pmc", "--",
                                                sys.executable, "-u", os.path.abspath(__file__), "--steps", "1",
-                                               "--warmup", "0", "--forms", "fp64", "--no-cpu-baseline"]
+                                               "--warmup", "0", "--forms", "fp64", "--no-cpu-baseline",
+                                               "--serve", "0"]
     print("[bench] counter pass: %s" % " ".join(cmd), file=sys.stderr, flush=True)
     log = os.path.join(d, "pass.log")
     with open(log, "w") as lf:
@@ -195,6 +203,68 @@ def cpu_baseline(ctx, r, problem_text, threads):
     }
 
 
+SERVE_RANGES = dict(m_range=(8, 14), eps1_set=(1.0, 2.0), nstate_range=(20, 200), eps2_set=(0.0, 0.01, 0.1))
+
+
+def serving(nreq, slots, load, seed=7):
+    """Serving stream in the style of workload c3 with narrowed ranges
+    (SERVE_RANGES): each request alone on a one-slot server sets the solo
+    FP64 rate; then Poisson arrivals at `load` x that rate go through a
+    `slots`-slot server, once per form.  Latency = submission to result."""
+    import threading
+
+    texts = [synth.request_input(r, **SERVE_RANGES) for r in range(nreq)]
+    srv = vhci_amd.Server(slots=1)
+    alone = []
+    for t in texts:
+        rid = srv.submit(t)
+        srv.wait(rid)
+        alone.append(srv.result(rid))
+    srv.close()
+    service = [a["t_end"] - a["t_start"] for a in alone]
+    rate = load * nreq / sum(service)
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    out = {"requests": nreq, "slots": slots, "ranges": {k: list(v) for k, v in SERVE_RANGES.items()},
+           "solo_fp64_solves_per_s": nreq / sum(service), "offered_rate_per_s": rate,
+           "basis_sizes": [a["nvar"] for a in alone], "pt2_sizes": [a["ns2"] for a in alone]}
+    for name, form in (("fp64", vhci_amd.FP64), ("mixed", vhci_amd.MIXED)):
+        srv = vhci_amd.Server(slots=slots)
+        ids = [None] * nreq
+
+        def feed():
+            t0 = time.perf_counter()
+            due = 0.0
+            for i, t in enumerate(texts):
+                due += gaps[i]
+                dt = due - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                ids[i] = srv.submit(t, form=form)
+
+        th = threading.Thread(target=feed)
+        th.start()
+        th.join()
+        res = []
+        for rid in ids:
+            srv.wait(rid)
+            res.append(srv.result(rid))
+        st = srv.stats()
+        srv.close()
+        lat = [r["t_end"] - r["t_submit"] for r in res]
+        span = max(r["t_end"] for r in res) - min(r["t_submit"] for r in res)
+        same = sum(1 for a, r in zip(alone, res) if a["checksum"] == r["checksum"])
+        err = max(float(np.max(np.abs(a["E"] - r["E"]))) for a, r in zip(alone, res))
+        out[name] = {"solves_per_s": nreq / span, "p50_latency_s": percentile(lat, 50),
+                     "p99_latency_s": percentile(lat, 99), "max_concurrent": st["max_running"],
+                     "basis_identical_to_alone_fp64": same, "max_abs_err_vs_alone_fp64_cm": err}
+        print("[bench] serving %s: %.2f solves/s p50 %.3f s p99 %.3f s" % (name, nreq / span, out[name]["p50_latency_s"],
+                                                                       out[name]["p99_latency_s"]),
+              file=sys.stderr, flush=True)
+    out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,6 +274,9 @@ def main():
     ap.add_argument("--spmm-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--serve", type=int, default=32, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-load", type=float, default=1.2, help="offered load / solo FP64 rate")
     ap.add_argument("--traffic", action="store_true",
                     help="also measure the SpMM kernel's HBM traffic with two rocprofv3 counter passes "
                          "(child processes, run before this process uses the GPU)")
@@ -303,6 +376,8 @@ def main():
     cms = ctx64.copy_probe(1 << 30)
     out["hbm_copy"] = {"kernel": "k_copy16", "bytes_moved": 2 * (1 << 30), "ms": cms,
                        "GBs": 2.0 * (1 << 30) / cms / 1e6}
+    if args.serve > 0:
+        out["serving"] = serving(args.serve, args.serve_slots, args.serve_load)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,
                                            args.cpu_threads)

@@ -170,6 +170,39 @@ int vhci_run_ctx(vhci_ctx* ctx, double tol, int restart, double* E, double* Evar
  * this context (the initial size first); returns the count written. */
 int vhci_iteration_sizes(const vhci_ctx* ctx, int64_t* sizes, int max);
 
+/* ---- serving loop ----------------------------------------------------
+ * No counterpart in the reference (one input per process, src/VHCI.cpp).
+ * Requests (input-file text, precision form, tolerance) are queued FIFO and
+ * admitted to one of `slots` worker threads when a slot is free and their
+ * predicted device bytes fit under mem_limit next to the reservations of
+ * the running requests (a request is always admitted when none runs).  A
+ * running request owns a device context on its own stream for its whole
+ * solve; its reservation is revised after every basis growth.  Each
+ * request runs the same loop as vhci_run_ctx, so its result does not
+ * depend on what else runs. */
+typedef struct vhci_server vhci_server;
+int vhci_server_create(int device, int slots, int64_t mem_limit, vhci_server** out);
+/* Waits for the running requests; queued ones end as failed. */
+void vhci_server_destroy(vhci_server* s);
+const char* vhci_server_error(const vhci_server* s);
+/* Parses the input (VHCI_E_INPUT with the reference's message when it is
+ * bad) and queues it; *id receives the request id. */
+int vhci_server_submit(vhci_server* s, const char* text, int form, double tol, int64_t* id);
+/* *done = 1 when the request finished within timeout_s (< 0: no limit). */
+int vhci_server_wait(vhci_server* s, int64_t id, double timeout_s, int* done);
+/* Result of a finished request: E/Evar (nullable, room for Nstate values),
+ * *nE; info (nullable, 6): state (0 queued, 1 running, 2 done, 3 failed),
+ * final basis size, PT2 space size, heat-bath iterations, basis checksum
+ * (as vhci_run_ctx), worker slot; times (nullable, 6): submit, start and
+ * end time in seconds since the server started, predicted bytes at
+ * submission, measured peak device bytes, final reservation.  A failed
+ * request returns VHCI_E_* with its message in vhci_server_error(). */
+int vhci_server_result(vhci_server* s, int64_t id, double* E, double* Evar, int* nE, int64_t* info,
+                       double* times);
+/* out[5]: running, queued, reserved bytes, most requests run at once,
+ * finished. */
+int vhci_server_stats(vhci_server* s, double* out);
+
 /* The drop-in program: same flags, input, freqs.dat and checkpoint formats
  * and log lines as the reference's `vhci -n -i -o -c`. argv as in main. */
 int vhci_main(int argc, char** argv);

@@ -64,6 +64,13 @@ _SIGS = {
     "vhci_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _i64p, _i64p, _dp]),
     "vhci_run_ctx": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _ip, _i64p, _dp]),
     "vhci_iteration_sizes": (ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
+    "vhci_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int64, ctypes.POINTER(_vp)]),
+    "vhci_server_destroy": (None, [_vp]),
+    "vhci_server_error": (ctypes.c_char_p, [_vp]),
+    "vhci_server_submit": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_double, _i64p]),
+    "vhci_server_wait": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_double, _ip]),
+    "vhci_server_result": (ctypes.c_int, [_vp, ctypes.c_int64, _dp, _dp, _ip, _i64p, _dp]),
+    "vhci_server_stats": (ctypes.c_int, [_vp, _dp]),
     "vhci_main": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p)]),
 }
 
@@ -333,3 +340,68 @@ def solve(problem, form=FP64, device=0, tol=1e-12):
         raise VhciError(L.vhci_problem_error().decode())
     keys = ["total_ms", "build_ms", "eig_ms", "hb_ms", "pt2_select_ms", "pt2_sum_ms", "iterations", "spmm_vectors"]
     return {"E": E[: nE.value].copy(), "nvar": nvar.value, "ns2": ns2.value, "timings": dict(zip(keys, t.tolist()))}
+
+
+class Server:
+    """The serving loop: requests (input text, form, tolerance) queue FIFO
+    and run on `slots` worker threads, each request on its own device
+    context and stream, admitted when a slot is free and its predicted device
+    bytes fit under mem_limit next to the running requests' reservations."""
+
+    STATES = ("queued", "running", "done", "failed")
+
+    def __init__(self, device=0, slots=4, mem_limit=200 << 30):
+        L = lib()
+        h = _vp()
+        rc = L.vhci_server_create(device, slots, int(mem_limit), ctypes.byref(h))
+        if rc != 0:
+            raise VhciError(L.vhci_problem_error().decode())
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().vhci_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise VhciError(lib().vhci_server_error(self.h).decode())
+
+    def submit(self, text, form=FP64, tol=0.0):
+        rid = ctypes.c_int64()
+        self._check(lib().vhci_server_submit(self.h, text.encode(), form, tol, ctypes.byref(rid)))
+        return rid.value
+
+    def wait(self, rid, timeout=-1.0):
+        done = ctypes.c_int()
+        self._check(lib().vhci_server_wait(self.h, rid, float(timeout), ctypes.byref(done)))
+        return bool(done.value)
+
+    def result(self, rid):
+        """Record of a request; raises VhciError if it failed."""
+        E = np.zeros(1024)
+        Ev = np.zeros(1024)
+        nE = ctypes.c_int()
+        info = np.zeros(6, dtype=np.int64)
+        t = np.zeros(6)
+        rc = lib().vhci_server_result(self.h, rid, _ptr(E, _dp), _ptr(Ev, _dp), ctypes.byref(nE), _ptr(info, _i64p),
+                                      _ptr(t, _dp))
+        self._check(rc)
+        return {"state": self.STATES[int(info[0])], "E": E[: nE.value].copy(), "E_var": Ev[: nE.value].copy(),
+                "nvar": int(info[1]), "ns2": int(info[2]), "hb_iterations": int(info[3]),
+                "checksum": int(info[4]) & ((1 << 64) - 1), "slot": int(info[5]), "t_submit": float(t[0]),
+                "t_start": float(t[1]), "t_end": float(t[2]), "predicted_bytes": float(t[3]),
+                "peak_bytes": float(t[4]), "reserved_bytes": float(t[5])}
+
+    def stats(self):
+        out = np.zeros(5)
+        self._check(lib().vhci_server_stats(self.h, _ptr(out, _dp)))
+        return {"running": int(out[0]), "queued": int(out[1]), "reserved_bytes": float(out[2]),
+                "max_running": int(out[3]), "finished": int(out[4])}
+

@@ -18,7 +18,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-I", os.path.join(ROOT, "include")]
 DEVICE = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES = ["problem.cpp", "engine.hip", "abi.cpp"]
+SOURCES = ["problem.cpp", "engine.hip", "driver.cpp", "server.cpp", "abi.cpp"]
 
 
 def run(cmd):

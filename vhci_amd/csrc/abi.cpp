@@ -3,6 +3,7 @@
 // The driver keeps the reference's control flow and log lines
 // (src/VHCI.cpp:5-185, src/HB.cpp:176-265, src/PT2.cpp:16-154) and replaces
 // its numerical kernels by the device engine.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +21,8 @@
 #include <vector>
 
 #include "../../include/vhci.h"
+#include "driver.hpp"
+#include "server.hpp"
 #include "engine.hpp"
 
 using vhci::Engine;
@@ -27,6 +30,11 @@ using vhci::Problem;
 
 struct vhci_problem {
     Problem P;
+};
+
+struct vhci_server {
+    std::unique_ptr<vhci::srv::Server> s;
+    std::string err;
 };
 
 struct vhci_ctx {
@@ -51,6 +59,22 @@ int classify(const std::string& m) {
 }
 
 template <class F>
+int sguard(vhci_server* s, F&& f) {
+    if (!s || !s->s) return VHCI_E_ARG;
+    try {
+        f();
+        s->err.clear();
+        return VHCI_OK;
+    } catch (const std::invalid_argument& e) {
+        s->err = e.what();
+        return VHCI_E_INPUT;
+    } catch (const std::exception& e) {
+        s->err = e.what();
+        return classify(s->err);
+    }
+}
+
+template <class F>
 int guard(vhci_ctx* ctx, F&& f) {
     if (!ctx || !ctx->eng) return VHCI_E_ARG;
     try {
@@ -63,182 +87,11 @@ int guard(vhci_ctx* ctx, F&& f) {
     }
 }
 
-// ---------------------------------------------------------------------------
-// the iterative select -> build -> diagonalize -> PT2 loop
-// ---------------------------------------------------------------------------
-struct RunOpts {
-    double tol = 1e-12;
-    int maxit = 500;
-    std::string cpname = "none";
-    bool restart = false;
-    bool log = false;
-    int stop_after = 0;  // failure injection: end the run after this HCI iteration
-};
-
-struct RunResult {
-    std::vector<double> E;  // final energies (after PT2)
-    std::vector<double> Evar;
-    int64_t nvar = 0;
-    int64_t ns2 = 0;
-    double t_build = 0, t_eig = 0, t_hb = 0, t_pt2sel = 0, t_pt2sum = 0;
-    int iters = 0;
-    int hb_iters = 0;
-    int64_t spmm_vectors = 0;
-    std::vector<int64_t> sizes;
-};
-
-struct Logger {
-    bool on;
-    void operator()(const std::string& s) const {
-        if (on) {
-            std::cout << s;
-            std::cout.flush();
-        }
-    }
-};
-
-std::string fmt_fixed(double v, int prec) {
-    std::ostringstream os;
-    os << std::fixed << std::setprecision(prec) << v;
-    return os.str();
-}
-
-std::string sparse_line(const vhci::HStats& h) {
-    // "% sparse" (Ham.cpp:341-342) from the structural count when it was
-    // computed, else from the stored entries plus the diagonal.
-    double nnz = h.nnz_struct >= 0 ? (double)h.nnz_struct : (double)(h.nnz_offdiag + h.n);
-    double pct = 100 * (1. - nnz / ((double)h.n * (double)h.n));
-    return "The Hamiltonian is " + fmt_fixed(pct, 2) + "% sparse.\n";
-}
-
-void save_checkpoint(Engine& eng, const Problem& P, const std::vector<double>& E, const RunOpts& o,
-                     const Logger& log) {
-    std::ofstream cf(o.cpname, std::ios::out | std::ios::trunc);
-    if (!cf.good() || o.cpname == "none") {
-        log("\n Error: checkpoint file could not be found or no name was given.\n\n");
-        return;
-    }
-    const int64_t n = eng.n();
-    const int nc = eng.ritz_cols();
-    std::vector<int32_t> q((size_t)n * P.M);
-    std::vector<double> C((size_t)n * nc);
-    eng.get_basis(q.data());
-    eng.get_vectors(C.data(), nc);
-    vhci::write_checkpoint(cf, P, q.data(), n, E, C.data(), nc);
-}
-
-void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Logger& log) {
-    std::vector<double> E;
-    R.sizes.push_back(eng.n());
-    if (P.eps1 == 0.) {
-        // one-shot VCI (VHCI.cpp:49-76)
-        if (!o.restart) {
-            log("Constructing the Vibrational CI Hamiltonian...");
-            auto hs = eng.build_h();
-            R.t_build += hs.build_ms;
-            if (log.on) hs.nnz_struct = eng.structural_nnz();
-            log(sparse_line(hs));
-            if (P.neig == 1)
-                log("done.\n\nSolving for the ground state...\n\n");
-            else
-                log("done.\n\nSolving for the lowest " + std::to_string(P.neig) + " eigenstates...\n\n");
-            auto es = eng.eigsolve(P.neig, o.tol, o.maxit, E);
-            R.t_eig += es.ms;
-            R.spmm_vectors += es.spmm_vectors;
-            if (eng.n() > 100000) {
-                log("Writing checkpoint file...");
-                save_checkpoint(eng, P, E, o, log);
-                log("done.\n\n");
-            }
-        } else {
-            E = R.E;
-        }
-    } else {
-        // Perform_HCI (HB.cpp:176-265)
-        log("Beginning Heat Bath CI procedure...\n");
-        int iter = 1;
-        if (!o.restart) {
-            log(" Initial basis size is: " + std::to_string(eng.n()) + "\n");
-            log(" Constructing Hamiltonian...");
-            auto hs = eng.build_h();
-            R.t_build += hs.build_ms;
-            log(" Diagonalizing...\n");
-            auto es = eng.dense_all(E);
-            R.t_eig += es.ms;
-            log(" ZPE is: " + fmt_fixed(E[0], 2) + "\n\n");
-        } else {
-            E = R.E;
-        }
-        while (true) {
-            log(" Performing HCI iteration: " + std::to_string(iter) + "\n");
-            const int64_t n = eng.n();
-            const int nopt = (int)std::min<int64_t>(P.neig, n);
-            auto hb = eng.hb_select(nullptr, nopt, P.eps1);
-            R.t_hb += hb.ms;
-            R.hb_iters++;
-            if ((double)eng.new_count() / (double)n <= 0.01) {
-                log(" Basis converged after " + std::to_string(iter - 1) + " iterations and contains " +
-                    std::to_string(n) + " product states.\n");
-                break;
-            }
-            eng.append_new_states();
-            R.sizes.push_back(eng.n());
-            log(" Basis size after iteration " + std::to_string(iter) + " is: " + std::to_string(eng.n()) + "\n");
-            ++iter;
-            log(" Constructing Hamiltonian...");
-            auto hs = eng.build_h();
-            R.t_build += hs.build_ms;
-            if (log.on) hs.nnz_struct = eng.structural_nnz();
-            log(sparse_line(hs));
-            if (P.neig == 1)
-                log(" Finding the ground state...\n");
-            else
-                log(" Finding the first " + std::to_string(P.neig) + " eigenstates...\n");
-            auto es = eng.eigsolve(P.neig, o.tol, o.maxit, E);
-            R.t_eig += es.ms;
-            R.spmm_vectors += es.spmm_vectors;
-            R.iters++;
-            log(" ZPE is: " + fmt_fixed(E[0], 2) + "\n\n");
-            if (eng.n() > 100000) {
-                log(" Writing checkpoint file...");
-                save_checkpoint(eng, P, E, o, log);
-                log("done.\n\n");
-            }
-            if (o.stop_after > 0 && iter - 1 >= o.stop_after)
-                throw std::runtime_error("stopped after heat-bath iteration " + std::to_string(iter - 1) +
-                                         " (VHCI_STOP_AFTER_ITERATION)");
-        }
-    }
-    R.Evar = E;
-    R.nvar = eng.n();
-    if (P.perturb) {
-        const int64_t n = eng.n();
-        const int nopt = (int)std::min<int64_t>(P.neig, n);
-        std::vector<double> dE;
-        if (P.eps2 > 0) {
-            std::ostringstream os;
-            os << " Using heat bath sorting of perturbative states with cutoff: " << std::scientific
-               << std::setprecision(2) << P.eps2 << "\n";
-            log(os.str());
-        } else {
-            log(" Including all connected states in PT2 calculation.\n");
-        }
-        log(" Finding connected states...\n");
-        auto ps = eng.pt2(E, nopt, P.eps2, dE);
-        R.ns2 = ps.ns2;
-        R.t_pt2sel += ps.select_ms;
-        R.t_pt2sum += ps.sum_ms;
-        log(" Perturbative space contains " + std::to_string(ps.ns2) + " states.\n");
-        if (P.neig == 1)
-            log(" Calculating the 2nd-order perturbative correction on the ground state energy\n");
-        else
-            log(" Calculating the 2nd-order perturbative correction for on first " + std::to_string(nopt) +
-                " eigenvalues.\n");
-        for (int s = 0; s < nopt; s++) E[s] += dE[s];
-        log(" New ZPE after PT2 correction is: " + fmt_fixed(E[0], 2) + "\n");
-    }
-    R.E = E;
-}
+using vhci::drv::Logger;
+using vhci::drv::RunOpts;
+using vhci::drv::RunResult;
+using vhci::drv::fmt_fixed;
+using vhci::drv::run;
 
 std::string time_str(double secs, std::string& units) {
     double t = secs;
@@ -527,19 +380,7 @@ int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
     });
 }
 
-static uint64_t basis_checksum(Engine& eng) {
-    const int64_t n = eng.n();
-    const int M = eng.problem().M;
-    std::vector<int32_t> q((size_t)n * M);
-    eng.get_basis(q.data());
-    const uint64_t P = 1099511628211ull;
-    uint64_t h = 0, pw = 1;
-    for (int32_t v : q) {
-        pw *= P;
-        h += (uint64_t)(int64_t)v * pw;
-    }
-    return h;
-}
+static uint64_t basis_checksum(Engine& eng) { return vhci::drv::basis_checksum(eng); }
 
 static void fill_timings(double* t, const RunResult& R, double total) {
     if (!t) return;
@@ -611,6 +452,72 @@ int vhci_solve(const vhci_problem* p, int form, int device, double tol, double* 
 }
 
 // ---------------------------------------------------------------- driver
+// ---------------------------------------------------------------- serving
+int vhci_server_create(int device, int slots, int64_t mem_limit, vhci_server** out) {
+    if (!out || slots < 1 || mem_limit <= 0) return VHCI_E_ARG;
+    *out = nullptr;
+    try {
+        auto s = new vhci_server;
+        s->s = std::make_unique<vhci::srv::Server>(device, slots, mem_limit);
+        *out = s;
+        return VHCI_OK;
+    } catch (const std::exception& e) {
+        g_problem_err = e.what();
+        return VHCI_E_DEVICE;
+    }
+}
+
+void vhci_server_destroy(vhci_server* s) { delete s; }
+
+const char* vhci_server_error(const vhci_server* s) { return s ? s->err.c_str() : ""; }
+
+int vhci_server_submit(vhci_server* s, const char* text, int form, double tol, int64_t* id) {
+    if (!text || !id) return VHCI_E_ARG;
+    return sguard(s, [&] { *id = s->s->submit(text, form, tol); });
+}
+
+int vhci_server_wait(vhci_server* s, int64_t id, double timeout_s, int* done) {
+    if (!done) return VHCI_E_ARG;
+    return sguard(s, [&] { *done = s->s->wait(id, timeout_s) ? 1 : 0; });
+}
+
+int vhci_server_result(vhci_server* s, int64_t id, double* E, double* Evar, int* nE, int64_t* info,
+                       double* times) {
+    if (!s || !s->s) return VHCI_E_ARG;
+    vhci::srv::Request r;
+    int rc = sguard(s, [&] { r = s->s->get(id); });
+    if (rc != VHCI_OK) return rc;
+    if (info) {
+        info[0] = (int64_t)r.state;
+        info[1] = r.R.nvar;
+        info[2] = r.R.ns2;
+        info[3] = r.R.hb_iters;
+        info[4] = (int64_t)r.checksum;
+        info[5] = r.slot;
+    }
+    if (times) {
+        times[0] = r.t_submit;
+        times[1] = r.t_start;
+        times[2] = r.t_end;
+        times[3] = r.predicted;
+        times[4] = (double)r.peak;
+        times[5] = r.reserved;
+    }
+    if (nE) *nE = (int)r.R.E.size();
+    if (E) std::copy(r.R.E.begin(), r.R.E.end(), E);
+    if (Evar) std::copy(r.R.Evar.begin(), r.R.Evar.end(), Evar);
+    if (r.state == vhci::srv::State::Failed) {
+        s->err = r.err;
+        return classify(r.err);
+    }
+    return VHCI_OK;
+}
+
+int vhci_server_stats(vhci_server* s, double* out) {
+    if (!out) return VHCI_E_ARG;
+    return sguard(s, [&] { s->s->stats(out); });
+}
+
 int vhci_main(int argc, char** argv) {
     const auto start = std::chrono::steady_clock::now();
     std::cout << std::fixed << std::setprecision(12);

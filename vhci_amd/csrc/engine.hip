@@ -1447,9 +1447,21 @@ __global__ __launch_bounds__(64) void k_pt2_num(Pt2NumArgs a) {
 // ==========================================================================
 // DevBuf
 // ==========================================================================
+namespace {
+thread_local int64_t t_dev_bytes = 0;
+thread_local int64_t t_dev_peak = 0;
+}  // namespace
+
+int64_t thread_device_bytes() { return t_dev_bytes; }
+int64_t thread_device_peak() { return t_dev_peak; }
+void thread_device_reset_peak() { t_dev_peak = t_dev_bytes; }
+
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+        (void)hipFree(p);
+        t_dev_bytes -= (int64_t)cap;
+    }
     p = nullptr;
     cap = 0;
 }
@@ -1464,6 +1476,8 @@ void* DevBuf::ensure(size_t bytes) {
                                  hipGetErrorString(e));
     }
     cap = want;
+    t_dev_bytes += (int64_t)want;
+    t_dev_peak = std::max(t_dev_peak, t_dev_bytes);
     return p;
 }
 
@@ -1661,6 +1675,7 @@ void Engine::build_h_t(HStats& st) {
 
 HStats Engine::build_h() {
     HIPCHK(hipSetDevice(device_));
+    if (n_ < 1) throw std::runtime_error("build_h: the basis is empty");
     auto t0 = clk::now();
     HStats st;
     switch (KW_) {

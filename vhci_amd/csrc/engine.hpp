@@ -19,6 +19,12 @@ namespace vhci {
 enum Form { kFP64 = 0, kMixed = 1 };
 
 // Growable device allocation; contents are not preserved on growth.
+// Device bytes held by the DevBufs that this thread allocated and not yet
+// released, and the peak since the last reset (serving-loop accounting).
+int64_t thread_device_bytes();
+int64_t thread_device_peak();
+void thread_device_reset_peak();
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;

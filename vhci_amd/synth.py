@@ -13,6 +13,15 @@ Workloads (seed = 0x5648434900 + config):
       eps1 = 1.0, Nstate = 100, eps2 = 0.01.
   c2  48 modes: cubic and quartic (semi-diagonal + iijk) pools sampled with
       |f| >= 0.5 kept; Ntot 2, caps 6, eps1 = 1, Nstate = 128, eps2 = 0.2.
+  c3  serving stream: request r (seed 0x5648434900 + 1000 + r) draws
+      M uniform in [8, 36], eps1 from {0.5, 1, 2}, Nstate uniform in
+      [20, 200] and eps2 from {0, 0.01, 0.1}; the field has all cubic and
+      all semi-diagonal quartic constants with frequencies in [700, 3300]
+      as for c0, but cubic |f| in [1, 100] and f_iiii > 0: with c0's
+      cubic range and random quartic signs, many small-M draws have a
+      potential unbounded below, and heat bath (which caps no quanta)
+      then grows the basis into the collapse without end.  Ntot 2, caps 6,
+      PT2 on.  The ranges can be narrowed for short runs.
 Frequencies are uniform in [lo, hi]; |f| is log-uniform in [a, b] with a
 random sign.  Values are written with 9 decimals.
 """
@@ -117,7 +126,7 @@ def write_input(path_or_none, comment, eps1, nstate, pt2, eps2, ntot, freqs, cap
 
 
 def field(M, flo, fhi, seed, cubic=(1.0, 300.0), quartic=(0.1, 60.0), n_extra_iijk=0, cutoff=0.0,
-          cubic_pool=None, quartic_pool=None):
+          cubic_pool=None, quartic_pool=None, positive_iiii=False):
     rng = Rng(seed)
     freqs = [rng.uniform(flo, fhi) for _ in range(M)]
     fcs = []
@@ -128,6 +137,8 @@ def field(M, flo, fhi, seed, cubic=(1.0, 300.0), quartic=(0.1, 60.0), n_extra_ii
     qp = quartic_pool if quartic_pool is not None else _semidiag_quartic(M)
     for t in qp:
         v = rng.logmag(*quartic)
+        if positive_iiii and t[0] == t[1] == t[2] == t[3]:
+            v = abs(v)
         if abs(v) >= cutoff:
             fcs.append((t, v))
     if n_extra_iijk:
@@ -162,6 +173,26 @@ def config_input(c, path=None, **override):
     caps = args.pop("caps")
     return write_input(path, "synthetic workload c%d seed %#x" % (c, seed), freqs=freqs, caps=caps, fcs=fcs,
                        **args)
+
+
+def request_params(r, m_range=(8, 36), eps1_set=(0.5, 1.0, 2.0), nstate_range=(20, 200),
+                   eps2_set=(0.0, 0.01, 0.1)):
+    """Parameters of serving request r: (M, eps1, Nstate, eps2, field seed)."""
+    seed = BASE_SEED + 1000 + r
+    rng = Rng(seed)
+    M = m_range[0] + rng.randint(m_range[1] - m_range[0] + 1)
+    eps1 = eps1_set[rng.randint(len(eps1_set))]
+    nstate = nstate_range[0] + rng.randint(nstate_range[1] - nstate_range[0] + 1)
+    eps2 = eps2_set[rng.randint(len(eps2_set))]
+    return M, eps1, nstate, eps2, (seed + (1 << 40)) & MASK
+
+
+def request_input(r, path=None, **ranges):
+    """Input text of serving request r (workload c3)."""
+    M, eps1, nstate, eps2, fseed = request_params(r, **ranges)
+    freqs, fcs = field(M, 700.0, 3300.0, fseed, cubic=(1.0, 100.0), positive_iiii=True)
+    return write_input(path, "synthetic serving request %d seed %#x" % (r, BASE_SEED + 1000 + r), eps1=eps1,
+                       nstate=nstate, pt2=True, eps2=eps2, ntot=2, freqs=freqs, caps=[6] * M, fcs=fcs)
 
 
 if __name__ == "__main__":
