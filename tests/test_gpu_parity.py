@@ -101,6 +101,26 @@ def test_h_build_c1_grown(c1_iter1):
     assert ctx.structural_nnz() == orp[-1]
 
 
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_h_build_sorted_in_slices(c1_iter1, monkeypatch, form):
+    """The row sort of the H build runs in slices of rows (rocPRIM counts
+    entries in 32 bits); many small slices give the same matrix as one."""
+    path, q0, cm, new = c1_iter1
+    q = np.vstack([q0, new])
+    p = vhci_amd.Problem(path)
+    ctx = vhci_amd.Context(p, form)
+    ctx.set_basis(q)
+    ctx.build_h()
+    ref = ctx.get_h()
+    monkeypatch.setenv("VHCI_SORT_SLICE", "20000")  # above any row length (8440 patterns)
+    ctx.build_h()
+    got = ctx.get_h()
+    _say("H build in slices: nnz", len(ref[1]), "slices of 20000 entries")
+    assert len(ref[1]) > 50 * 20000
+    for a, b in zip(ref, got):
+        assert np.array_equal(a, b)
+
+
 def test_hb_select_matches_oracle_order(c1_iter1):
     """Same Cmax in, same new states out, in the same (first-discovery) order."""
     path, q0, cm, new = c1_iter1
@@ -210,6 +230,33 @@ def test_cli_clo2_fixtures(tmp_path, name):
     if "pt2_space" in pins:
         assert int(m.group(1)) == pins["pt2_space"]
     # the structural "% sparse" lines (explicit zeros counted)
+    pct = [float(x) for x in re.findall(r"The Hamiltonian is ([\d.]+)% sparse", log)]
+    assert pct == pins["sparse_pct"]
+
+
+@pytest.mark.parametrize("name", ["acn_vci", "acn_vhci", "acn_pt2"])
+def test_cli_acetonitrile_fixtures(tmp_path, name):
+    """The drop-in `vhci` on the acetonitrile examples (12 modes): the exact
+    basis sizes 91 -> 9749 -> 22857 -> 28666 -> 29900, the exact PT2 space
+    of 1,978,296 states, the "% sparse" lines, and all 70 energies within
+    1e-7 cm^-1 of the reference's freqs.dat.  Assignment strings of the
+    near-degenerate E pairs depend on the eigenvector rotation and are not
+    compared."""
+    _say("CLI", name)
+    out, log = _run_cli(tmp_path, name)
+    e, _ = read_freqs(out)
+    er, _ = read_freqs(os.path.join(GOLDEN, name + ".freqs"))
+    pins = manifest()[name]
+    sizes = [int(x) for x in re.findall(r"Basis size after iteration \d+ is: (\d+)", log)]
+    d = np.abs(absolute(e) - absolute(er))
+    _say("sizes", sizes, "pins", pins["iterations"], "max |dE|", d.max(), "at state", int(d.argmax()),
+         re.findall(r"Perturbative space contains (\d+)", log))
+    assert len(e) == len(er)
+    assert np.max(np.abs(absolute(e) - absolute(er))) < 1e-7
+    assert sizes == pins["iterations"]
+    m = re.search(r"Perturbative space contains (\d+) states", log)
+    if "pt2_space" in pins:
+        assert int(m.group(1)) == pins["pt2_space"]
     pct = [float(x) for x in re.findall(r"The Hamiltonian is ([\d.]+)% sparse", log)]
     assert pct == pins["sparse_pct"]
 

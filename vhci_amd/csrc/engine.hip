@@ -571,6 +571,12 @@ __global__ void k_unpack_hilo(int64_t nnz, const uint64_t* __restrict__ in, floa
     }
 }
 
+// row offsets of a slice of rows relative to the slice's first entry
+__global__ void k_rel_offsets(const int64_t* __restrict__ rp, int64_t m, int64_t base, uint32_t* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < m) out[i] = (uint32_t)(rp[i] - base);
+}
+
 // ------------------------------------------------------------------------
 // Davidson helpers
 // ------------------------------------------------------------------------
@@ -1629,38 +1635,65 @@ void Engine::build_h_t(HStats& st) {
         k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     }
-    // columns ascending within each row
+    // columns ascending within each row.  rocPRIM's segmented sort counts
+    // entries in 32 bits, so rows are sorted in slices of at most kSortSlice
+    // entries, each with its own row offsets relative to the slice start.
     if (nnz > 0) {
         unsigned endbit = 1;
         while (endbit < 31 && ((uint64_t)(n - 1) >> endbit) != 0) endbit++;
-        DevBuf ctmp, vin, vout;
-        ctmp.ensure(nnz * sizeof(int32_t));
-        vout.ensure(nnz * sizeof(uint64_t));
-        uint64_t* vals_in;
-        if (form_ == kFP64) {
-            vals_in = val_.as<uint64_t>();
-        } else {
-            vin.ensure(nnz * sizeof(uint64_t));
-            k_pack_hilo<<<grid_for(nnz, 256), 256, 0, stream_>>>(nnz, hi_.as<float>(), lo_.as<float>(),
-                                                                 vin.as<uint64_t>());
-            vals_in = vin.as<uint64_t>();
+        std::vector<int64_t> rph(n + 1);
+        HIPCHK(hipMemcpyAsync(rph.data(), rowptr_.p, (n + 1) * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        int64_t slice = kSortSlice;
+        if (const char* e = std::getenv("VHCI_SORT_SLICE")) slice = std::max<int64_t>(1, std::atoll(e));
+        const int64_t slice_cap = std::min<int64_t>(nnz, slice);
+        DevBuf ctmp, vin, vout, rel;
+        ctmp.ensure(slice_cap * sizeof(int32_t));
+        if (form_ != kFP64) vin.ensure(slice_cap * sizeof(uint64_t));
+        vout.ensure(slice_cap * sizeof(uint64_t));
+        int64_t r0 = 0;
+        while (r0 < n) {
+            // the longest run of rows from r0 whose entries fit in a slice (a
+            // row holds at most one entry per connection pattern, far below)
+            int64_t r1 = std::upper_bound(rph.begin() + r0 + 1, rph.end(), rph[r0] + slice) - rph.begin() - 1;
+            r1 = std::max(r1, r0 + 1);
+            const int64_t k0 = rph[r0], cnt = rph[r1] - rph[r0];
+            if (cnt > slice) throw std::runtime_error("H build: a row has more entries than a sort slice");
+            const int64_t segs = r1 - r0;
+            if (cnt > 0) {
+                rel.ensure((segs + 1) * sizeof(uint32_t));
+                k_rel_offsets<<<grid_for(segs + 1, 256), 256, 0, stream_>>>(rowptr_.as<int64_t>() + r0, segs + 1, k0,
+                                                                           rel.as<uint32_t>());
+                uint64_t* vals_in;
+                if (form_ == kFP64) {
+                    vals_in = val_.as<uint64_t>() + k0;
+                } else {
+                    k_pack_hilo<<<grid_for(cnt, 256), 256, 0, stream_>>>(cnt, hi_.as<float>() + k0,
+                                                                         lo_.as<float>() + k0, vin.as<uint64_t>());
+                    vals_in = vin.as<uint64_t>();
+                }
+                const uint32_t* ro = rel.as<uint32_t>();
+                size_t tmp = 0;
+                HIPCHK(rocprim::segmented_radix_sort_pairs(nullptr, tmp, col_.as<int32_t>() + k0, ctmp.as<int32_t>(),
+                                                           vals_in, vout.as<uint64_t>(), (unsigned)cnt,
+                                                           (unsigned)segs, ro, ro + 1, 0, endbit, stream_));
+                scan_tmp_.ensure(tmp);
+                HIPCHK(rocprim::segmented_radix_sort_pairs(scan_tmp_.p, tmp, col_.as<int32_t>() + k0,
+                                                           ctmp.as<int32_t>(), vals_in, vout.as<uint64_t>(),
+                                                           (unsigned)cnt, (unsigned)segs, ro, ro + 1, 0, endbit,
+                                                           stream_));
+                HIPCHK(hipMemcpyAsync(col_.as<int32_t>() + k0, ctmp.p, cnt * sizeof(int32_t),
+                                      hipMemcpyDeviceToDevice, stream_));
+                if (form_ == kFP64)
+                    HIPCHK(hipMemcpyAsync(val_.as<double>() + k0, vout.p, cnt * sizeof(double),
+                                          hipMemcpyDeviceToDevice, stream_));
+                else
+                    k_unpack_hilo<<<grid_for(cnt, 256), 256, 0, stream_>>>(cnt, vout.as<uint64_t>(),
+                                                                           hi_.as<float>() + k0, lo_.as<float>() + k0);
+                HIPCHK(hipGetLastError());
+            }
+            r0 = r1;
         }
-        size_t tmp = 0;
-        const int64_t* rp = rowptr_.as<int64_t>();
-        HIPCHK(rocprim::segmented_radix_sort_pairs(nullptr, tmp, col_.as<int32_t>(), ctmp.as<int32_t>(), vals_in,
-                                                   vout.as<uint64_t>(), (size_t)nnz, (unsigned)n, rp, rp + 1, 0,
-                                                   endbit, stream_));
-        scan_tmp_.ensure(tmp);
-        HIPCHK(rocprim::segmented_radix_sort_pairs(scan_tmp_.p, tmp, col_.as<int32_t>(), ctmp.as<int32_t>(), vals_in,
-                                                   vout.as<uint64_t>(), (size_t)nnz, (unsigned)n, rp, rp + 1, 0,
-                                                   endbit, stream_));
-        HIPCHK(hipMemcpyAsync(col_.p, ctmp.p, nnz * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
-        if (form_ == kFP64)
-            HIPCHK(hipMemcpyAsync(val_.p, vout.p, nnz * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-        else
-            k_unpack_hilo<<<grid_for(nnz, 256), 256, 0, stream_>>>(nnz, vout.as<uint64_t>(), hi_.as<float>(),
-                                                                   lo_.as<float>());
-        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream_));
     }
     D_.ensure(n * sizeof(double));
@@ -1922,7 +1955,13 @@ void Engine::dense_solve(int nev, std::vector<double>& E, EigStats& st) {
                                                form_ == kFP64 ? val_.as<double>() : nullptr, hi_.as<float>(),
                                                lo_.as<float>(), D_.as<double>(), A.as<double>());
     HIPCHK(hipGetLastError());
-    BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)n, A.as<double>(),
+    // The lower triangle is reduced from the first column, as Eigen's
+    // SelfAdjointEigenSolver (src/Ham.cpp:372-379) and LAPACK's lower form
+    // do.  For exactly degenerate eigenvalues (symmetry-equivalent modes)
+    // the eigenvectors then come out in the same rotation as the
+    // reference's, which fixes Cmax and so the first heat-bath selection;
+    // reducing the upper triangle picks another rotation.
+    BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_lower, (rocblas_int)n, A.as<double>(),
                              (rocblas_int)n, w.as<double>(), e.as<double>(), info_.as<rocblas_int>()));
     int info = 0;
     HIPCHK(hipMemcpyAsync(&info, info_.p, sizeof(int), hipMemcpyDeviceToHost, stream_));

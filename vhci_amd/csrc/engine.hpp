@@ -118,6 +118,8 @@ class Engine {
     double copy_probe(size_t bytes);
     // kernel launches per SpMM of b vectors
     int spmm_launches(int b) const;
+    // entries per call of the row sort of the H build (rocPRIM counts in 32 bits)
+    static constexpr int64_t kSortSlice = int64_t(1) << 30;
     // per-launch timing of the b-column products that the solver issues:
     // b > 0 starts (and clears) collection, b = 0 stops it.  The read gives
     // total ms, total algorithmic bytes and the launch count since the start.
