@@ -861,7 +861,8 @@ __global__ __launch_bounds__(256) void k_tsm_reduce(const double* __restrict__ p
 // LDS (p*q <= 4096), one row per thread, q <= 16 per launch tile.
 __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double alpha, const double* __restrict__ A,
                                                 int64_t lda, const double* __restrict__ S, int lds, double beta,
-                                                double* __restrict__ Y, int64_t ldy) {
+                                                double* __restrict__ Y, int64_t ldy, const int* __restrict__ gate) {
+    if (gate && *gate == 0) return;  // the product was found negligible (k_gate_absmax)
     __shared__ double Ss[4096];
     const int c0 = blockIdx.y * 16;
     const int qq = min(16, q - c0);
@@ -884,6 +885,19 @@ __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double 
             *y = (beta == 0.0 ? 0.0 : beta * *y) + alpha * acc[c];
         }
     }
+}
+
+// *gate = 1 if any |S[i]| exceeds thr, else 0 (one workgroup, fixed order)
+__global__ __launch_bounds__(256) void k_gate_absmax(const double* __restrict__ S, int64_t cnt, double thr,
+                                                     int* __restrict__ gate) {
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    int mine = 0;
+    for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) mine |= !(fabs(S[i]) <= thr);
+    if (mine) atomicOr(&any, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) *gate = any;
 }
 
 // ------------------------------------------------------------------------
@@ -2005,11 +2019,12 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
 
 // Y = beta Y + alpha A S (A n x p, S p x q col-major with lds)
 static void tsm_nn(hipStream_t s, int64_t n, int p, int q, double alpha, const double* A, int64_t lda,
-                   const double* S, int lds, double beta, double* Y, int64_t ldy) {
+                   const double* S, int lds, double beta, double* Y, int64_t ldy, const int* gate = nullptr) {
     for (int a0 = 0; a0 < p; a0 += 256) {
         const int pp = std::min(256, p - a0);
         dim3 g(std::min<unsigned>(grid_for(n, 256), 2048), (q + 15) / 16);
-        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy);
+        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy,
+                                   gate);
     }
     HIPCHK(hipGetLastError());
 }
@@ -2112,7 +2127,7 @@ void Engine::random_cm(double* V, int64_t ld, int k, uint64_t seed) {
 // classical Gram-Schmidt) and within itself (SVQB: eigendecomposition of the
 // scaled Gram matrix, near-dependent directions dropped); returns the number
 // of columns kept, packed at m0.
-int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
+int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
     const int64_t n = n_;
     const int ni = (int)n;
     const double one = 1.0, zero = 0.0, mone = -1.0;
@@ -2133,10 +2148,21 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
         HIPCHK(hipMemcpyAsync(g.data(), gram_.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
     };
-    auto cgs = [&]() {
+    std::vector<double> g, before;
+    auto cgs = [&](bool gated) {
         if (small) {
             tsm_tn(stream_, tsm_part_, n, m0, k, V, ld, T, ld, S_.as<double>(), m0);
-            tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), m0, 1.0, T, ld);
+            int* gate = nullptr;
+            if (gated) {
+                // skip the correction when every coefficient is below
+                // kOrthoSkip times the smallest column norm of the block
+                double mn = before[0];
+                for (int i = 1; i < k; i++) mn = std::min(mn, before[i]);
+                gate = info_.as<int>() + 2;
+                k_gate_absmax<<<1, 256, 0, stream_>>>(S_.as<double>(), (int64_t)m0 * k,
+                                                      kOrthoSkip * std::sqrt(std::max(mn, 0.0)), gate);
+            }
+            tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), m0, 1.0, T, ld, gate);
         } else {
             BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, m0, k, ni, &one, V, ld,
                                   T, ld, &zero, S_.as<double>(), m0));
@@ -2145,7 +2171,6 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
         }
     };
     // squared column norms before projection (DGKS test)
-    std::vector<double> g, before;
     if (m0 > 0) {
         gram_host(k, g);
         before.resize(k);
@@ -2154,7 +2179,7 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k) {
     int passes = 0;
     for (int round = 0; round < 4; round++) {
         if (m0 > 0) {
-            cgs();
+            cgs(lanczos && round == 0);
             passes++;
         }
         gram_host(k, g);
@@ -2460,7 +2485,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
                 tsm_tn(stream_, tsm_part_, n, p, bs, V + (int64_t)lo * ld, ld, T, ld, S_.as<double>(), p);
                 tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
             }
-            int kept = ortho_cm(V, ld, j + bs, bs);
+            int kept = ortho_cm(V, ld, j + bs, bs, j > k);
             for (int tries = 0; kept < bs && tries < 4; tries++) {
                 // invariant subspace reached for some directions: refill
                 random_cm(V + (int64_t)(j + bs + kept) * ld, ld, bs - kept, seed + 977 * (it * m + j + tries + 1));

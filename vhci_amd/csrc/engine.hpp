@@ -137,7 +137,12 @@ class Engine {
     void dense_solve(int nev, std::vector<double>& E, EigStats& st);
     // column-major block helpers of the iterative eigensolvers
     void apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st);
-    int ortho_cm(double* V, int64_t ld, int m0, int k);
+    // orthonormalise the k columns at V + m0*ld against the first m0 and
+    // among themselves; lanczos: the block already had its local Lanczos
+    // components removed, so the first projection is applied only when one
+    // of its coefficients exceeds kOrthoSkip times the block's smallest norm
+    int ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos = false);
+    static constexpr double kOrthoSkip = 1e-13;
     void random_cm(double* V, int64_t ld, int k, uint64_t seed);
     void davidson(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st);
     void krylov_schur(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st);
