@@ -96,8 +96,9 @@ int vhci_build_h(vhci_ctx* ctx, int64_t* stats);
  * when the displacement ball exceeds 500000 vectors (then the driver prints
  * the line from the stored entries), or a negative error code. */
 int64_t vhci_structural_nnz(vhci_ctx* ctx);
-/* Copy H to the host: rowptr[n+1], col/val[nnz] (off-diagonal entries in
- * stored order), diag[n].  Pass rowptr=NULL to query nnz via *nnz. */
+/* Copy H to the host: rowptr[n+1], col/val[nnz] (off-diagonal entries,
+ * rows in basis order, columns ascending), diag[n].  Pass rowptr=NULL to
+ * query nnz via *nnz. */
 int vhci_get_h(vhci_ctx* ctx, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val, double* diag);
 
 /* SparseDiagonalize (src/Ham.cpp:347-370): the nev lowest eigenpairs,

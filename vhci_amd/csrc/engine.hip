@@ -209,6 +209,8 @@ struct HBuildArgs {
     const uint32_t* fclist;
     const ElemFC* elem;
     const double* ladder;
+    const int32_t* perm;  // solver row -> basis state (null: identity)
+    const int32_t* inv;   // basis state -> solver row (null: identity)
     int64_t* rowcnt;
     const int64_t* rowptr;
     int32_t* col;
@@ -247,9 +249,10 @@ template <int NW, bool FILL>
 __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     __shared__ uint8_t sq[64];
     const int lane = threadIdx.x;
-    const int64_t row = blockIdx.x;
+    const int64_t row = blockIdx.x;                      // stored row
+    const int64_t src = a.perm ? a.perm[row] : row;      // its basis state
     uint32_t w[NW];
-    key_load<NW>(a.keys, row, w);
+    key_load<NW>(a.keys, src, w);
     if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
     __syncthreads();
     int64_t base = FILL ? a.rowptr[row] : 0;
@@ -285,8 +288,9 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
             if (hit) {
                 const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
                 const GroupEnt ge = a.groups[e.meta & 0x03ffffffu];
-                const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, row < j);
-                a.col[pos] = (int32_t)j;
+                // the element starts from the lower basis index, as AnharmPot(min, max)
+                const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, src < j);
+                a.col[pos] = a.inv ? a.inv[j] : (int32_t)j;
                 if (a.form == kFP64) {
                     a.val[pos] = v;
                 } else {
@@ -306,11 +310,12 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
 template <int NW>
 __global__ void k_diag(int64_t n, int M, const uint4* __restrict__ keys, const double* __restrict__ freq,
                        const uint32_t* __restrict__ dfc, int ndfc, const ElemFC* __restrict__ elem,
-                       const double* __restrict__ L, int nq, double* __restrict__ D) {
+                       const double* __restrict__ L, int nq, const int32_t* __restrict__ perm,
+                       double* __restrict__ D) {
     int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= n) return;
     uint32_t w[NW];
-    key_load<NW>(keys, i, w);
+    key_load<NW>(keys, perm ? (int64_t)perm[i] : i, w);
     double e = 0.;
     for (int j = 0; j < M; j++) {
         double ej = 0.5;
@@ -521,7 +526,8 @@ __global__ __launch_bounds__(256) void k_spmm_lane(int64_t n, const int64_t* __r
     }
 }
 
-// single vector: 16 lanes per row, 16 rows per 256-thread block
+// single vector: 32 lanes per row, 8 rows per 256-thread block, four
+// entries per lane in flight
 template <int MODE>
 __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ col, const double* __restrict__ val,
@@ -529,31 +535,31 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t* __restri
                                               const double* __restrict__ D, const float* __restrict__ x32,
                                               const double* __restrict__ x, double* __restrict__ y) {
     using AT = typename SpAcc<MODE>::T;
-    const int sub = threadIdx.x & 15;
-    const int64_t row = (int64_t)blockIdx.x * 16 + (threadIdx.x >> 4);
+    const int sub = threadIdx.x & 31;
+    const int64_t row = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
     const bool live = row < n;
     AT a0 = 0, a1 = 0, a2 = 0, a3 = 0;
     if (live) {
         const int64_t k1 = rowptr[row + 1];
         int64_t k = rowptr[row] + sub;
-        for (; k + 48 < k1; k += 64) {
+        for (; k + 96 < k1; k += 128) {
             const int64_t j0 = __builtin_nontemporal_load(col + k);
-            const int64_t j1 = __builtin_nontemporal_load(col + k + 16);
-            const int64_t j2 = __builtin_nontemporal_load(col + k + 32);
-            const int64_t j3 = __builtin_nontemporal_load(col + k + 48);
+            const int64_t j1 = __builtin_nontemporal_load(col + k + 32);
+            const int64_t j2 = __builtin_nontemporal_load(col + k + 64);
+            const int64_t j3 = __builtin_nontemporal_load(col + k + 96);
             a0 = sp_fma(sp_val<MODE>(val, hi, lo, k), sp_x<MODE>(x, x32, j0, j0), a0);
-            a1 = sp_fma(sp_val<MODE>(val, hi, lo, k + 16), sp_x<MODE>(x, x32, j1, j1), a1);
-            a2 = sp_fma(sp_val<MODE>(val, hi, lo, k + 32), sp_x<MODE>(x, x32, j2, j2), a2);
-            a3 = sp_fma(sp_val<MODE>(val, hi, lo, k + 48), sp_x<MODE>(x, x32, j3, j3), a3);
+            a1 = sp_fma(sp_val<MODE>(val, hi, lo, k + 32), sp_x<MODE>(x, x32, j1, j1), a1);
+            a2 = sp_fma(sp_val<MODE>(val, hi, lo, k + 64), sp_x<MODE>(x, x32, j2, j2), a2);
+            a3 = sp_fma(sp_val<MODE>(val, hi, lo, k + 96), sp_x<MODE>(x, x32, j3, j3), a3);
         }
-        for (; k < k1; k += 16) {
+        for (; k < k1; k += 32) {
             const int64_t j0 = __builtin_nontemporal_load(col + k);
             a0 = sp_fma(sp_val<MODE>(val, hi, lo, k), sp_x<MODE>(x, x32, j0, j0), a0);
         }
     }
     AT acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
-    for (int off = 8; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 16);
+    for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 32);
     if (live && sub == 0) y[row] = D[row] * x[row] + (double)acc;
 }
 
@@ -575,6 +581,42 @@ __global__ void k_unpack_hilo(int64_t nnz, const uint64_t* __restrict__ in, floa
 __global__ void k_rel_offsets(const int64_t* __restrict__ rp, int64_t m, int64_t base, uint32_t* __restrict__ out) {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < m) out[i] = (uint32_t)(rp[i] - base);
+}
+
+// ------------------------------------------------------------------------
+// solver order of the basis: states sorted by their quanta read as one
+// integer with the highest mode most significant, so that states adjacent
+// in the order share most quanta and their Hamiltonian rows gather nearby
+// rows of X in the block products
+// ------------------------------------------------------------------------
+__global__ void k_iota32(int32_t* __restrict__ p, int64_t n) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = (int32_t)i;
+}
+
+// out[r] = 64-bit chunk c (quanta of modes 8c..8c+7) of the key of state perm[r]
+__global__ void k_key_chunk(const uint32_t* __restrict__ keys, int wpk, const int32_t* __restrict__ perm, int64_t n,
+                            int c, uint64_t* __restrict__ out) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const uint32_t* k = keys + (int64_t)perm[r] * wpk;
+    out[r] = ((uint64_t)k[2 * c + 1] << 32) | (uint64_t)k[2 * c];
+}
+
+__global__ void k_invert32(const int32_t* __restrict__ perm, int64_t n, int32_t* __restrict__ inv) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r < n) inv[perm[r]] = (int32_t)r;
+}
+
+// V[r, c] += T[perm[r], c] for the rows whose state existed in the previous
+// basis (perm[r] < rows); T column-major with leading dimension rows
+__global__ void k_add_perm_rows(double* __restrict__ V, int64_t ldv, int64_t n, int k, const double* __restrict__ T,
+                                int64_t rows, const int32_t* __restrict__ perm) {
+    const int c = blockIdx.y;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t src = perm[r];
+        if (src < rows) V[r + (int64_t)c * ldv] += T[src + (int64_t)c * rows];
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -648,12 +690,12 @@ __global__ void k_densify(int64_t n, const int64_t* __restrict__ rowptr, const i
 
 // column-major (ld) eigenvector block -> row-major n x ncol
 __global__ void k_colmajor_to_rows(const double* __restrict__ A, int64_t lda, int64_t n, int ncol,
-                                   double* __restrict__ X) {
+                                   double* __restrict__ X, const int32_t* __restrict__ rowmap) {
     int64_t total = n * ncol;
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
         const int64_t r = t / ncol;
         const int c = (int)(t - r * ncol);
-        X[t] = A[(int64_t)c * lda + r];
+        X[(rowmap ? (int64_t)rowmap[r] : r) * ncol + c] = A[(int64_t)c * lda + r];
     }
 }
 
@@ -664,7 +706,7 @@ __global__ void k_colmajor_to_rows(const double* __restrict__ A, int64_t lda, in
 // tiles so both sides are read and written along contiguous addresses.
 template <class TO>
 __global__ __launch_bounds__(256) void k_cm_to_rm(const double* __restrict__ A, int64_t lda, int64_t n, int k,
-                                                  TO* __restrict__ B, int ldb) {
+                                                  TO* __restrict__ B, int ldb, const int32_t* __restrict__ rowmap) {
     __shared__ double tile[32][33];
     const int64_t r0 = (int64_t)blockIdx.x * 32;
     const int c0 = blockIdx.y * 32;
@@ -678,7 +720,7 @@ __global__ __launch_bounds__(256) void k_cm_to_rm(const double* __restrict__ A, 
     for (int j = ty; j < 32; j += 8) {
         const int64_t r = r0 + j;
         const int c = c0 + tx;
-        if (r < n && c < k) B[r * ldb + c] = (TO)tile[tx][j];
+        if (r < n && c < k) B[(rowmap ? (int64_t)rowmap[r] : r) * ldb + c] = (TO)tile[tx][j];
     }
 }
 
@@ -1596,6 +1638,39 @@ void Engine::rebuild_hash(int64_t n) {
 }
 
 // ---------------------------------------------------------------- H build
+void Engine::make_solver_order() {
+    const int64_t n = n_;
+    const char* e = std::getenv("VHCI_REORDER");
+    const char* alg = std::getenv("VHCI_EIGSOLVER");
+    const bool off = (e && std::string(e) == "0") || (alg && std::string(alg) == "davidson");
+    perm_on_ = !off && n >= kReorderMin;
+    if (!perm_on_) return;
+    perm_.ensure(n * sizeof(int32_t));
+    inv_.ensure(n * sizeof(int32_t));
+    DevBuf kin, kout, pout;
+    kin.ensure(n * sizeof(uint64_t));
+    kout.ensure(n * sizeof(uint64_t));
+    pout.ensure(n * sizeof(int32_t));
+    k_iota32<<<grid_for(n, 256), 256, 0, stream_>>>(perm_.as<int32_t>(), n);
+    // least significant chunk first; each radix sort is stable
+    const int nchunk = (P_.M + 7) / 8;
+    for (int c = 0; c < nchunk; c++) {
+        k_key_chunk<<<grid_for(n, 256), 256, 0, stream_>>>(keys_.as<uint32_t>(), 4 * KW_, perm_.as<int32_t>(), n, c,
+                                                           kin.as<uint64_t>());
+        const unsigned endbit = 8u * (unsigned)std::min(8, P_.M - 8 * c);
+        size_t tmp = 0;
+        HIPCHK(rocprim::radix_sort_pairs(nullptr, tmp, kin.as<uint64_t>(), kout.as<uint64_t>(), perm_.as<int32_t>(),
+                                         pout.as<int32_t>(), (size_t)n, 0, endbit, stream_));
+        scan_tmp_.ensure(tmp);
+        HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, tmp, kin.as<uint64_t>(), kout.as<uint64_t>(),
+                                         perm_.as<int32_t>(), pout.as<int32_t>(), (size_t)n, 0, endbit, stream_));
+        HIPCHK(hipMemcpyAsync(perm_.p, pout.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+    }
+    k_invert32<<<grid_for(n, 256), 256, 0, stream_>>>(perm_.as<int32_t>(), n, inv_.as<int32_t>());
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
 template <int NW>
 void Engine::build_h_t(HStats& st) {
     const int64_t n = n_;
@@ -1613,6 +1688,9 @@ void Engine::build_h_t(HStats& st) {
     a.elem = d_elem_.as<ElemFC>();
     a.ladder = d_ladder_.as<double>();
     a.form = form_;
+    make_solver_order();
+    a.perm = perm_on_ ? perm_.as<int32_t>() : nullptr;
+    a.inv = perm_on_ ? inv_.as<int32_t>() : nullptr;
     rowcnt_.ensure((n + 1) * sizeof(int64_t));
     rowptr_.ensure((n + 1) * sizeof(int64_t));
     a.rowcnt = rowcnt_.as<int64_t>();
@@ -1712,7 +1790,7 @@ void Engine::build_h_t(HStats& st) {
     }
     D_.ensure(n * sizeof(double));
     k_diag<NW><<<grid_for(n, 128), 128, 0, stream_>>>(n, P_.M, a.keys, d_freq_.as<double>(), d_diag_.as<uint32_t>(),
-                                                      (int)T_.diag_fcs.size(), a.elem, a.ladder, 256,
+                                                      (int)T_.diag_fcs.size(), a.elem, a.ladder, 256, a.perm,
                                                       D_.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
@@ -1758,6 +1836,34 @@ void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std:
         }
     }
     HIPCHK(hipStreamSynchronize(stream_));
+    if (perm_on_) {
+        // back to basis order: row perm[r] holds stored row r, columns mapped
+        // through perm and sorted again
+        std::vector<int32_t> perm(n_);
+        HIPCHK(hipMemcpy(perm.data(), perm_.p, n_ * sizeof(int32_t), hipMemcpyDeviceToHost));
+        std::vector<int32_t> inv(n_);
+        for (int64_t r = 0; r < n_; r++) inv[perm[r]] = (int32_t)r;
+        std::vector<int64_t> rp(n_ + 1, 0);
+        for (int64_t i = 0; i < n_; i++) rp[i + 1] = rp[i] + (rowptr[inv[i] + 1] - rowptr[inv[i]]);
+        std::vector<int32_t> c2(nnz_);
+        std::vector<double> v2(nnz_), d2(n_);
+        std::vector<std::pair<int32_t, double>> row;
+        for (int64_t i = 0; i < n_; i++) {
+            const int64_t r = inv[i];
+            row.clear();
+            for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) row.emplace_back(perm[col[k]], val[k]);
+            std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+            for (size_t t = 0; t < row.size(); t++) {
+                c2[rp[i] + t] = row[t].first;
+                v2[rp[i] + t] = row[t].second;
+            }
+            d2[i] = diag[r];
+        }
+        rowptr.swap(rp);
+        col.swap(c2);
+        val.swap(v2);
+        diag.swap(d2);
+    }
 }
 
 // Structural nonzeros of the reference's triplet list (Ham.cpp:237-326):
@@ -1812,7 +1918,7 @@ void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low,
     const float* h = hi_.as<float>();
     const float* l = lo_.as<float>();
     if (b == 1) {
-        const unsigned grid = grid_for(n, 16);
+        const unsigned grid = grid_for(n, 8);
         if (mode == 0) k_spmv<0><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
         else if (mode == 1) k_spmv<1><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
         else k_spmv<2><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
@@ -1985,7 +2091,8 @@ void Engine::dense_solve(int nev, std::vector<double>& E, EigStats& st) {
     if (info != 0) throw std::runtime_error("Error: Eigenvalues did not converge.");
     E.assign(wh.begin(), wh.begin() + nev);
     X_.ensure((size_t)n * nev * sizeof(double));
-    k_colmajor_to_rows<<<grid_for(n * nev, 256), 256, 0, stream_>>>(A.as<double>(), n, n, nev, X_.as<double>());
+    k_colmajor_to_rows<<<grid_for(n * nev, 256), 256, 0, stream_>>>(A.as<double>(), n, n, nev, X_.as<double>(),
+                                                                     perm_on_ ? perm_.as<int32_t>() : nullptr);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
     xrows_ = n;
@@ -2091,10 +2198,10 @@ void Engine::apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, E
         stage_in_.ensure((size_t)n * kk * sizeof(double));
         stage_out_.ensure((size_t)n * kk * sizeof(double));
         dim3 tg(grid_for(n, 32), (kk + 31) / 32);
-        k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X + c * ld, ld, n, kk, stage_in_.as<double>(), kk);
+        k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X + c * ld, ld, n, kk, stage_in_.as<double>(), kk, nullptr);
         if (f32) {
             stage_in32_.ensure((size_t)n * kk * sizeof(float));
-            k_cm_to_rm<float><<<tg, 256, 0, stream_>>>(X + c * ld, ld, n, kk, stage_in32_.as<float>(), kk);
+            k_cm_to_rm<float><<<tg, 256, 0, stream_>>>(X + c * ld, ld, n, kk, stage_in32_.as<float>(), kk, nullptr);
             spmm(stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk, true, stage_in32_.as<float>(), kk);
         } else {
             spmm(stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk, false);
@@ -2350,7 +2457,7 @@ void Engine::davidson(int nev, double tol, int maxit, std::vector<double>& E, Ei
             E.assign(thh.begin(), thh.begin() + nev);
             X_.ensure((size_t)n * nev * sizeof(double));
             dim3 tg(grid_for(n, 32), (nev + 31) / 32);
-            k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X, ld, n, nev, X_.as<double>(), nev);
+            k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X, ld, n, nev, X_.as<double>(), nev, perm_on_ ? perm_.as<int32_t>() : nullptr);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(stream_));
             xrows_ = n;
@@ -2451,9 +2558,23 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         Rd.ensure(R.size() * sizeof(double));
         HIPCHK(hipMemcpyAsync(Rd.p, R.data(), R.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
         const double big = 1e3;
-        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, (int)xrows_, bs, xcols_,
-                              &big, X_.as<double>(), xcols_, Rd.as<double>(), xcols_, &one, V, ld));
-        HIPCHK(hipStreamSynchronize(stream_));
+        if (perm_on_) {
+            // previous vectors are in basis order: combine, then scatter into solver rows
+            DevBuf T;
+            T.ensure((size_t)xrows_ * bs * sizeof(double));
+            const double zero0 = 0.0;
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, (int)xrows_, bs, xcols_,
+                                  &big, X_.as<double>(), xcols_, Rd.as<double>(), xcols_, &zero0, T.as<double>(),
+                                  (int)xrows_));
+            dim3 g(std::min<unsigned>(grid_for(n, 256), 4096), bs);
+            k_add_perm_rows<<<g, 256, 0, stream_>>>(V, ld, n, bs, T.as<double>(), xrows_, perm_.as<int32_t>());
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream_));
+        } else {
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, (int)xrows_, bs, xcols_,
+                                  &big, X_.as<double>(), xcols_, Rd.as<double>(), xcols_, &one, V, ld));
+            HIPCHK(hipStreamSynchronize(stream_));
+        }
     }
     int got = ortho_cm(V, ld, 0, bs);
     if (got < bs) throw std::runtime_error("Krylov-Schur: degenerate start block");
@@ -2525,7 +2646,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
             E.assign(thh.begin(), thh.begin() + nev);
             X_.ensure((size_t)n * nev * sizeof(double));
             dim3 tg(grid_for(n, 32), (nev + 31) / 32);
-            k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X, ld, n, nev, X_.as<double>(), nev);
+            k_cm_to_rm<double><<<tg, 256, 0, stream_>>>(X, ld, n, nev, X_.as<double>(), nev, perm_on_ ? perm_.as<int32_t>() : nullptr);
             HIPCHK(hipGetLastError());
             HIPCHK(hipStreamSynchronize(stream_));
             xrows_ = n;

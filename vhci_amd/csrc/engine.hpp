@@ -118,6 +118,8 @@ class Engine {
     double copy_probe(size_t bytes);
     // kernel launches per SpMM of b vectors
     int spmm_launches(int b) const;
+    // bases of at least this many states are stored in solver order
+    static constexpr int64_t kReorderMin = 16384;
     // entries per call of the row sort of the H build (rocPRIM counts in 32 bits)
     static constexpr int64_t kSortSlice = int64_t(1) << 30;
     // per-launch timing of the b-column products that the solver issues:
@@ -131,6 +133,7 @@ class Engine {
 
   private:
     void upload_tables();
+    void make_solver_order();
     void rebuild_hash(int64_t cap_hint);
     void spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32 = nullptr,
               int ldx32 = 0);
@@ -175,6 +178,11 @@ class Engine {
     bool h_valid_ = false;
     int64_t nnz_ = 0;
     DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_;
+    // solver order: H, D and the eigensolver's blocks index states in the
+    // order perm_ (solver row -> basis state; inv_ the inverse); the
+    // resident Ritz block X_ stays in basis order
+    DevBuf perm_, inv_;
+    bool perm_on_ = false;
     // Ritz block (row-major n x xcols_, ld = xcols_)
     DevBuf X_;
     int64_t xrows_ = 0;
