@@ -2514,13 +2514,17 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     const int ni = (int)n;
     int bs = 8;
     if (const char* e = std::getenv("VHCI_KS_BLOCK")) bs = std::max(1, std::min(32, std::atoi(e)));
-    // basis of m vectors (a multiple of bs, at least 2 nev and nev + 6 blocks);
-    // each cycle adds about half of m - nev new vectors
-    int m = ((std::max(std::max(3 * nev, 2 * nev + bs), nev + 6 * bs) + bs - 1) / bs) * bs;
-    if (const char* e = std::getenv("VHCI_KS_DIM")) m = std::max(m, (std::atoi(e) / bs) * bs);
+    // basis of m = 3.5 nev vectors (a multiple of bs, at least nev + 6
+    // blocks), restarted from the keep = 1.6 nev lowest Ritz vectors: each
+    // cycle adds about 1.9 nev new vectors per Rayleigh-Ritz step (on
+    // config 1 this is 12 % faster than m = 3 nev, keep = 2.1 nev)
+    const auto up = [&](double x) { return ((int)std::ceil(x) + bs - 1) / bs * bs; };
+    int m = std::max(up(3.5 * nev), up((double)nev + 6 * bs));
+    if (const char* e = std::getenv("VHCI_KS_DIM")) m = std::max(nev + 2 * bs, (std::atoi(e) / bs) * bs);
     while (m + bs > n) m -= bs;
     if (m < nev + 2 * bs) throw std::runtime_error("Krylov-Schur: basis too small for the requested states");
-    const int keep = m - std::max(1, (m - nev) / (2 * bs)) * bs;
+    int keep = std::min(m - bs, std::max(up(1.6 * nev), up((double)nev + bs)));
+    if (const char* e = std::getenv("VHCI_KS_KEEP")) keep = std::max(nev, std::min(m - bs, (std::atoi(e) / bs) * bs));
     const int64_t ld = n;
     V_.ensure((size_t)n * (m + bs) * sizeof(double));
     W_.ensure((size_t)n * m * sizeof(double));
