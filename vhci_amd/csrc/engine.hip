@@ -903,8 +903,7 @@ __global__ __launch_bounds__(256) void k_tsm_reduce(const double* __restrict__ p
 // LDS (p*q <= 4096), one row per thread, q <= 16 per launch tile.
 __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double alpha, const double* __restrict__ A,
                                                 int64_t lda, const double* __restrict__ S, int lds, double beta,
-                                                double* __restrict__ Y, int64_t ldy, const int* __restrict__ gate) {
-    if (gate && *gate == 0) return;  // the product was found negligible (k_gate_absmax)
+                                                double* __restrict__ Y, int64_t ldy) {
     __shared__ double Ss[4096];
     const int c0 = blockIdx.y * 16;
     const int qq = min(16, q - c0);
@@ -929,17 +928,53 @@ __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double 
     }
 }
 
-// *gate = 1 if any |S[i]| exceeds thr, else 0 (one workgroup, fixed order)
-__global__ __launch_bounds__(256) void k_gate_absmax(const double* __restrict__ S, int64_t cnt, double thr,
-                                                     int* __restrict__ gate) {
+// R (p x k, column-major, ld p) holds V^T T in rows 0..m0 and T^T T in rows
+// m0..m0+k.  *gate = 1 if any projection coefficient exceeds rel times the
+// smallest column norm of T, else 0 (one workgroup)
+__global__ __launch_bounds__(256) void k_gate_proj(const double* __restrict__ R, int m0, int k, int p, double rel,
+                                                   int* __restrict__ gate) {
     __shared__ int any;
-    if (threadIdx.x == 0) any = 0;
+    __shared__ double thr;
+    if (threadIdx.x == 0) {
+        double mn = R[m0];
+        for (int c = 1; c < k; c++) mn = fmin(mn, R[(int64_t)c * p + m0 + c]);
+        thr = rel * sqrt(fmax(mn, 0.0));
+        any = 0;
+    }
     __syncthreads();
     int mine = 0;
-    for (int64_t i = threadIdx.x; i < cnt; i += blockDim.x) mine |= !(fabs(S[i]) <= thr);
+    for (int64_t i = threadIdx.x; i < (int64_t)m0 * k; i += blockDim.x) {
+        const int c = (int)(i / m0), r = (int)(i - (int64_t)c * m0);
+        mine |= !(fabs(R[(int64_t)c * p + r]) <= thr);
+    }
     if (mine) atomicOr(&any, 1);
     __syncthreads();
     if (threadIdx.x == 0) *gate = any;
+}
+
+// T[:, 0..kk) = T[:, 0..k) Z in place (T column-major n x k, ld; Z k x kk
+// column-major, kk <= k <= KMAX): each row is independent
+template <int KMAX>
+__global__ __launch_bounds__(256) void k_rowmix(int64_t n, double* __restrict__ T, int64_t ld, int k, int kk,
+                                                const double* __restrict__ Z) {
+    __shared__ double Zs[KMAX * KMAX];
+    for (int i = threadIdx.x; i < k * kk; i += blockDim.x) Zs[i] = Z[i];
+    __syncthreads();
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        double t[KMAX];
+#pragma unroll
+        for (int a = 0; a < KMAX; a++) t[a] = a < k ? T[r + a * ld] : 0.0;
+#pragma unroll
+        for (int c = 0; c < KMAX; c++) {
+            if (c < kk) {
+                double acc = 0.0;
+#pragma unroll
+                for (int a = 0; a < KMAX; a++)
+                    if (a < k) acc = fma(t[a], Zs[c * k + a], acc);
+                T[r + c * ld] = acc;
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------------------------
@@ -2126,12 +2161,11 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
 
 // Y = beta Y + alpha A S (A n x p, S p x q col-major with lds)
 static void tsm_nn(hipStream_t s, int64_t n, int p, int q, double alpha, const double* A, int64_t lda,
-                   const double* S, int lds, double beta, double* Y, int64_t ldy, const int* gate = nullptr) {
+                   const double* S, int lds, double beta, double* Y, int64_t ldy) {
     for (int a0 = 0; a0 < p; a0 += 256) {
         const int pp = std::min(256, p - a0);
         dim3 g(std::min<unsigned>(grid_for(n, 256), 2048), (q + 15) / 16);
-        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy,
-                                   gate);
+        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy);
     }
     HIPCHK(hipGetLastError());
 }
@@ -2241,7 +2275,7 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
     double* T = V + m0 * ld;
     S_.ensure((size_t)std::max(1, m0) * k * sizeof(double));
     gram_.ensure((size_t)k * k * sizeof(double));
-    otmp_.ensure((size_t)n * k * sizeof(double));
+    if (k > 32) otmp_.ensure((size_t)n * k * sizeof(double));
     work_.ensure((size_t)std::max<size_t>(work_.cap / sizeof(double), 2 * (size_t)k) * sizeof(double));
     info_.ensure(sizeof(int) * 4);
     const bool small = k <= 32;
@@ -2256,20 +2290,10 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
         HIPCHK(hipStreamSynchronize(stream_));
     };
     std::vector<double> g, before;
-    auto cgs = [&](bool gated) {
+    auto cgs = [&]() {
         if (small) {
             tsm_tn(stream_, tsm_part_, n, m0, k, V, ld, T, ld, S_.as<double>(), m0);
-            int* gate = nullptr;
-            if (gated) {
-                // skip the correction when every coefficient is below
-                // kOrthoSkip times the smallest column norm of the block
-                double mn = before[0];
-                for (int i = 1; i < k; i++) mn = std::min(mn, before[i]);
-                gate = info_.as<int>() + 2;
-                k_gate_absmax<<<1, 256, 0, stream_>>>(S_.as<double>(), (int64_t)m0 * k,
-                                                      kOrthoSkip * std::sqrt(std::max(mn, 0.0)), gate);
-            }
-            tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), m0, 1.0, T, ld, gate);
+            tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), m0, 1.0, T, ld);
         } else {
             BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, m0, k, ni, &one, V, ld,
                                   T, ld, &zero, S_.as<double>(), m0));
@@ -2278,18 +2302,47 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
         }
     };
     // squared column norms before projection (DGKS test)
-    if (m0 > 0) {
+    bool first_done = false;
+    if (m0 > 0 && small && lanczos) {
+        // A Lanczos block (its local components already removed): one pass
+        // over [V, T] gives the projection coefficients V^T T (rows 0..m0)
+        // and the Gram matrix of T (rows m0..m0+k).  The projection is
+        // applied only when a coefficient exceeds kOrthoSkip times the
+        // smallest column norm of T (it is 1e-16..1e-14 otherwise), and the
+        // Gram matrix is then formed again.
+        const int p = m0 + k;
+        S_.ensure((size_t)p * k * sizeof(double));
+        tsm_tn(stream_, tsm_part_, n, p, k, V, ld, T, ld, S_.as<double>(), p);
+        int* gate = info_.as<int>() + 2;
+        k_gate_proj<<<1, 256, 0, stream_>>>(S_.as<double>(), m0, k, p, kOrthoSkip, gate);
+        std::vector<double> R((size_t)p * k);
+        int flag = 0;
+        HIPCHK(hipMemcpyAsync(R.data(), S_.p, R.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(&flag, gate, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        g.assign((size_t)k * k, 0.0);
+        for (int c = 0; c < k; c++)
+            for (int i = 0; i < k; i++) g[(size_t)c * k + i] = R[(size_t)c * p + m0 + i];
+        before.resize(k);
+        for (int i = 0; i < k; i++) before[i] = g[(size_t)i * k + i];
+        if (flag) {
+            tsm_nn(stream_, n, m0, k, -1.0, V, ld, S_.as<double>(), p, 1.0, T, ld);
+            gram_host(k, g);
+        }
+        first_done = true;
+    } else if (m0 > 0) {
         gram_host(k, g);
         before.resize(k);
         for (int i = 0; i < k; i++) before[i] = g[(size_t)i * k + i];
     }
     int passes = 0;
     for (int round = 0; round < 4; round++) {
+        const bool given = first_done && round == 0;  // projection and Gram matrix from the fast path
         if (m0 > 0) {
-            cgs(lanczos && round == 0);
+            if (!given) cgs();
             passes++;
         }
-        gram_host(k, g);
+        if (!given) gram_host(k, g);
         if (m0 > 0 && passes == 1 && round == 0) {
             // a column that lost more than half its squared norm to the
             // projection gets a second Gram-Schmidt pass (Daniel, Gragg,
@@ -2332,13 +2385,18 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
         }
         S_.ensure((size_t)std::max(std::max(1, m0) * k, k * kk) * sizeof(double));
         HIPCHK(hipMemcpyAsync(S_.p, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
-        if (small)
-            tsm_nn(stream_, n, k, kk, 1.0, T, ld, S_.as<double>(), k, 0.0, otmp_.as<double>(), n);
-        else
+        if (small) {
+            const unsigned rg = std::min<unsigned>(grid_for(n, 256), 2048);
+            if (k <= 8) k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>());
+            else if (k <= 16) k_rowmix<16><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>());
+            else k_rowmix<32><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>());
+            HIPCHK(hipGetLastError());
+        } else {
             BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, kk, k, &one, T, ld,
                                   S_.as<double>(), k, &zero, otmp_.as<double>(), ni));
-        HIPCHK(hipMemcpy2DAsync(T, ld * sizeof(double), otmp_.p, n * sizeof(double), n * sizeof(double), kk,
-                                hipMemcpyDeviceToDevice, stream_));
+            HIPCHK(hipMemcpy2DAsync(T, ld * sizeof(double), otmp_.p, n * sizeof(double), n * sizeof(double), kk,
+                                    hipMemcpyDeviceToDevice, stream_));
+        }
         HIPCHK(hipStreamSynchronize(stream_));
         k = kk;
         // a well-conditioned block is done; otherwise repeat projection and
