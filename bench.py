@@ -61,6 +61,7 @@ def run_form(problem, form, steps, warmup, bs):
     lat = []
     last = None
     sp = [0.0, 0.0, 0]
+    gm = [0.0, 0.0, 0]
     for _ in range(steps):
         t0 = time.perf_counter()
         ctx = vhci_amd.Context(problem, form)
@@ -68,14 +69,45 @@ def run_form(problem, form, steps, warmup, bs):
         r = ctx.run()
         lat.append(time.perf_counter() - t0)
         ms, by, nl = ctx.spmm_profile_read()
+        gms, gfl, gn = ctx.gemm_profile_read()
         ctx.spmm_profile(0)
         sp = [sp[0] + ms, sp[1] + by, sp[2] + nl]
+        gm = [gm[0] + gms, gm[1] + gfl, gm[2] + gn]
         if last is not None:
             last[0].close()
         last = (ctx, r)
         print("[bench] form=%s solve %.3f s nvar=%d ns2=%d" % (form, lat[-1], r["nvar"], r["ns2"]),
               file=sys.stderr, flush=True)
-    return lat, last, {"ms": sp[0], "bytes": sp[1], "launches": sp[2]}
+    return lat, last, {"ms": sp[0], "bytes": sp[1], "launches": sp[2], "gemm_ms": gm[0], "gemm_flops": gm[1],
+                       "gemm_calls": gm[2], "phases": last[0].phase_stats()}
+
+
+FP64_PEAK_TFLOPS = 78.6
+
+
+def phase_rooflines(ph, sp):
+    """Algorithmic-bytes bandwidth of the H build, the PT2 selection and the
+    PT2 sum of the last timed solve (wall times of the phases, syncs
+    included), and the Ritz rotation GEMMs' FP64 rate."""
+    K = ph["key_bytes"]
+    n, nnz, P = ph["n"], ph["nnz"], ph["patterns"]
+    sv = 8.0  # FP64 value, or FP32 hi + FP32 lo
+    probes = n * P
+    hb = probes * (K + 4) + nnz * (sv + 4) + 8 * (n + 1) + n * K
+    sel = n * (K + 8) + ph["hb_candidates"] * (K + 4) + ph["hb_unique"] * (2 * K + 4)
+    ns = ph["nstate"]
+    pt2 = n * (K + 8 * ns) + probes * (K + 4) + ph["pairs"] * 8 * ns * 2 + ph["ns2"] * (2 * K + 4 + 8 * ns)
+    out = {"h_build": {"n": n, "stored_entries": nnz, "probes": probes, "bytes": hb, "ms": ph["build_ms"],
+                       "GBs": hb / ph["build_ms"] / 1e6 if ph["build_ms"] > 0 else None},
+           "pt2_select": {"candidates": ph["hb_candidates"], "unique": ph["hb_unique"], "bytes": sel,
+                          "ms": ph["hb_ms"], "GBs": sel / ph["hb_ms"] / 1e6 if ph["hb_ms"] > 0 else None},
+           "pt2_sum": {"external": ph["ns2"], "pairs": ph["pairs"], "probes": probes, "bytes": pt2,
+                       "ms": ph["pt2_sum_ms"], "GBs": pt2 / ph["pt2_sum_ms"] / 1e6 if ph["pt2_sum_ms"] > 0 else None}}
+    if sp["gemm_ms"] > 0:
+        tf = sp["gemm_flops"] / sp["gemm_ms"] / 1e9
+        out["ritz_gemm"] = {"calls": sp["gemm_calls"], "ms": sp["gemm_ms"], "TFLOPs": tf,
+                            "frac_of_fp64_peak": tf / FP64_PEAK_TFLOPS}
+    return out
 
 
 def spmm_kernel_name(bs, mode):
@@ -360,6 +392,7 @@ def main():
                            "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "avg_launch_ms": sp["ms"] / sp["launches"],
                            "launches": sp["launches"], "algorithmic_bytes_per_launch": per}
+        out["phases"] = phase_rooflines(sp["phases"], sp)
         if traffic is not None:
             t, detail = traffic
             out["roofline"]["traffic"] = t

@@ -145,6 +145,17 @@ int vhci_copy_probe(vhci_ctx* ctx, int64_t bytes, double* ms);
  * Measurement support; no counterpart in the reference. */
 int vhci_spmm_profile(vhci_ctx* ctx, int b);
 int vhci_spmm_profile_read(vhci_ctx* ctx, double* out);
+/* While SpMM profiling is on, the eigensolver's Ritz rotations (two
+ * n x m x keep GEMMs per restart, MFMA through rocBLAS) are timed too:
+ * out[0] = total ms, out[1] = flops, out[2] = GEMM calls.  Measurement
+ * support; no counterpart in the reference. */
+int vhci_gemm_profile_read(vhci_ctx* ctx, double* out);
+/* Counters of the last phases (measurement support; no counterpart in the
+ * reference), out[12]: last H build n, stored entries, connection
+ * patterns, build ms; last heat-bath selection (after vhci_pt2: the PT2
+ * one) candidates, unique states, ms; last PT2 external states,
+ * (external, variational) pairs, sum ms; key bytes; PT2 states. */
+int vhci_phase_stats(vhci_ctx* ctx, double* out);
 
 /* ---- whole solve (Perform_HCI + DoPT2 of one input) --------------------
  * Runs the reference's control flow (src/VHCI.cpp:39-114) on a device

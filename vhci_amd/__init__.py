@@ -61,6 +61,8 @@ _SIGS = {
     "vhci_copy_probe": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
     "vhci_spmm_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
     "vhci_spmm_profile_read": (ctypes.c_int, [_vp, _dp]),
+    "vhci_gemm_profile_read": (ctypes.c_int, [_vp, _dp]),
+    "vhci_phase_stats": (ctypes.c_int, [_vp, _dp]),
     "vhci_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _i64p, _i64p, _dp]),
     "vhci_run_ctx": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _ip, _i64p, _dp]),
     "vhci_iteration_sizes": (ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
@@ -310,6 +312,20 @@ class Context:
         out = np.zeros(3)
         self._check(lib().vhci_spmm_profile_read(self.h, _ptr(out, _dp)))
         return float(out[0]), float(out[1]), int(out[2])
+
+    def gemm_profile_read(self):
+        """(total ms, flops, calls) of the Ritz rotation GEMMs while SpMM
+        profiling is on"""
+        out = np.zeros(3)
+        self._check(lib().vhci_gemm_profile_read(self.h, _ptr(out, _dp)))
+        return float(out[0]), float(out[1]), int(out[2])
+
+    def phase_stats(self):
+        out = np.zeros(12)
+        self._check(lib().vhci_phase_stats(self.h, _ptr(out, _dp)))
+        keys = ["n", "nnz", "patterns", "build_ms", "hb_candidates", "hb_unique", "hb_ms", "ns2", "pairs",
+                "pt2_sum_ms", "key_bytes", "nstate"]
+        return dict(zip(keys, out.tolist()))
 
     def copy_probe(self, nbytes):
         ms = ctypes.c_double()

@@ -371,6 +371,14 @@ int vhci_spmm_profile_read(vhci_ctx* ctx, double* out) {
     return guard(ctx, [&] { ctx->eng->spmm_profile_read(out); });
 }
 
+int vhci_gemm_profile_read(vhci_ctx* ctx, double* out) {
+    return guard(ctx, [&] { ctx->eng->gemm_profile_read(out); });
+}
+
+int vhci_phase_stats(vhci_ctx* ctx, double* out) {
+    return guard(ctx, [&] { ctx->eng->phase_stats(out); });
+}
+
 int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
     return guard(ctx, [&] {
         double bytes = 0;

@@ -1599,6 +1599,7 @@ Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), dev
 
 Engine::~Engine() {
     for (hipEvent_t e : prof_.ev) (void)hipEventDestroy(e);
+    for (hipEvent_t e : prof_.gev) (void)hipEventDestroy(e);
     if (blas_) rocblas_destroy_handle(blas_);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
@@ -1846,6 +1847,7 @@ HStats Engine::build_h() {
     }
     h_valid_ = true;
     st.build_ms = ms_since(t0);
+    last_h_ = st;
     return st;
 }
 
@@ -2082,6 +2084,30 @@ void Engine::spmm_profile(int b) {
     prof_.b = b;
     prof_.used = 0;
     prof_.bytes = 0;
+    prof_.gemm_ms = 0;
+    prof_.gemm_flops = 0;
+    prof_.gemm_calls = 0;
+}
+
+void Engine::gemm_profile_read(double out[3]) {
+    out[0] = prof_.gemm_ms;
+    out[1] = prof_.gemm_flops;
+    out[2] = (double)prof_.gemm_calls;
+}
+
+void Engine::phase_stats(double out[12]) const {
+    out[0] = (double)last_h_.n;
+    out[1] = (double)last_h_.nnz_offdiag;
+    out[2] = (double)P_deltas_;
+    out[3] = last_h_.build_ms;
+    out[4] = (double)last_hb_.candidates;
+    out[5] = (double)last_hb_.unique;
+    out[6] = last_hb_.ms;
+    out[7] = (double)last_pt2_.ns2;
+    out[8] = (double)last_pt2_.pairs;
+    out[9] = last_pt2_.sum_ms;
+    out[10] = 16.0 * KW_;
+    out[11] = (double)last_pt2_nopt_;
 }
 
 void Engine::spmm_profile_read(double out[3]) {
@@ -2682,10 +2708,27 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         HIPCHK(hipMemcpyAsync(Gw, G, (size_t)m * m * sizeof(double), hipMemcpyDeviceToDevice, stream_));
         BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, m, Gw, m, th, work_.as<double>(),
                                  info_.as<rocblas_int>()));
+        const bool gprof = prof_.b > 0;
+        if (gprof) {
+            if (prof_.gev.empty()) {
+                prof_.gev.resize(2);
+                for (auto& e : prof_.gev) HIPCHK(hipEventCreate(&e));
+            }
+            HIPCHK(hipEventRecord(prof_.gev[0], stream_));
+        }
         BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, V, ld, Gw, m,
                               &zero, X, ld));
         BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, W, ld, Gw, m,
                               &zero, HX, ld));
+        if (gprof) {
+            HIPCHK(hipEventRecord(prof_.gev[1], stream_));
+            HIPCHK(hipEventSynchronize(prof_.gev[1]));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, prof_.gev[0], prof_.gev[1]));
+            prof_.gemm_ms += t;
+            prof_.gemm_flops += 2.0 * 2.0 * (double)n * m * keep;
+            prof_.gemm_calls += 2;
+        }
         dim3 rg(nblk, nev);
         k_resid_cm<<<rg, 256, 0, stream_>>>(n, HX, X, ld, th, rpb, part_.as<double>());
         k_sum_parts<<<grid_for(nev, 64), 64, 0, stream_>>>(part_.as<double>(), nblk, nev, work_.as<double>());
@@ -2916,6 +2959,10 @@ HbStats Engine::hb_select(const double* cmax_host, int nopt, double eps) {
         default: hb_select_t<16>(eps, st); break;
     }
     st.ms = ms_since(t0);
+    last_hb_ = st;
+    if (std::getenv("VHCI_DEBUG"))
+        std::fprintf(stderr, "[heat-bath] n %lld eps %.3g candidates %lld unique %lld %.1f ms\n", (long long)n, eps,
+                     (long long)st.candidates, (long long)st.unique, st.ms);
     return st;
 }
 
@@ -3129,6 +3176,13 @@ Pt2Stats Engine::pt2(const std::vector<double>& E, int nopt, double eps2, std::v
         case 3: pt2_t<12>(E, nopt, st, dE); break;
         default: pt2_t<16>(E, nopt, st, dE); break;
     }
+    last_pt2_ = st;
+    last_pt2_n_ = n_;
+    last_pt2_nopt_ = nopt;
+    if (std::getenv("VHCI_DEBUG"))
+        std::fprintf(stderr, "[pt2] n %lld candidates %lld external %lld pairs %lld select %.1f ms sum %.1f ms\n",
+                     (long long)n_, (long long)st.candidates, (long long)st.ns2, (long long)st.pairs, st.select_ms,
+                     st.sum_ms);
     return st;
 }
 

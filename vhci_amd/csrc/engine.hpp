@@ -127,6 +127,14 @@ class Engine {
     // total ms, total algorithmic bytes and the launch count since the start.
     void spmm_profile(int b);
     void spmm_profile_read(double out[3]);
+    // the eigensolver's Ritz rotations (two n x m x keep GEMMs per restart)
+    // while SpMM profiling is on: total ms, flops, calls
+    void gemm_profile_read(double out[3]);
+    // last H build: n, stored entries, connection patterns, ms; last
+    // heat-bath selection (the PT2 one after pt2()): candidates, unique, ms;
+    // last PT2: external states, (external, variational) pairs, sum ms; key
+    // bytes; PT2 states
+    void phase_stats(double out[12]) const;
     // algorithmic bytes of one product of b vectors: stored entries (value +
     // column), row pointers, the diagonal, X read once and Y written once
     double spmm_bytes(int b, int mode) const;
@@ -204,7 +212,15 @@ class Engine {
         std::vector<hipEvent_t> ev;  // start, end pairs
         size_t used = 0;
         double bytes = 0;
+        std::vector<hipEvent_t> gev;  // Ritz rotation GEMMs
+        double gemm_ms = 0, gemm_flops = 0;
+        int64_t gemm_calls = 0;
     } prof_;
+    HStats last_h_;
+    HbStats last_hb_;
+    Pt2Stats last_pt2_;
+    int64_t last_pt2_n_ = 0;
+    int last_pt2_nopt_ = 0;
 };
 
 }  // namespace vhci
