@@ -108,11 +108,13 @@ __device__ __forceinline__ bool key_eq(const uint4* __restrict__ keys, uint32_t 
     return eq;
 }
 
+// Word i of a key held in registers.  Every word is masked and or-ed in, so
+// the array is only ever indexed by constants and stays out of scratch.
 template <int NW>
 __device__ __forceinline__ uint32_t key_word(const uint32_t (&w)[NW], int i) {
-    uint32_t r = w[0];
+    uint32_t r = 0;
 #pragma unroll
-    for (int t = 1; t < NW; t++) r = (i == t) ? w[t] : r;
+    for (int t = 0; t < NW; t++) r |= w[t] & (0u - (uint32_t)(i == t));
     return r;
 }
 
@@ -128,8 +130,7 @@ __device__ __forceinline__ void key_add(uint32_t (&w)[NW], int m, int d) {
     const uint32_t add = ((uint32_t)d) << ((m & 3) * 8);
     const int wi = m >> 2;
 #pragma unroll
-    for (int t = 0; t < NW; t++)
-        if (t == wi) w[t] += add;
+    for (int t = 0; t < NW; t++) w[t] += add & (0u - (uint32_t)(t == wi));
 }
 
 template <int NW>
@@ -144,6 +145,39 @@ __device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots,
         if ((uint32_t)(s >> 32) == tag && key_eq<NW>(keys, (uint32_t)s, w)) return (int64_t)(uint32_t)s;
         pos = (pos + 1) & mask;
     }
+}
+
+// ------------------------------------------------------------------------
+// table records in registers.  DeltaEnt and HbVar (16 bytes) are read as one
+// uint4, ElemFC as its FC plus the 16 bytes after it; their byte fields are
+// taken with constant shifts in fully unrolled loops, since a runtime index
+// into a local struct would place it in scratch memory.
+//   DeltaEnt: mode[s] = byte s, delta[s] = byte 6 + s, meta = word 3
+//   HbVar:    d = byte 0, mode[t] = byte 1 + t, delta[t] = byte 7 + t
+//   ElemFC:   d = byte 0, mode[t] = byte 1 + t, pow[t] = byte 7 + t
+// ------------------------------------------------------------------------
+constexpr int kMaxD = 6;  // modes per pattern, variant or FC
+
+__device__ __forceinline__ uint32_t rec_u8(const uint4& r, int i) {
+    const uint32_t w = i < 4 ? r.x : i < 8 ? r.y : i < 12 ? r.z : r.w;
+    return (w >> ((i & 3) * 8)) & 0xffu;
+}
+__device__ __forceinline__ int rec_i8(const uint4& r, int i) { return (int)(int8_t)(uint8_t)rec_u8(r, i); }
+
+template <class T>
+__device__ __forceinline__ uint4 rec_load(const T* base, int64_t i) {
+    static_assert(sizeof(T) == 16, "16-byte record");
+    return reinterpret_cast<const uint4*>(base)[i];
+}
+
+__device__ __forceinline__ int delta_nz(const uint4& e) { return (int)(e.w >> 28); }
+__device__ __forceinline__ uint32_t delta_group(const uint4& e) { return e.w & 0x03ffffffu; }
+
+__device__ __forceinline__ double elem_load(const ElemFC* __restrict__ elem, uint32_t i, uint4& b) {
+    const uint64_t* q = reinterpret_cast<const uint64_t*>(elem + i);
+    const uint64_t u = q[1], v = q[2];
+    b = make_uint4((uint32_t)u, (uint32_t)(u >> 32), (uint32_t)v, (uint32_t)(v >> 32));
+    return __longlong_as_double((long long)q[0]);
 }
 
 // ------------------------------------------------------------------------
@@ -222,27 +256,69 @@ struct HBuildArgs {
 
 // sum over the group's FCs of fc * prod_t L[k][q_start][q_end], in the
 // element order (= the reference's Quartic/Sextic or Cubic/Quintic order)
+// (sgn = -1 applies the pattern's deltas negated)
 __device__ __forceinline__ double group_element(const GroupEnt ge, const uint32_t* __restrict__ fclist,
                                                 const ElemFC* __restrict__ elem, const double* __restrict__ L,
-                                                int nq, const uint8_t* __restrict__ qrow, const DeltaEnt& e, int nz,
-                                                bool row_is_start) {
+                                                int nq, const uint8_t* __restrict__ qrow, const uint4 e, int nz,
+                                                bool row_is_start, int sgn = 1) {
     double v = 0.0;
     for (uint32_t f = 0; f < ge.fc_count; f++) {
-        const ElemFC fc = elem[fclist[ge.fc_start + f]];
-        double x = fc.fc;
-        for (int t = 0; t < fc.d; t++) {
-            const int m = fc.mode[t];
-            const int k = fc.pow[t];
-            const int qr = qrow[m];
-            int dq = 0;
-            for (int s = 0; s < nz; s++) dq = (e.mode[s] == m) ? (int)e.delta[s] : dq;
-            const int qs = row_is_start ? qr : qr + dq;
-            const int qe = row_is_start ? qr + dq : qr;
-            x = x * L[((size_t)k * nq + qs) * kLadderW + (qe - qs + k)];
+        uint4 b;
+        double x = elem_load(elem, fclist[ge.fc_start + f], b);
+        const int d = (int)rec_u8(b, 0);
+#pragma unroll
+        for (int t = 0; t < kMaxD; t++) {
+            if (t < d) {
+                const int m = (int)rec_u8(b, 1 + t);
+                const int k = (int)rec_u8(b, 7 + t);
+                const int qr = qrow[m];
+                int dq = 0;
+#pragma unroll
+                for (int s = 0; s < kMaxD; s++) dq = (s < nz && (int)rec_u8(e, s) == m) ? sgn * rec_i8(e, 6 + s) : dq;
+                const int qs = row_is_start ? qr : qr + dq;
+                const int qe = row_is_start ? qr + dq : qr;
+                x = x * L[((size_t)k * nq + qs) * kLadderW + (qe - qs + k)];
+            }
         }
         v = v + x;
     }
     return v;
+}
+
+// apply pattern e (nz modes) to key t; false when a quantum leaves 0..255
+template <int NW>
+__device__ __forceinline__ bool apply_pattern(const uint4 e, int nz, const uint8_t* __restrict__ sq,
+                                              uint32_t (&t)[NW]) {
+    bool ok = true;
+#pragma unroll
+    for (int s = 0; s < kMaxD; s++) {
+        if (s < nz) {
+            const int m = (int)rec_u8(e, s);
+            const int dl = rec_i8(e, 6 + s);
+            const int nqv = (int)sq[m] + dl;
+            ok = ok && nqv >= 0 && nqv <= 255;
+            key_add<NW>(t, m, dl);
+        }
+    }
+    return ok;
+}
+
+// sum over a list of zero-pattern FCs of fc * prod_t L[k][q][q] (in list order)
+__device__ __forceinline__ double diag_fc_term(const ElemFC* __restrict__ elem, uint32_t i,
+                                               const double* __restrict__ L, int nq,
+                                               const uint8_t* __restrict__ sq) {
+    uint4 b;
+    double x = elem_load(elem, i, b);
+    const int d = (int)rec_u8(b, 0);
+#pragma unroll
+    for (int t = 0; t < kMaxD; t++) {
+        if (t < d) {
+            const int q = sq[rec_u8(b, 1 + t)];
+            const int k = (int)rec_u8(b, 7 + t);
+            x = x * L[((size_t)k * nq + q) * kLadderW + k];
+        }
+    }
+    return x;
 }
 
 template <int NW, bool FILL>
@@ -261,22 +337,15 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         const int p = p0 + lane;
         bool hit = false;
         int64_t j = -1;
-        DeltaEnt e;
+        uint4 e = make_uint4(0u, 0u, 0u, 0u);
         int nz = 0;
         if (p < a.P) {
-            e = a.deltas[p];
-            nz = (int)(e.meta >> 28);
+            e = rec_load(a.deltas, p);
+            nz = delta_nz(e);
             uint32_t t[NW];
 #pragma unroll
             for (int i = 0; i < NW; i++) t[i] = w[i];
-            bool ok = true;
-            for (int s = 0; s < nz; s++) {
-                const int m = e.mode[s];
-                const int nqv = (int)sq[m] + (int)e.delta[s];
-                ok = ok && nqv >= 0 && nqv <= 255;
-                key_add<NW>(t, m, e.delta[s]);
-            }
-            if (ok) {
+            if (apply_pattern<NW>(e, nz, sq, t)) {
                 j = hash_find<NW>(a.slots, a.mask, a.keys, t);
                 hit = j >= 0;
             }
@@ -287,7 +356,7 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         } else {
             if (hit) {
                 const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-                const GroupEnt ge = a.groups[e.meta & 0x03ffffffu];
+                const GroupEnt ge = a.groups[delta_group(e)];
                 // the element starts from the lower basis index, as AnharmPot(min, max)
                 const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, src < j);
                 a.col[pos] = a.inv ? a.inv[j] : (int32_t)j;
@@ -325,12 +394,16 @@ __global__ void k_diag(int64_t n, int M, const uint4* __restrict__ keys, const d
     }
     double v = 0.0;
     for (int f = 0; f < ndfc; f++) {
-        const ElemFC fc = elem[dfc[f]];
-        double x = fc.fc;
-        for (int t = 0; t < fc.d; t++) {
-            const int q = key_q<NW>(w, fc.mode[t]);
-            const int k = fc.pow[t];
-            x = x * L[((size_t)k * nq + q) * kLadderW + k];
+        uint4 b;
+        double x = elem_load(elem, dfc[f], b);
+        const int d = (int)rec_u8(b, 0);
+#pragma unroll
+        for (int t = 0; t < kMaxD; t++) {
+            if (t < d) {
+                const int q = key_q<NW>(w, (int)rec_u8(b, 1 + t));
+                const int k = (int)rec_u8(b, 7 + t);
+                x = x * L[((size_t)k * nq + q) * kLadderW + k];
+            }
         }
         v = v + x;
     }
@@ -1070,47 +1143,58 @@ __device__ void new_insert(const HbGenArgs& a, const uint32_t (&w)[NW], uint64_t
             s = prev;
         }
         if ((uint32_t)(s >> 32) == tag && pool_key_eq<NW>(a.pool_keys, (uint32_t)s, w)) {
-            atomicMin((unsigned long long*)&a.pool_rank[(uint32_t)s], (unsigned long long)rank);
+            // most candidates repeat a state found with a smaller rank: skip
+            // the atomic then (within one selection the stored rank only
+            // decreases, from the fill value down)
+            uint64_t* pr = &a.pool_rank[(uint32_t)s];
+            if (rank < __hip_atomic_load(pr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMin((unsigned long long*)pr, (unsigned long long)rank);
             return;
         }
         pos = (pos + 1) & a.nmask;
     }
 }
 
-// one thread per candidate g: g = (state n, sorted FC, variant) flattened in
-// the reference's loop order, so g itself is the first-discovery rank.
+// candidate g = (state n, sorted FC, variant) flattened in the reference's
+// loop order, so g = candoff[n] + r is the first-discovery rank.  One wave
+// per basis state (grid-stride over states): the lanes walk its candidates.
 template <int NW>
 __global__ __launch_bounds__(256) void k_hb_gen(HbGenArgs a) {
-    for (int64_t g = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; g < a.total;
-         g += (int64_t)gridDim.x * blockDim.x) {
-        int64_t lo = 0, hi = a.n;  // largest n with candoff[n] <= g
-        while (hi - lo > 1) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (a.candoff[mid] <= g)
-                lo = mid;
-            else
-                hi = mid;
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    for (int64_t n = wave; n < a.n; n += nwaves) {
+        const int64_t g0 = a.candoff[n], cnt = a.candoff[n + 1] - g0;
+        if (cnt == 0) continue;
+        uint32_t w0[NW];
+        key_load<NW>(a.keys, n, w0);
+        for (int64_t r = lane; r < cnt; r += 64) {
+            const uint4 v = rec_load(a.vars, r);
+            const int d = (int)rec_u8(v, 0);
+            uint32_t w[NW];
+#pragma unroll
+            for (int i = 0; i < NW; i++) w[i] = w0[i];
+            bool ok = true;
+            bool over = false;
+#pragma unroll
+            for (int t = 0; t < kMaxD; t++) {
+                if (t < d) {
+                    const int q = key_q<NW>(w, (int)rec_u8(v, 1 + t)) + rec_i8(v, 7 + t);
+                    ok = ok && q >= 0;
+                    over = over || q > 255;
+                }
+            }
+            if (!ok) continue;
+            if (over) {
+                atomicOr(&a.counters[1], 1u);
+                continue;
+            }
+#pragma unroll
+            for (int t = 0; t < kMaxD; t++)
+                if (t < d) key_add<NW>(w, (int)rec_u8(v, 1 + t), rec_i8(v, 7 + t));
+            if (hash_find<NW>(a.bslots, a.bmask, a.keys, w) >= 0) continue;
+            new_insert<NW>(a, w, (uint64_t)(g0 + r));
         }
-        const int64_t n = lo;
-        const int64_t r = g - a.candoff[n];
-        const HbVar v = a.vars[r];
-        uint32_t w[NW];
-        key_load<NW>(a.keys, n, w);
-        bool ok = true;
-        bool over = false;
-        for (int t = 0; t < v.d; t++) {
-            const int q = key_q<NW>(w, v.mode[t]) + v.delta[t];
-            ok = ok && q >= 0;
-            over = over || q > 255;
-        }
-        if (!ok) continue;
-        if (over) {
-            atomicOr(&a.counters[1], 1u);
-            continue;
-        }
-        for (int t = 0; t < v.d; t++) key_add<NW>(w, v.mode[t], v.delta[t]);
-        if (hash_find<NW>(a.bslots, a.bmask, a.keys, w) >= 0) continue;
-        new_insert<NW>(a, w, (uint64_t)g);
     }
 }
 
@@ -1196,22 +1280,15 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
             const int p = p0 + lane;
             bool hit = false;
             int64_t j = -1;
-            DeltaEnt e;
+            uint4 e = make_uint4(0u, 0u, 0u, 0u);
             int nz = 0;
             if (p < a.P) {
-                e = a.deltas[p];
-                nz = (int)(e.meta >> 28);
+                e = rec_load(a.deltas, p);
+                nz = delta_nz(e);
                 uint32_t t[NW];
 #pragma unroll
                 for (int i = 0; i < NW; i++) t[i] = w[i];
-                bool ok = true;
-                for (int s = 0; s < nz; s++) {
-                    const int m = e.mode[s];
-                    const int nqv = (int)sq[m] + (int)e.delta[s];
-                    ok = ok && nqv >= 0 && nqv <= 255;
-                    key_add<NW>(t, m, e.delta[s]);
-                }
-                if (ok) {
+                if (apply_pattern<NW>(e, nz, sq, t)) {
                     j = hash_find<NW>(a.vslots, a.vmask, a.vkeys, t);
                     hit = j >= 0;
                 }
@@ -1219,7 +1296,7 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
             const uint64_t bal = __ballot(hit);
             if (hit) {
                 const int pos = nh + __popcll(bal & ((1ull << lane) - 1ull));
-                const GroupEnt ge = a.groups[e.meta & 0x03ffffffu];
+                const GroupEnt ge = a.groups[delta_group(e)];
                 // PT2.cpp:70-118: AnharmPot(a, i) starts at the external state
                 hidx[pos] = (int32_t)j;
                 hval[pos] = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, true);
@@ -1251,14 +1328,7 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
         for (int f0 = 0; f0 < a.ndfc || f0 == 0; f0 += kWave) {
             const int f = f0 + lane;
             double x = 0.0;
-            if (f < a.ndfc) {
-                const ElemFC fc = a.elem[a.dfc[f]];
-                x = fc.fc;
-                for (int t = 0; t < fc.d; t++) {
-                    const int q = sq[fc.mode[t]];
-                    x = x * a.ladder[((size_t)fc.pow[t] * a.nq + q) * kLadderW + fc.pow[t]];
-                }
-            }
+            if (f < a.ndfc) x = diag_fc_term(a.elem, a.dfc[f], a.ladder, a.nq, sq);
             sdiag[lane] = x;
             __syncthreads();
             if (lane == 0) {
@@ -1356,18 +1426,11 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
         bool hit = false;
         int64_t idx = -1;
         if (p < a.P) {
-            const DeltaEnt e = a.deltas[p];
-            const int nz = (int)(e.meta >> 28);
+            const uint4 e = rec_load(a.deltas, p);
             uint32_t t[NW];
 #pragma unroll
             for (int i = 0; i < NW; i++) t[i] = w[i];
-            bool ok = true;
-            for (int s = 0; s < nz; s++) {
-                const int nqv = (int)sq[e.mode[s]] + (int)e.delta[s];
-                ok = ok && nqv >= 0 && nqv <= 255;
-                key_add<NW>(t, e.mode[s], e.delta[s]);
-            }
-            if (ok) {
+            if (apply_pattern<NW>(e, delta_nz(e), sq, t)) {
                 idx = hash_find<NW>(a.aslots, a.amask, a.apool, t);
                 hit = idx >= 0;
             }
@@ -1462,12 +1525,11 @@ __global__ __launch_bounds__(64) void k_pt2_num(Pt2NumArgs a) {
             const int64_t h = h0 + lane;
             if (h < s1) {
                 const uint64_t v = a.sval[h];
-                DeltaEnt e = a.deltas[(uint32_t)v];
-                const int nz = (int)(e.meta >> 28);
-                for (int s = 0; s < nz; s++) e.delta[s] = (int8_t)(-e.delta[s]);  // i = a - delta
-                const GroupEnt ge = a.groups[e.meta & 0x03ffffffu];
+                const uint4 e = rec_load(a.deltas, (uint32_t)v);
+                const GroupEnt ge = a.groups[delta_group(e)];
                 hidx[lane] = (int32_t)(v >> 32);
-                hval[lane] = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, true);
+                // i = a - delta: the pattern applied negated from a
+                hval[lane] = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, delta_nz(e), true, -1);
             }
             __syncthreads();
             const int lim = (int)min((int64_t)kWave, s1 - h0);
@@ -1491,14 +1553,7 @@ __global__ __launch_bounds__(64) void k_pt2_num(Pt2NumArgs a) {
         for (int f0 = 0; f0 < a.ndfc || f0 == 0; f0 += kWave) {
             const int f = f0 + lane;
             double x = 0.0;
-            if (f < a.ndfc) {
-                const ElemFC fc = a.elem[a.dfc[f]];
-                x = fc.fc;
-                for (int t = 0; t < fc.d; t++) {
-                    const int q = sq[fc.mode[t]];
-                    x = x * a.ladder[((size_t)fc.pow[t] * a.nq + q) * kLadderW + fc.pow[t]];
-                }
-            }
+            if (f < a.ndfc) x = diag_fc_term(a.elem, a.dfc[f], a.ladder, a.nq, sq);
             sdiag[lane] = x;
             __syncthreads();
             if (lane == 0) {
@@ -2877,6 +2932,10 @@ void Engine::hb_select_t(double eps, HbStats& st) {
         pool_rank_.ensure((size_t)pool_cap * sizeof(uint64_t));
         k_fill<uint64_t><<<grid_for(cap, 256), 256, 0, stream_>>>(nslots_.as<uint64_t>(), (int64_t)cap, kEmpty);
         k_fill<uint32_t><<<1, 4, 0, stream_>>>(counters_.as<uint32_t>(), 4, 0u);
+        // ranks start above every candidate, so a rank read while another
+        // thread inserts is never a value left by an earlier selection
+        k_fill<uint64_t><<<grid_for(pool_cap, 256), 256, 0, stream_>>>(pool_rank_.as<uint64_t>(), (int64_t)pool_cap,
+                                                                       ~0ull);
         HbGenArgs a{};
         a.n = n;
         a.total = total;
@@ -2891,7 +2950,7 @@ void Engine::hb_select_t(double eps, HbStats& st) {
         a.pool_rank = pool_rank_.as<uint64_t>();
         a.counters = counters_.as<uint32_t>();
         a.pool_cap = pool_cap;
-        k_hb_gen<NW><<<grid_for(total, 256), 256, 0, stream_>>>(a);
+        k_hb_gen<NW><<<std::min<unsigned>(grid_for(n * 64, 256), 16384), 256, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
         uint32_t cnt[4];
         HIPCHK(hipMemcpyAsync(cnt, counters_.p, sizeof(cnt), hipMemcpyDeviceToHost, stream_));
