@@ -227,7 +227,7 @@ class Context:
                                  _ptr(diag, _dp)))
         return rp, col[: nnz.value], val[: nnz.value], diag
 
-    def eigsolve(self, nev, tol=1e-12, maxit=500, vectors=True):
+    def eigsolve(self, nev, tol=1e-10, maxit=500, vectors=True):
         n = self.n
         nev = min(nev, n)
         E = np.zeros(nev)
@@ -275,7 +275,7 @@ class Context:
         self._check(lib().vhci_pt2(self.h, _ptr(E, _dp), nopt, float(eps2), _ptr(dE, _dp), ctypes.byref(ns2)))
         return dE, ns2.value
 
-    def run(self, tol=1e-12, restart=False):
+    def run(self, tol=1e-10, restart=False):
         """The whole select -> build -> diagonalize -> PT2 loop on this context."""
         nmax = max(self.problem.neig, self.n, 1)
         E = np.zeros(nmax)
@@ -341,7 +341,7 @@ def basis_checksum(q):
         return int(np.sum(v * pw, dtype=np.uint64))
 
 
-def solve(problem, form=FP64, device=0, tol=1e-12):
+def solve(problem, form=FP64, device=0, tol=1e-10):
     """Whole select -> build -> diagonalize -> PT2 run of one problem."""
     L = lib()
     nmax = max(problem.neig, problem.n0, 1)

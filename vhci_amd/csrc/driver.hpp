@@ -14,7 +14,7 @@ namespace vhci {
 namespace drv {
 
 struct RunOpts {
-    double tol = 1e-12;
+    double tol = 1e-10;  // as the reference's Spectra call (src/Ham.cpp:363)
     int maxit = 500;
     std::string cpname = "none";
     bool restart = false;

@@ -324,6 +324,10 @@ __device__ __forceinline__ double diag_fc_term(const ElemFC* __restrict__ elem, 
 template <int NW, bool FILL>
 __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     __shared__ uint8_t sq[64];
+    // fill pass: hits wait here (in pattern order) until 64 of them can be
+    // evaluated at once, one per lane, instead of a few lanes per chunk
+    __shared__ int32_t qj[2 * kWave];
+    __shared__ int32_t qp[2 * kWave];
     const int lane = threadIdx.x;
     const int64_t row = blockIdx.x;                      // stored row
     const int64_t src = a.perm ? a.perm[row] : row;      // its basis state
@@ -333,19 +337,38 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     __syncthreads();
     int64_t base = FILL ? a.rowptr[row] : 0;
     int64_t cnt = 0;
+    int nq = 0;
+    // evaluate and store the first `count` (<= 64) queued hits
+    auto drain = [&](int count) {
+        if (lane < count) {
+            const int p = qp[lane];
+            const int64_t j = qj[lane];
+            const uint4 e = rec_load(a.deltas, p);
+            const GroupEnt ge = a.groups[delta_group(e)];
+            // the element starts from the lower basis index, as AnharmPot(min, max)
+            const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, delta_nz(e), src < j);
+            const int64_t pos = base + lane;
+            a.col[pos] = a.inv ? a.inv[j] : (int32_t)j;
+            if (a.form == kFP64) {
+                a.val[pos] = v;
+            } else {
+                const float h = (float)v;
+                a.hi[pos] = h;
+                a.lo[pos] = (float)(v - (double)h);
+            }
+        }
+        base += count;
+    };
     for (int p0 = 0; p0 < a.P; p0 += kWave) {
         const int p = p0 + lane;
         bool hit = false;
         int64_t j = -1;
-        uint4 e = make_uint4(0u, 0u, 0u, 0u);
-        int nz = 0;
         if (p < a.P) {
-            e = rec_load(a.deltas, p);
-            nz = delta_nz(e);
+            const uint4 e = rec_load(a.deltas, p);
             uint32_t t[NW];
 #pragma unroll
             for (int i = 0; i < NW; i++) t[i] = w[i];
-            if (apply_pattern<NW>(e, nz, sq, t)) {
+            if (apply_pattern<NW>(e, delta_nz(e), sq, t)) {
                 j = hash_find<NW>(a.slots, a.mask, a.keys, t);
                 hit = j >= 0;
             }
@@ -355,22 +378,25 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
             cnt += __popcll(bal);
         } else {
             if (hit) {
-                const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-                const GroupEnt ge = a.groups[delta_group(e)];
-                // the element starts from the lower basis index, as AnharmPot(min, max)
-                const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, src < j);
-                a.col[pos] = a.inv ? a.inv[j] : (int32_t)j;
-                if (a.form == kFP64) {
-                    a.val[pos] = v;
-                } else {
-                    const float h = (float)v;
-                    a.hi[pos] = h;
-                    a.lo[pos] = (float)(v - (double)h);
-                }
+                const int slot = nq + __popcll(bal & ((1ull << lane) - 1ull));
+                qj[slot] = (int32_t)j;
+                qp[slot] = p;
             }
-            base += __popcll(bal);
+            nq += __popcll(bal);
+            __syncthreads();
+            if (nq >= kWave) {
+                drain(kWave);
+                __syncthreads();
+                if (lane < nq - kWave) {
+                    qj[lane] = qj[kWave + lane];
+                    qp[lane] = qp[kWave + lane];
+                }
+                nq -= kWave;
+                __syncthreads();
+            }
         }
     }
+    if (FILL && nq > 0) drain(nq);
     if (!FILL && lane == 0) a.rowcnt[row] = cnt;
 }
 
@@ -798,8 +824,9 @@ __global__ __launch_bounds__(256) void k_cm_to_rm(const double* __restrict__ A, 
 }
 
 // A (column-major, lda) <- B (row-major, ldb)
+// (A2, when not null, receives a second copy with the same leading dim)
 __global__ __launch_bounds__(256) void k_rm_to_cm(const double* __restrict__ B, int ldb, int64_t n, int k,
-                                                  double* __restrict__ A, int64_t lda) {
+                                                  double* __restrict__ A, int64_t lda, double* __restrict__ A2) {
     __shared__ double tile[32][33];
     const int64_t r0 = (int64_t)blockIdx.x * 32;
     const int c0 = blockIdx.y * 32;
@@ -813,7 +840,10 @@ __global__ __launch_bounds__(256) void k_rm_to_cm(const double* __restrict__ B, 
     for (int j = ty; j < 32; j += 8) {
         const int64_t r = r0 + tx;
         const int c = c0 + j;
-        if (r < n && c < k) A[r + (int64_t)c * lda] = tile[tx][j];
+        if (r < n && c < k) {
+            A[r + (int64_t)c * lda] = tile[tx][j];
+            if (A2) A2[r + (int64_t)c * lda] = tile[tx][j];
+        }
     }
 }
 
@@ -947,28 +977,32 @@ __global__ __launch_bounds__(256) void k_tsm_tn(int64_t n, int p, int q, const d
         }
 }
 
-// C[a + c*ldc] = sum over chunks: 64 outputs per block, the chunks split
-// over 4 waves, combined in a fixed order (deterministic)
+// C[a + c*ldc] = sum over chunks: 16 outputs per block, the chunks split
+// over 16 lanes groups (chunk w, w + 16, ...), combined in a fixed order
+// (deterministic)
 __global__ __launch_bounds__(256) void k_tsm_reduce(const double* __restrict__ part, int nchunk, int p, int q,
                                                     double* __restrict__ C, int ldc) {
-    __shared__ double red[4][64];
-    const int o = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int64_t t = (int64_t)blockIdx.x * 64 + o;
+    __shared__ double red[16][17];
+    const int o = threadIdx.x & 15, w = threadIdx.x >> 4;
+    const int64_t t = (int64_t)blockIdx.x * 16 + o;
     const int64_t pq = (int64_t)p * q;
     double s0 = 0.0, s1 = 0.0;
     if (t < pq) {
         int k = w;
-        for (; k + 4 < nchunk; k += 8) {
+        for (; k + 16 < nchunk; k += 32) {
             s0 += part[(int64_t)k * pq + t];
-            s1 += part[(int64_t)(k + 4) * pq + t];
+            s1 += part[(int64_t)(k + 16) * pq + t];
         }
         if (k < nchunk) s0 += part[(int64_t)k * pq + t];
     }
     red[w][o] = s0 + s1;
     __syncthreads();
     if (w == 0 && t < pq) {
+        double s = 0.0;
+#pragma unroll
+        for (int i = 0; i < 16; i++) s += red[i][o];
         const int c = (int)(t / p), a = (int)(t - (int64_t)c * p);
-        C[a + (int64_t)c * ldc] = (red[0][o] + red[1][o]) + (red[2][o] + red[3][o]);
+        C[a + (int64_t)c * ldc] = s;
     }
 }
 
@@ -2236,7 +2270,7 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
     part.ensure((size_t)nchunk * p * q * sizeof(double));
     dim3 g(nchunk, (p + 63) / 64, (q + 15) / 16);
     k_tsm_tn<<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
-    k_tsm_reduce<<<grid_for((int64_t)p * q, 64), 256, 0, s>>>(part.as<double>(), nchunk, p, q, C, ldc);
+    k_tsm_reduce<<<grid_for((int64_t)p * q, 16), 256, 0, s>>>(part.as<double>(), nchunk, p, q, C, ldc);
     HIPCHK(hipGetLastError());
 }
 
@@ -2305,7 +2339,7 @@ static void small_eigh(int k, std::vector<double> A, std::vector<double>& lam, s
 // Y = H X for the k column-major vectors of X (ld), through row-major
 // staging in chunks of at most 64 columns.  low selects the mixed form's
 // FP32 product.
-void Engine::apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st) {
+void Engine::apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st, double* Y2) {
     const int64_t n = n_;
     const bool f32 = low && form_ == kMixed;
     for (int c = 0; c < k; c += 64) {
@@ -2321,7 +2355,8 @@ void Engine::apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, E
         } else {
             spmm(stage_in_.as<double>(), kk, stage_out_.as<double>(), kk, kk, false);
         }
-        k_rm_to_cm<<<tg, 256, 0, stream_>>>(stage_out_.as<double>(), kk, n, kk, Y + c * ld, ld);
+        k_rm_to_cm<<<tg, 256, 0, stream_>>>(stage_out_.as<double>(), kk, n, kk, Y + c * ld, ld,
+                                            Y2 ? Y2 + c * ld : nullptr);
         HIPCHK(hipGetLastError());
         st.spmm_calls++;
     }
@@ -2733,9 +2768,8 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     for (int it = 0; it < maxit; it++) {
         st.iterations = it + 1;
         for (int j = k; j < m; j += bs) {
-            apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st);
-            HIPCHK(hipMemcpyAsync(V + (int64_t)(j + bs) * ld, W + (int64_t)j * ld, (size_t)n * bs * sizeof(double),
-                                  hipMemcpyDeviceToDevice, stream_));
+            // W_j = H V_j, also written to V_(j+bs) as the next block's start
+            apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st, V + (int64_t)(j + bs) * ld);
             if (j > k) {
                 // Lanczos step: H V_j has large components only along V_j and
                 // V_(j-bs) (the block-tridiagonal part of the projected

@@ -147,7 +147,7 @@ class Engine {
               int ldx32 = 0);
     void dense_solve(int nev, std::vector<double>& E, EigStats& st);
     // column-major block helpers of the iterative eigensolvers
-    void apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st);
+    void apply_cm(const double* X, double* Y, int64_t ld, int k, bool low, EigStats& st, double* Y2 = nullptr);
     // orthonormalise the k columns at V + m0*ld against the first m0 and
     // among themselves; lanczos: the block already had its local Lanczos
     // components removed, so the first projection is applied only when one

@@ -60,7 +60,7 @@ int64_t Server::submit(const std::string& text, int form, double tol) {
     if (!err.empty()) throw std::invalid_argument(err);
     if (form != 0 && form != 1) throw std::invalid_argument("unknown precision form");
     r->form = form;
-    r->tol = tol > 0 ? tol : 1e-12;
+    r->tol = tol > 0 ? tol : 1e-10;
     Tables T;
     build_tables(r->P, T);
     // first heat-bath iteration: assume 64-fold growth of the initial basis

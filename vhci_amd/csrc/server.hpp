@@ -35,7 +35,7 @@ struct Request {
     int64_t id = 0;
     Problem P;
     int form = 0;
-    double tol = 1e-12;
+    double tol = 1e-10;
     State state = State::Queued;
     std::string err;
     drv::RunResult R;
