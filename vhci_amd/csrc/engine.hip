@@ -1530,15 +1530,17 @@ struct Pt2NumArgs {
 
 // one wave per external state: numerators sum_i H_ai C_in over its pairs in
 // ascending i (FP64: product then sum, as the reference's SumHaiCi), E_a,
-// then Kahan sums of num^2 / (E_n - E_a) per column
-template <int NW, bool MIXED>
+// then Kahan sums of num^2 / (E_n - E_a) per column.  R = columns / 64
+// (nopt <= 64 R); the C rows of U pairs are loaded before they are summed
+// in order, so U row reads are in flight at once.
+template <int NW, bool MIXED, int R>
 __global__ __launch_bounds__(64) void k_pt2_num(Pt2NumArgs a) {
     __shared__ uint8_t sq[64];
     __shared__ int32_t hidx[64];
     __shared__ double hval[64];
     __shared__ double sdiag[64];
     const int lane = threadIdx.x;
-    constexpr int R = 4;
+    constexpr int U = 8;
     double ks[R], kc[R];
 #pragma unroll
     for (int r = 0; r < R; r++) ks[r] = kc[r] = 0.0;
@@ -1567,17 +1569,33 @@ __global__ __launch_bounds__(64) void k_pt2_num(Pt2NumArgs a) {
             }
             __syncthreads();
             const int lim = (int)min((int64_t)kWave, s1 - h0);
-            for (int l = 0; l < lim; l++) {
-                const int64_t i = hidx[l];
-                const double hv = hval[l];
+            for (int l0 = 0; l0 < lim; l0 += U) {
+                double cv[U][R];
+                float cv32[U][R];
 #pragma unroll
-                for (int r = 0; r < R; r++) {
-                    const int c = lane + 64 * r;
-                    if (c < a.nopt) {
+                for (int u = 0; u < U; u++) {
+                    const int64_t i = hidx[min(l0 + u, lim - 1)];
+#pragma unroll
+                    for (int r = 0; r < R; r++) {
+                        const int c = lane + 64 * r;
+                        const bool on = c < a.nopt && l0 + u < lim;
                         if (MIXED)
-                            num32[r] = fmaf((float)hv, a.C32[i * a.ldc + c], num32[r]);
+                            cv32[u][r] = on ? a.C32[i * a.ldc + c] : 0.0f;
                         else
-                            num[r] = num[r] + hv * a.C[i * a.ldc + c];
+                            cv[u][r] = on ? a.C[i * a.ldc + c] : 0.0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (l0 + u < lim) {
+                        const double hv = hval[l0 + u];
+#pragma unroll
+                        for (int r = 0; r < R; r++) {
+                            if (MIXED)
+                                num32[r] = fmaf((float)hv, cv32[u][r], num32[r]);
+                            else
+                                num[r] = num[r] + hv * cv[u][r];
+                        }
                     }
                 }
             }
@@ -3183,10 +3201,14 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st) {
     b.ldc = mixed ? nopt : xcols_;
     b.E = Ed;
     b.part = pt2_part_.as<double>();
-    if (mixed)
-        k_pt2_num<NW, true><<<nwaves, 64, 0, stream_>>>(b);
+    if (mixed && nopt <= 128)
+        k_pt2_num<NW, true, 2><<<nwaves, 64, 0, stream_>>>(b);
+    else if (mixed)
+        k_pt2_num<NW, true, 4><<<nwaves, 64, 0, stream_>>>(b);
+    else if (nopt <= 128)
+        k_pt2_num<NW, false, 2><<<nwaves, 64, 0, stream_>>>(b);
     else
-        k_pt2_num<NW, false><<<nwaves, 64, 0, stream_>>>(b);
+        k_pt2_num<NW, false, 4><<<nwaves, 64, 0, stream_>>>(b);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
     return true;
