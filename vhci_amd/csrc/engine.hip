@@ -920,36 +920,38 @@ __global__ void k_start_block(int64_t n, int b, int k0, const double* __restrict
 // tall-skinny products of the Krylov basis (n >> p, q)
 // ------------------------------------------------------------------------
 // partial C = A^T B over one row chunk: A n x p, B n x q, column-major.
-// Block: 64 columns of A x 16 columns of B; 32-row tiles staged in LDS with
-// loads along the rows (coalesced), 4 outputs per thread.
+// Block: AW columns of A x QW columns of B (AW * QW = 1024: 64 x 16, or
+// 128 x 8 when q <= 8); 32-row tiles staged in LDS with loads along the rows
+// (coalesced), 4 outputs per thread.
 // part layout: [chunk][c][a] (c < q, a < p)
+template <int AW, int QW>
 __global__ __launch_bounds__(256) void k_tsm_tn(int64_t n, int p, int q, const double* __restrict__ A, int64_t lda,
                                                 const double* __restrict__ B, int64_t ldb, int64_t rows_per_chunk,
                                                 double* __restrict__ part) {
-    __shared__ double As[32][65];
-    __shared__ double Bs[32][17];
+    static_assert(AW * QW == 1024, "256 threads x 4 outputs");
+    constexpr int NA = AW / 8, NB = QW / 8;  // staged values per thread
+    __shared__ double As[32][AW + 1];
+    __shared__ double Bs[32][QW + 1];
     const int t = threadIdx.x;
-    const int a0 = blockIdx.y * 64, c0 = blockIdx.z * 16;
+    const int a0 = blockIdx.y * AW, c0 = blockIdx.z * QW;
     const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
     const int64_t r1 = min(n, r0 + rows_per_chunk);
-    const int aa = t & 63, cg = t >> 6;  // outputs (aa, 4*cg .. 4*cg+3)
-    // each thread stages 8 A values and 2 B values per 32-row tile; the next
-    // tile is fetched into registers while the current one is consumed
-    const int ai = t & 31, aj = t >> 5;  // A: rows ai, columns aj + 8 u
-    const int bi = t & 31, bj = t >> 5;  // B: rows bi, columns bj, bj + 8
-    double ra[8], rb[2];
+    const int aa = t % AW, cg = t / AW;  // outputs (aa, 4*cg .. 4*cg+3)
+    // each thread stages NA values of A and NB of B per 32-row tile; the
+    // next tile is fetched into registers while the current one is consumed
+    const int ai = t & 31, aj = t >> 5;  // rows ai, columns aj + 8 u
+    double ra[NA], rb[NB];
     auto fetch = [&](int64_t rt) {
         const int64_t r = rt + ai;
 #pragma unroll
-        for (int u = 0; u < 8; u++) {
+        for (int u = 0; u < NA; u++) {
             const int j = aj + 8 * u;
             ra[u] = (r < r1 && a0 + j < p) ? A[r + (int64_t)(a0 + j) * lda] : 0.0;
         }
-        const int64_t rbb = rt + bi;
 #pragma unroll
-        for (int u = 0; u < 2; u++) {
-            const int j = bj + 8 * u;
-            rb[u] = (rbb < r1 && c0 + j < q) ? B[rbb + (int64_t)(c0 + j) * ldb] : 0.0;
+        for (int u = 0; u < NB; u++) {
+            const int j = aj + 8 * u;
+            rb[u] = (r < r1 && c0 + j < q) ? B[r + (int64_t)(c0 + j) * ldb] : 0.0;
         }
     };
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -957,9 +959,9 @@ __global__ __launch_bounds__(256) void k_tsm_tn(int64_t n, int p, int q, const d
     for (int64_t rt = r0; rt < r1; rt += 32) {
         __syncthreads();
 #pragma unroll
-        for (int u = 0; u < 8; u++) As[ai][aj + 8 * u] = ra[u];
+        for (int u = 0; u < NA; u++) As[ai][aj + 8 * u] = ra[u];
 #pragma unroll
-        for (int u = 0; u < 2; u++) Bs[bi][bj + 8 * u] = rb[u];
+        for (int u = 0; u < NB; u++) Bs[ai][aj + 8 * u] = rb[u];
         __syncthreads();
         if (rt + 32 < r1) fetch(rt + 32);
 #pragma unroll 8
@@ -1700,6 +1702,8 @@ Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), dev
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     BLASCHK(rocblas_create_handle(&blas_));
     BLASCHK(rocblas_set_stream(blas_, stream_));
+    // no atomics-based algorithms: the same inputs give the same bits on every run
+    BLASCHK(rocblas_set_atomics_mode(blas_, rocblas_atomics_not_allowed));
     build_tables(P_, T_);
     upload_tables();
 }
@@ -2286,8 +2290,13 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
     const int64_t rpc = std::max<int64_t>(256, ((n + 383) / 384 + 31) / 32 * 32);
     const int nchunk = (int)((n + rpc - 1) / rpc);
     part.ensure((size_t)nchunk * p * q * sizeof(double));
-    dim3 g(nchunk, (p + 63) / 64, (q + 15) / 16);
-    k_tsm_tn<<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
+    if (q <= 8) {
+        dim3 g(nchunk, (p + 127) / 128, 1);
+        k_tsm_tn<128, 8><<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
+    } else {
+        dim3 g(nchunk, (p + 63) / 64, (q + 15) / 16);
+        k_tsm_tn<64, 16><<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
+    }
     k_tsm_reduce<<<grid_for((int64_t)p * q, 16), 256, 0, s>>>(part.as<double>(), nchunk, p, q, C, ldc);
     HIPCHK(hipGetLastError());
 }
@@ -2810,7 +2819,8 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
             }
             if (kept < bs) throw std::runtime_error("Krylov-Schur: could not extend the basis");
         }
-        // G[:, k:m] = V[:, :m]^T W[:, k:m] (upper triangle of the new columns)
+        // G[:, k:m] = V[:, :m]^T W[:, k:m] (upper triangle of the new columns).
+        // rocBLAS is slow for this shape (K = n): 2.3 TFLOP/s measured.
         tsm_tn(stream_, tsm_part_, n, m, m - k, V, ld, W + (int64_t)k * ld, ld, G + (int64_t)k * m, m);
         HIPCHK(hipMemcpyAsync(Gw, G, (size_t)m * m * sizeof(double), hipMemcpyDeviceToDevice, stream_));
         BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, m, Gw, m, th, work_.as<double>(),
