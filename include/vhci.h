@@ -72,6 +72,23 @@ int vhci_problem_fc(const vhci_problem* p, int which, int idx, double* fc, int* 
  * order (CreationLO/AnnihilationLO/AnharmPot, src/Ham.cpp:16-104). */
 double vhci_ladder_sum(int k, int qs, int qe);
 
+/* Checkpoint files (host only).  Write replaces WriteCheckpoint
+ * (src/Core_functions.cpp:94-117): n M NEig, then the frequencies and the nE
+ * energies, then one line per state with its M quanta and ncol coefficients
+ * (C row-major n x ncol), the reals in fixed notation with 17 decimals,
+ * formatted on `threads` host threads.  binary != 0 writes the opt-in
+ * binary form (same content).  Read replaces ReadCheckpoint
+ * (src/Core_functions.cpp:119-167) for either form with the reference's
+ * error messages (through vhci_problem_error): *n receives the number of
+ * states; with q (n*M), E and C (n*NEig) non-null and qcap, ccap (in
+ * elements) large enough the basis, the NEig energies and coefficients are
+ * filled; with q, E or C null only the header is read, *n is set and
+ * VHCI_E_ARG returned (a size query). */
+int vhci_checkpoint_write(const vhci_problem* p, const char* path, const int32_t* q, int64_t n, const double* E,
+                          int nE, const double* C, int ncol, int threads, int binary);
+int vhci_checkpoint_read(const vhci_problem* p, const char* path, int threads, int64_t* n, int32_t* q,
+                         int64_t qcap, double* E, double* C, int64_t ccap);
+
 /* ---- device context ---------------------------------------------------- */
 
 int vhci_ctx_create(const vhci_problem* p, int form, int device, vhci_ctx** out);

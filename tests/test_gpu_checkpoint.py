@@ -90,3 +90,23 @@ def test_damaged_checkpoints(tmp_path):
     empty.write_text("")
     r = _vhci(["-n", "1", "-i", inp, "-o", str(tmp_path / "y.freqs"), "-c", str(empty)])
     assert r.returncode == 0 and "No checkpoint file found. Calculation will start from the beginning." in r.stdout
+
+
+def test_resume_from_binary_checkpoint(c1):
+    """The opt-in binary checkpoint (VHCI_CHECKPOINT_FORMAT=binary) keeps the
+    coefficients' bits, so the resumed run repeats the uninterrupted one."""
+    d, inp, full_log = c1
+    ck = str(d / "c.ck")
+    env = {"VHCI_CHECKPOINT_FORMAT": "binary"}
+    r = _vhci(["-n", "2", "-i", inp, "-o", str(d / "c.freqs"), "-c", ck],
+              env=dict(env, VHCI_STOP_AFTER_ITERATION="2"))
+    assert r.returncode != 0 and "Writing checkpoint file...done." in r.stdout
+    with open(ck, "rb") as f:
+        assert f.read(8) == b"VHCICKB1"
+    r2 = _vhci(["-n", "2", "-i", inp, "-o", str(d / "c.freqs"), "-c", ck], env=env)
+    assert r2.returncode == 0, r2.stdout[-2000:]
+    assert "Checkpoint file found. Calculation will be resumed." in r2.stdout
+    assert _sizes(r2.stdout) == _sizes(full_log)[2:]
+    e, _ = read_freqs(str(d / "c.freqs"))
+    ef, _ = read_freqs(str(d / "full.freqs"))
+    assert np.max(np.abs(absolute(e) - absolute(ef))) < 1e-9

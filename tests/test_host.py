@@ -107,3 +107,96 @@ def test_synthetic_generator_deterministic(tmp_path):
     # first draws of the xoshiro256** stream seeded by splitmix64(0)
     r = synth.Rng(0)
     assert [r.next() for _ in range(3)] == [11091344671253066420, 13793997310169335082, 1900383378846508768]
+
+
+# ---- checkpoint files (WriteCheckpoint / ReadCheckpoint) ------------------
+
+def _ck_data(P, n, ncol, seed=5):
+    r = np.random.default_rng(seed)
+    q = r.integers(0, 7, size=(n, P.M)).astype(np.int32)
+    E = np.sort(r.uniform(1000, 20000, size=ncol))
+    C = r.standard_normal((n, ncol)) * 10.0 ** r.integers(-19, 1, size=(n, ncol))
+    C[0, 0] = 0.0
+    C[0, 1] = -0.0
+    C[1, 0] = 1.0 - 2.0 ** -53
+    return q, E, C
+
+
+def _ck_text(P, q, E, C):
+    """The reference's checkpoint text: fixed notation with 17 decimals
+    (std::fixed << std::setprecision(17), i.e. printf %.17f), one space after
+    every value, one line per state."""
+    n = q.shape[0]
+    lines = ["%d %d %d\n" % (n, P.M, P.neig)]
+    lines.append("".join("%.17f " % x for x in list(P.freqs()) + list(E)) + "\n")
+    for i in range(n):
+        lines.append("".join("%d " % x for x in q[i]) + "".join("%.17f " % x for x in C[i]) + "\n")
+    return "".join(lines).encode()
+
+
+@pytest.fixture(scope="module")
+def ck_problem():
+    return vhci_amd.Problem(os.path.join(GOLDEN, "clo2_vhci.inp"))
+
+
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_checkpoint_text_is_the_reference_format(ck_problem, tmp_path, threads):
+    P = ck_problem
+    q, E, C = _ck_data(P, 5000, P.neig)
+    path = str(tmp_path / "t.ck")
+    P.write_checkpoint(path, q, E, C, threads=threads)
+    with open(path, "rb") as f:
+        assert f.read() == _ck_text(P, q, E, C)
+
+
+def test_checkpoint_text_round_trip(ck_problem, tmp_path):
+    P = ck_problem
+    q, E, C = _ck_data(P, 7000, P.neig, seed=9)
+    path = str(tmp_path / "r.ck")
+    P.write_checkpoint(path, q, E, C, threads=4)
+    for threads in (1, 5):
+        q2, E2, C2 = P.read_checkpoint(path, threads=threads)
+        assert np.array_equal(q2, q)
+        # the values come back as the 17-decimal text rounds them
+        assert np.array_equal(E2, np.array([float("%.17f" % x) for x in E]))
+        assert np.array_equal(C2, np.vectorize(lambda x: float("%.17f" % x))(C))
+
+
+def test_checkpoint_binary_round_trip_is_exact(ck_problem, tmp_path):
+    P = ck_problem
+    q, E, C = _ck_data(P, 3000, P.neig, seed=11)
+    path = str(tmp_path / "b.ck")
+    P.write_checkpoint(path, q, E, C, binary=True)
+    with open(path, "rb") as f:
+        assert f.read(8) == b"VHCICKB1"
+    q2, E2, C2 = P.read_checkpoint(path)
+    assert np.array_equal(q2, q) and np.array_equal(E2, E) and np.array_equal(C2, C)
+
+
+def test_checkpoint_errors(ck_problem, tmp_path):
+    P = ck_problem
+    q, E, C = _ck_data(P, 100, P.neig)
+    good = str(tmp_path / "g.ck")
+    P.write_checkpoint(good, q, E, C)
+    text = open(good, "rb").read()
+    lines = text.split(b"\n")
+
+    def bad(name, data, msg):
+        p = str(tmp_path / name)
+        with open(p, "wb") as f:
+            f.write(data)
+        with pytest.raises(vhci_amd.VhciError, match=re.escape(msg)):
+            P.read_checkpoint(p)
+
+    n, M, ne = lines[0].split()
+    bad("m.ck", b" ".join([n, b"%d" % (int(M) + 1), ne]) + b"\n" + b"\n".join(lines[1:]),
+        "wrong number of fundamental modes")
+    bad("e.ck", b" ".join([n, M, b"%d" % (int(ne) + 1)]) + b"\n" + b"\n".join(lines[1:]),
+        "wrong number of eigenstates")
+    f0 = lines[1].split(b" ")
+    f0[0] = b"%.17f" % (float(f0[0]) + 1.0)
+    bad("f.ck", b"\n".join([lines[0], b" ".join(f0)] + lines[2:]), "wrong fundamental frequencies")
+    bad("t.ck", text[: len(text) // 2], "truncated")
+    P.write_checkpoint(str(tmp_path / "bb.ck"), q, E, C, binary=True)
+    bb = open(str(tmp_path / "bb.ck"), "rb").read()
+    bad("bt.ck", bb[: len(bb) - 8], "truncated")

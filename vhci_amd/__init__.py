@@ -36,6 +36,10 @@ _SIGS = {
     "vhci_problem_real": (ctypes.c_double, [_vp, ctypes.c_int]),
     "vhci_problem_freqs": (ctypes.c_int, [_vp, _dp]),
     "vhci_problem_initial_basis": (ctypes.c_int, [_vp, _i32p]),
+    "vhci_checkpoint_write": (ctypes.c_int, [_vp, ctypes.c_char_p, _i32p, ctypes.c_int64, _dp, ctypes.c_int, _dp,
+                                             ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "vhci_checkpoint_read": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                            _i32p, ctypes.c_int64, _dp, _dp, ctypes.c_int64]),
     "vhci_problem_fc_count": (ctypes.c_int, [_vp, ctypes.c_int]),
     "vhci_problem_fc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _ip, _ip]),
     "vhci_ladder_sum": (ctypes.c_double, [ctypes.c_int, ctypes.c_int, ctypes.c_int]),
@@ -145,6 +149,38 @@ class Problem:
         q = np.zeros((self.n0, self.M), dtype=np.int32)
         lib().vhci_problem_initial_basis(self.h, _ptr(q, _i32p))
         return q
+
+    def write_checkpoint(self, path, q, E, C, threads=1, binary=False):
+        """The checkpoint file of basis q (n x M), energies E and coefficients
+        C (n x ncol), as the reference's WriteCheckpoint writes it (or the
+        opt-in binary form)."""
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        if C.ndim != 2 or q.shape != (C.shape[0], self.M):
+            raise ValueError("q must be n x M and C n x ncol")
+        rc = lib().vhci_checkpoint_write(self.h, os.fsencode(path), _ptr(q, _i32p), q.shape[0], _ptr(E, _dp), E.size,
+                                         _ptr(C, _dp), C.shape[1], int(threads), int(bool(binary)))
+        if rc != 0:
+            raise VhciError(lib().vhci_problem_error().decode())
+
+    def read_checkpoint(self, path, threads=1):
+        """(q n x M, E NEig, C n x NEig) from a checkpoint file of either form."""
+        L = lib()
+        n = ctypes.c_int64()
+        rc = L.vhci_checkpoint_read(self.h, os.fsencode(path), int(threads), ctypes.byref(n), None, 0, None, None, 0)
+        if rc == -2:
+            raise VhciError(L.vhci_problem_error().decode())
+        if n.value < 1:
+            raise VhciError("Error: checkpoint file is truncated.")
+        q = np.zeros((n.value, self.M), dtype=np.int32)
+        E = np.zeros(self.neig)
+        C = np.zeros((n.value, self.neig))
+        rc = L.vhci_checkpoint_read(self.h, os.fsencode(path), int(threads), ctypes.byref(n), _ptr(q, _i32p), q.size,
+                                    _ptr(E, _dp), _ptr(C, _dp), C.size)
+        if rc != 0:
+            raise VhciError(L.vhci_problem_error().decode())
+        return q, E, C
 
     def fc_list(self, which):
         """[(value, order, [(mode, power), ...]), ...] of one FC list."""

@@ -198,6 +198,69 @@ int vhci_problem_initial_basis(const vhci_problem* p, int32_t* q) {
     return VHCI_OK;
 }
 
+int vhci_checkpoint_write(const vhci_problem* p, const char* path, const int32_t* q, int64_t n, const double* E,
+                          int nE, const double* C, int ncol, int threads, int binary) {
+    if (!p || !path || !q || !E || !C || n < 0 || nE < 0 || ncol < 0) return VHCI_E_ARG;
+    try {
+        std::ofstream cf(path, std::ios::out | std::ios::trunc | std::ios::binary);
+        if (!cf.good()) {
+            g_problem_err = "Error: checkpoint file could not be found or no name was given.";
+            return VHCI_E_INPUT;
+        }
+        std::vector<double> ev(E, E + nE);
+        vhci::write_checkpoint(cf, p->P, q, n, ev, C, ncol, std::max(1, threads), binary != 0);
+        return cf.good() ? VHCI_OK : VHCI_E_INPUT;
+    } catch (const std::exception& e) {
+        g_problem_err = e.what();
+        return VHCI_E_INPUT;
+    }
+}
+
+int vhci_checkpoint_read(const vhci_problem* p, const char* path, int threads, int64_t* n, int32_t* q,
+                         int64_t qcap, double* E, double* C, int64_t ccap) {
+    if (!p || !path || !n) return VHCI_E_ARG;
+    try {
+        std::ifstream cf(path, std::ios::in | std::ios::binary);
+        if (!cf.good()) {
+            g_problem_err = "Error: checkpoint file could not be found or no name was given.";
+            return VHCI_E_INPUT;
+        }
+        if (!q || !E || !C) {
+            // size query: the state count from the header alone
+            char head[8 + sizeof(int64_t)] = {};
+            cf.read(head, sizeof(head));
+            if (cf.gcount() == (std::streamsize)sizeof(head) &&
+                std::memcmp(head, vhci::kCheckpointMagic, sizeof(vhci::kCheckpointMagic)) == 0) {
+                std::memcpy(n, head + 8, sizeof(int64_t));
+            } else {
+                cf.clear();
+                cf.seekg(0);
+                if (!(cf >> *n)) {
+                    g_problem_err = "Error: checkpoint file is truncated.";
+                    return VHCI_E_INPUT;
+                }
+            }
+            return VHCI_E_ARG;
+        }
+        std::vector<int32_t> qv;
+        std::vector<double> ev, cv;
+        std::string err = vhci::read_checkpoint(cf, p->P, qv, ev, cv, std::max(1, threads));
+        if (!err.empty()) {
+            g_problem_err = err;
+            return VHCI_E_INPUT;
+        }
+        *n = (int64_t)qv.size() / std::max(1, p->P.M);
+        if (qcap < (int64_t)qv.size() || ccap < (int64_t)cv.size()) return VHCI_E_ARG;
+        std::copy(qv.begin(), qv.end(), q);
+        std::copy(ev.begin(), ev.end(), E);
+        std::copy(cv.begin(), cv.end(), C);
+        return VHCI_OK;
+    } catch (const std::exception& e) {
+        g_problem_err = e.what();
+        return VHCI_E_INPUT;
+    }
+}
+
 static const vhci::ForceConst* fc_at(const Problem& P, int which, int idx, int& count) {
     const std::vector<int>* lst = nullptr;
     switch (which) {
@@ -643,6 +706,8 @@ int vhci_main(int argc, char** argv) {
     RunOpts o;
     if (const char* t = std::getenv("VHCI_TOL")) o.tol = std::atof(t);
     if (const char* t = std::getenv("VHCI_STOP_AFTER_ITERATION")) o.stop_after = std::atoi(t);
+    if (const char* t = std::getenv("VHCI_CHECKPOINT_FORMAT")) o.binary_checkpoint = std::string(t) == "binary";
+    o.threads = ncpus;
     o.cpname = cpname;
     o.restart = restart;
     o.log = true;
@@ -653,10 +718,10 @@ int vhci_main(int argc, char** argv) {
         if (restart) {
             std::cout << "\nLoading checkpoint file...";
             std::cout.flush();
-            std::ifstream cf(cpname);
+            std::ifstream cf(cpname, std::ios::in | std::ios::binary);
             std::vector<int32_t> q;
             std::vector<double> C;
-            std::string ce = vhci::read_checkpoint(cf, P, q, R.E, C);
+            std::string ce = vhci::read_checkpoint(cf, P, q, R.E, C, ncpus);
             if (!ce.empty()) {
                 std::cout << ce << "\n";
                 return 1;

@@ -31,7 +31,7 @@ std::string sparse_line(const vhci::HStats& h) {
 
 void save_checkpoint(Engine& eng, const Problem& P, const std::vector<double>& E, const RunOpts& o,
                      const Logger& log) {
-    std::ofstream cf(o.cpname, std::ios::out | std::ios::trunc);
+    std::ofstream cf(o.cpname, std::ios::out | std::ios::trunc | std::ios::binary);
     if (!cf.good() || o.cpname == "none") {
         log("\n Error: checkpoint file could not be found or no name was given.\n\n");
         return;
@@ -42,7 +42,7 @@ void save_checkpoint(Engine& eng, const Problem& P, const std::vector<double>& E
     std::vector<double> C((size_t)n * nc);
     eng.get_basis(q.data());
     eng.get_vectors(C.data(), nc);
-    vhci::write_checkpoint(cf, P, q.data(), n, E, C.data(), nc);
+    vhci::write_checkpoint(cf, P, q.data(), n, E, C.data(), nc, o.threads, o.binary_checkpoint);
 }
 
 }  // namespace

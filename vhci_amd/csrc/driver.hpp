@@ -20,6 +20,8 @@ struct RunOpts {
     bool restart = false;
     bool log = false;
     int stop_after = 0;  // failure injection: end the run after this HCI iteration
+    int threads = 1;     // host threads for checkpoint formatting (-n)
+    bool binary_checkpoint = false;  // opt-in binary checkpoint form
     // called after every basis growth with the new basis size (serving
     // loop: memory reservation updates)
     std::function<void(int64_t)> on_grow;

@@ -3,7 +3,10 @@
 #include "problem.hpp"
 
 #include <algorithm>
+#include <charconv>
+#include <cstring>
 #include <functional>
+#include <thread>
 #include <numeric>
 #include <cmath>
 #include <fstream>
@@ -431,45 +434,229 @@ void write_freqs(std::ostream& os, const Problem& P, const int32_t* basis, int64
     os.flush();
 }
 
-void write_checkpoint(std::ostream& os, const Problem& P, const int32_t* basis, int64_t n,
-                      const std::vector<double>& E, const double* C, int ncol) {
-    const int M = P.M;
-    os << n << " " << M << " " << P.neig << '\n';
-    os << std::fixed << std::setprecision(17);
-    for (int j = 0; j < M; j++) os << P.freq[j] << " ";
-    for (double e : E) os << e << " ";
-    os << '\n';
-    for (int64_t i = 0; i < n; i++) {
-        for (int j = 0; j < M; j++) os << basis[i * M + j] << " ";
-        for (int j = 0; j < ncol; j++) os << C[i * ncol + j] << " ";
-        os << '\n';
+namespace {
+
+// run f(t, lo, hi) over [0, n) split into `threads` contiguous ranges
+void parallel_ranges(int64_t n, int threads, const std::function<void(int, int64_t, int64_t)>& f) {
+    threads = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n / 2048 + 1));
+    if (threads == 1) {
+        f(0, 0, n);
+        return;
     }
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++)
+        pool.emplace_back(f, t, n * t / threads, n * (t + 1) / threads);
+    for (auto& th : pool) th.join();
+}
+
+// "%.17f" as std::fixed << std::setprecision(17) writes it (to_chars with a
+// precision follows printf, in the C locale)
+inline char* put_fixed17(char* p, char* end, double v) {
+    return std::to_chars(p, end, v, std::chars_format::fixed, 17).ptr;
+}
+
+template <class T>
+void put_raw(std::ostream& os, const T* p, size_t count) {
+    os.write(reinterpret_cast<const char*>(p), (std::streamsize)(sizeof(T) * count));
+}
+
+// largest fixed 17-decimal text of a double: sign, 309 integer digits, point, 17 decimals
+constexpr size_t kFixedMax = 1 + 309 + 1 + 17;
+
+}  // namespace
+
+void write_checkpoint(std::ostream& os, const Problem& P, const int32_t* basis, int64_t n,
+                      const std::vector<double>& E, const double* C, int ncol, int threads, bool binary) {
+    const int M = P.M;
+    if (binary) {
+        os.write(kCheckpointMagic, sizeof(kCheckpointMagic));
+        const int64_t head[4] = {n, M, P.neig, (int64_t)E.size()};
+        put_raw(os, head, 4);
+        put_raw(os, P.freq.data(), (size_t)M);
+        put_raw(os, E.data(), E.size());
+        const int64_t cols = ncol;
+        put_raw(os, &cols, 1);
+        std::vector<uint8_t> q((size_t)n * M);
+        for (size_t i = 0; i < q.size(); i++) q[i] = (uint8_t)basis[i];
+        put_raw(os, q.data(), q.size());
+        put_raw(os, C, (size_t)n * ncol);
+        os.flush();
+        return;
+    }
+    {
+        std::ostringstream h;
+        h << n << " " << M << " " << P.neig << '\n';
+        os << h.str();
+        std::string line;
+        char buf[kFixedMax + 1];
+        for (int j = 0; j < M; j++) {
+            line.append(buf, put_fixed17(buf, buf + kFixedMax, P.freq[j]));
+            line += ' ';
+        }
+        for (double e : E) {
+            line.append(buf, put_fixed17(buf, buf + kFixedMax, e));
+            line += ' ';
+        }
+        line += '\n';
+        os << line;
+    }
+    // rows: "q_0 .. q_(M-1) C_0 .. C_(ncol-1) \n", formatted in parallel ranges
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(threads, n / 2048 + 1));
+    std::vector<std::string> out(T);
+    parallel_ranges(n, T, [&](int t, int64_t lo, int64_t hi) {
+        std::string& s = out[t];
+        s.reserve((size_t)(hi - lo) * (M * 4 + ncol * 21 + 1));
+        char buf[kFixedMax + 1];
+        for (int64_t i = lo; i < hi; i++) {
+            for (int j = 0; j < M; j++) {
+                s.append(buf, std::to_chars(buf, buf + sizeof(buf), basis[i * M + j]).ptr);
+                s += ' ';
+            }
+            for (int j = 0; j < ncol; j++) {
+                s.append(buf, put_fixed17(buf, buf + kFixedMax, C[i * ncol + j]));
+                s += ' ';
+            }
+            s += '\n';
+        }
+    });
+    for (auto& s : out) os.write(s.data(), (std::streamsize)s.size());
     os.flush();
 }
 
+namespace {
+
+const char* kTruncated = "Error: checkpoint file is truncated.";
+
+// whitespace-separated number at p (the reference's operator>> reading)
+template <class T>
+bool next_num(const char*& p, const char* end, T& v) {
+    while (p < end && (*p == ' ' || *p == '\n' || *p == '\t' || *p == '\r')) p++;
+    if (p >= end) return false;
+    auto r = std::from_chars(p, end, v);
+    if (r.ec != std::errc()) return false;
+    p = r.ptr;
+    return true;
+}
+
+template <class T>
+bool get_raw(const char*& p, const char* end, T* v, size_t count) {
+    const size_t b = sizeof(T) * count;
+    if ((size_t)(end - p) < b) return false;
+    std::memcpy(v, p, b);
+    p += b;
+    return true;
+}
+
+std::string read_binary(const char* p, const char* end, const Problem& P, std::vector<int32_t>& basis,
+                        std::vector<double>& E, std::vector<double>& C) {
+    p += sizeof(kCheckpointMagic);
+    int64_t head[4];
+    if (!get_raw(p, end, head, 4)) return kTruncated;
+    const int64_t nb = head[0];
+    if (head[1] != P.M) return "Error: Checkpoint file contains wrong number of fundamental modes!";
+    if (head[2] != P.neig) return "Error: Checkpoint file contains wrong number of eigenstates!";
+    std::vector<double> f((size_t)P.M);
+    if (!get_raw(p, end, f.data(), f.size())) return kTruncated;
+    for (int i = 0; i < P.M; i++)
+        if (f[i] != P.freq[i]) return "Error: Checkpoint file contains wrong fundamental frequencies!";
+    if (head[3] < P.neig || nb < 1) return kTruncated;
+    std::vector<double> ew((size_t)head[3]);
+    if (!get_raw(p, end, ew.data(), ew.size())) return kTruncated;
+    E.assign(ew.begin(), ew.begin() + P.neig);
+    int64_t cols = 0;
+    if (!get_raw(p, end, &cols, 1) || cols < P.neig) return kTruncated;
+    std::vector<uint8_t> q((size_t)nb * P.M);
+    if (!get_raw(p, end, q.data(), q.size())) return kTruncated;
+    basis.assign(q.begin(), q.end());
+    std::vector<double> Cw((size_t)nb * cols);
+    if (!get_raw(p, end, Cw.data(), Cw.size())) return kTruncated;
+    C.assign((size_t)nb * P.neig, 0.0);
+    for (int64_t i = 0; i < nb; i++)
+        std::copy(Cw.begin() + i * cols, Cw.begin() + i * cols + P.neig, C.begin() + i * P.neig);
+    return "";
+}
+
+}  // namespace
+
 std::string read_checkpoint(std::istream& is, const Problem& P, std::vector<int32_t>& basis,
-                            std::vector<double>& E, std::vector<double>& C) {
+                            std::vector<double>& E, std::vector<double>& C, int threads) {
+    std::string text;
+    {
+        // whole file in one read where the stream can seek, else in blocks
+        const std::streampos at = is.tellg();
+        is.seekg(0, std::ios::end);
+        const std::streampos len = is.tellg();
+        if (at != std::streampos(-1) && len != std::streampos(-1) && len >= at) {
+            is.seekg(at);
+            text.resize((size_t)(len - at));
+            is.read(&text[0], (std::streamsize)text.size());
+            text.resize((size_t)is.gcount());
+        } else {
+            is.clear();
+            char buf[1 << 16];
+            while (is.read(buf, sizeof(buf)) || is.gcount() > 0) text.append(buf, (size_t)is.gcount());
+        }
+    }
+    const char* p = text.data();
+    const char* end = p + text.size();
+    if (text.size() >= sizeof(kCheckpointMagic) && std::memcmp(p, kCheckpointMagic, sizeof(kCheckpointMagic)) == 0)
+        return read_binary(p, end, P, basis, E, C);
     int64_t nb = 0;
     int par = 0;
-    is >> nb >> par;
+    if (!next_num(p, end, nb) || !next_num(p, end, par)) return kTruncated;
     if (par != P.M) return "Error: Checkpoint file contains wrong number of fundamental modes!";
-    is >> par;
+    if (!next_num(p, end, par)) return kTruncated;
     if (par != P.neig) return "Error: Checkpoint file contains wrong number of eigenstates!";
     for (int i = 0; i < P.M; i++) {
         double f = 0;
-        is >> f;
+        if (!next_num(p, end, f)) return kTruncated;
         if (f != P.freq[i]) return "Error: Checkpoint file contains wrong fundamental frequencies!";
     }
     E.assign(P.neig, 0.0);
-    for (int i = 0; i < P.neig; i++) is >> E[i];
-    if (nb < 1 || !is) return "Error: checkpoint file is truncated.";
+    for (int i = 0; i < P.neig; i++)
+        if (!next_num(p, end, E[i])) return kTruncated;
+    if (nb < 1) return kTruncated;
     basis.assign((size_t)nb * P.M, 0);
     C.assign((size_t)nb * P.neig, 0.0);
-    for (int64_t i = 0; i < nb; i++) {
-        for (int j = 0; j < P.M; j++) is >> basis[i * P.M + j];
-        for (int j = 0; j < P.neig; j++) is >> C[i * P.neig + j];
+    // Rows are read one line per basis state when the energy line ends after
+    // NEig values (the writer's output: Freqs.size() = NEig); otherwise the
+    // values are taken one after another, as the reference's stream reader
+    // does.
+    const char* eol = p;
+    while (eol < end && (*eol == ' ' || *eol == '\t' || *eol == '\r')) eol++;
+    if (eol >= end || *eol != '\n') {
+        for (int64_t i = 0; i < nb; i++) {
+            for (int j = 0; j < P.M; j++)
+                if (!next_num(p, end, basis[i * P.M + j])) return kTruncated;
+            for (int j = 0; j < P.neig; j++)
+                if (!next_num(p, end, C[i * P.neig + j])) return kTruncated;
+        }
+        return "";
     }
-    if (!is) return "Error: checkpoint file is truncated.";
+    p = eol;
+    // one line per basis state: find the line starts, then parse in ranges
+    std::vector<const char*> starts;
+    starts.reserve((size_t)nb + 1);
+    for (const char* s = p; s < end && (int64_t)starts.size() < nb;) {
+        const char* nl = (const char*)std::memchr(s, '\n', (size_t)(end - s));
+        if (!nl) break;
+        if (nl + 1 < end) starts.push_back(nl + 1);
+        s = nl + 1;
+    }
+    if ((int64_t)starts.size() < nb) return kTruncated;
+    starts.push_back(end);
+    const int T = (int)std::max<int64_t>(1, std::min<int64_t>(threads, nb / 2048 + 1));
+    std::vector<char> ok(T, 1);
+    parallel_ranges(nb, T, [&](int t, int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi && ok[t]; i++) {
+            const char* q = starts[i];
+            const char* qe = (i + 1 < nb) ? starts[i + 1] : end;
+            for (int j = 0; j < P.M && ok[t]; j++) ok[t] = next_num(q, qe, basis[i * P.M + j]);
+            for (int j = 0; j < P.neig && ok[t]; j++) ok[t] = next_num(q, qe, C[i * P.neig + j]);
+        }
+    });
+    for (char o : ok)
+        if (!o) return kTruncated;
     return "";
 }
 

@@ -118,11 +118,19 @@ constexpr int kLadderW = 13;
 // PrintFreqs (Ham.cpp:448-502): C row-major rows x ncol.
 void write_freqs(std::ostream& os, const Problem& P, const int32_t* basis, int64_t rows,
                  const std::vector<double>& E, const double* C, int ncol);
-// WriteCheckpoint (Core_functions.cpp:94-117)
+// WriteCheckpoint (Core_functions.cpp:94-117): the reference's text, byte for
+// byte (fixed, 17 decimals), with the rows formatted on `threads` host
+// threads.  binary = true writes the opt-in binary form instead (same
+// content: header, frequencies, energies, quanta as bytes, coefficients as
+// raw doubles).
 void write_checkpoint(std::ostream& os, const Problem& P, const int32_t* basis, int64_t n,
-                      const std::vector<double>& E, const double* C, int ncol);
-// ReadCheckpoint (Core_functions.cpp:119-167); returns "" or the error text
+                      const std::vector<double>& E, const double* C, int ncol, int threads = 1,
+                      bool binary = false);
+// ReadCheckpoint (Core_functions.cpp:119-167) of either form (the binary one
+// is recognised by its first bytes); returns "" or the error text
 std::string read_checkpoint(std::istream& is, const Problem& P, std::vector<int32_t>& basis,
-                            std::vector<double>& E, std::vector<double>& C);
+                            std::vector<double>& E, std::vector<double>& C, int threads = 1);
+// first bytes of a binary checkpoint
+constexpr char kCheckpointMagic[8] = {'V', 'H', 'C', 'I', 'C', 'K', 'B', '1'};
 
 }  // namespace vhci
