@@ -130,6 +130,14 @@ int vhci_dense_diag(vhci_ctx* ctx, double* E, double* C);
 int vhci_ritz_cols(const vhci_ctx* ctx);
 int vhci_get_vectors(vhci_ctx* ctx, double* C, int ncol);
 int vhci_set_vectors(vhci_ctx* ctx, const double* C, int ncol);
+/* The assignment scan of PrintFreqs (src/Ham.cpp:448-502) on the device,
+ * over the first ncol resident eigenvectors: for column c the basis rows m
+ * with |C_mc| >= max_m |C_mc| / 2, ascending.  counts[ncol] receives each
+ * column's row count and *total their sum; rows[] and absval[] receive the
+ * rows and |C_mc|, column after column, when cap >= *total (else
+ * VHCI_E_ARG, with counts and *total set). */
+int vhci_freq_assignments(vhci_ctx* ctx, int ncol, int64_t* counts, int64_t* rows, double* absval, int64_t cap,
+                          int64_t* total);
 
 /* The AddStatesHB loop of Perform_HCI (src/HB.cpp:204-223): Cmax over the
  * first nopt columns (or the given cmax[n]), then every basis state's

@@ -29,6 +29,8 @@ def fp64_run():
     ctx = vhci_amd.Context(p, vhci_amd.FP64)
     r = ctx.run()
     r["sizes"] = ctx.iteration_sizes()
+    r["C"] = ctx.vectors(p.neig)
+    r["assign"] = ctx.freq_assignments(p.neig)
     ctx.close()
     return r
 
@@ -49,3 +51,15 @@ def test_c1_mixed_matches_fp64(ref, fp64_run):
     assert ctx.iteration_sizes() == ref["sizes"]
     assert r["checksum"] == fp64_run["checksum"]
     assert np.max(np.abs(r["E_var"] - fp64_run["E_var"])) < 1e-2
+
+
+def test_c1_freq_assignments_match_full_scan(fp64_run):
+    """PrintFreqs' assignment scan on the device (rows with |C| >= max|C|/2,
+    ascending, Ham.cpp:448-502) equals the same scan over the downloaded
+    eigenvectors, row for row and bit for bit."""
+    C = fp64_run["C"]
+    for c, (rows, vals) in enumerate(fp64_run["assign"]):
+        col = np.abs(C[:, c])
+        want = np.nonzero(col >= col.max() / 2)[0]
+        assert np.array_equal(rows, want), c
+        assert np.array_equal(vals, col[want]), c

@@ -288,3 +288,43 @@ def test_solve_c0_vs_oracle(tmp_path):
         r = vhci_amd.solve(p, form)
         assert r["nvar"] == 1820
         assert np.max(np.abs(r["E"] - ref[:len(r["E"])])) < 1e-8
+
+
+def _capped(text, caps, ntot):
+    """The input with new per-mode caps and Max_quanta."""
+    lines = text.splitlines()
+    out, m = [], 0
+    for ln in lines:
+        f = ln.split()
+        if ln.startswith("Max_quanta"):
+            ln = "Max_quanta: %d" % ntot
+        elif len(f) == 3 and f[0].isdigit() and m < len(caps) and "." in f[1]:
+            ln = " %s %s %d" % (f[0], f[1], caps[m])
+            m += 1
+        out.append(ln)
+    return "\n".join(out) + "\n"
+
+
+def test_initial_basis_on_device(tmp_path):
+    """The initial product basis enumerated on the device (unranked in the
+    reference's order, Input_Reader.cpp:308-364) equals the host enumeration
+    row for row, including per-mode caps below Max_quanta."""
+    texts = {}
+    for f in ("clo2_vci.inp", "clo2_vhci.inp", "acn_vci.inp", "acn_vhci.inp"):
+        texts[f] = open(os.path.join(GOLDEN, f)).read()
+    for f in sorted(os.listdir(os.path.join(GOLDEN, "fields"))):
+        texts[f] = open(os.path.join(GOLDEN, "fields", f)).read()
+    for c in (0, 1, 2):
+        texts["c%d" % c] = synth.config_input(c)
+    for r in range(4):
+        texts["request%d" % r] = synth.request_input(r)
+    clo2 = texts["clo2_vci.inp"]
+    texts["clo2 caps 2,6,3 ntot 6"] = _capped(clo2, [2, 6, 3], 6)
+    texts["clo2 caps 0,9,1 ntot 4"] = _capped(clo2, [0, 9, 1], 4)
+    for name, text in texts.items():
+        p = vhci_amd.Problem(text=text)
+        ctx = vhci_amd.Context(p)
+        qd, qh = ctx.basis(), p.initial_basis()
+        _say("[initial basis]", name, qh.shape)
+        assert qd.shape == qh.shape and np.array_equal(qd, qh), name
+        ctx.close()

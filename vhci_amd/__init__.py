@@ -57,6 +57,7 @@ _SIGS = {
     "vhci_ritz_cols": (ctypes.c_int, [_vp]),
     "vhci_get_vectors": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
     "vhci_set_vectors": (ctypes.c_int, [_vp, _dp, ctypes.c_int]),
+    "vhci_freq_assignments": (ctypes.c_int, [_vp, ctypes.c_int, _i64p, _i64p, _dp, ctypes.c_int64, _i64p]),
     "vhci_hb_select": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _i64p]),
     "vhci_get_new_states": (ctypes.c_int, [_vp, _i32p]),
     "vhci_append_new_states": (ctypes.c_int, [_vp]),
@@ -283,6 +284,22 @@ class Context:
         C = np.zeros((self.n, ncol))
         self._check(lib().vhci_get_vectors(self.h, _ptr(C, _dp), ncol))
         return C
+
+    def freq_assignments(self, ncol):
+        """PrintFreqs' assignment scan on the device: per eigenvector, the
+        basis rows with |C| >= max |C| / 2 (ascending) and their |C|."""
+        L = lib()
+        counts = np.zeros(ncol, dtype=np.int64)
+        total = np.zeros(1, dtype=np.int64)
+        rc = L.vhci_freq_assignments(self.h, ncol, _ptr(counts, _i64p), None, None, 0, _ptr(total, _i64p))
+        if rc not in (0, -1):
+            self._check(rc)
+        rows = np.zeros(max(1, int(total[0])), dtype=np.int64)
+        vals = np.zeros(max(1, int(total[0])))
+        self._check(L.vhci_freq_assignments(self.h, ncol, _ptr(counts, _i64p), _ptr(rows, _i64p), _ptr(vals, _dp),
+                                            rows.size, _ptr(total, _i64p)))
+        off = np.concatenate([[0], np.cumsum(counts)])
+        return [(rows[off[c]:off[c + 1]].copy(), vals[off[c]:off[c + 1]].copy()) for c in range(ncol)]
 
     def set_vectors(self, C):
         C = np.ascontiguousarray(C, dtype=np.float64)

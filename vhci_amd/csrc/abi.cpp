@@ -309,7 +309,7 @@ int vhci_ctx_create(const vhci_problem* p, int form, int device, vhci_ctx** out)
     auto c = std::make_unique<vhci_ctx>();
     try {
         c->eng = std::make_unique<Engine>(p->P, form, device);
-        c->eng->set_basis(p->P.basis0.data(), p->P.n0());
+        c->eng->set_initial_basis();
     } catch (const std::exception& e) {
         g_problem_err = e.what();
         return classify(g_problem_err);
@@ -393,6 +393,33 @@ int vhci_get_vectors(vhci_ctx* ctx, double* C, int ncol) {
 
 int vhci_set_vectors(vhci_ctx* ctx, const double* C, int ncol) {
     return guard(ctx, [&] { ctx->eng->set_vectors(C, ncol); });
+}
+
+int vhci_freq_assignments(vhci_ctx* ctx, int ncol, int64_t* counts, int64_t* rows, double* absval, int64_t cap,
+                          int64_t* total) {
+    if (!counts || !total) return VHCI_E_ARG;
+    bool small = false;
+    int rc = guard(ctx, [&] {
+        std::vector<vhci::FreqColumn> cols;
+        ctx->eng->freq_assignments(ncol, cols);
+        int64_t t = 0;
+        for (int c = 0; c < ncol; c++) {
+            counts[c] = (int64_t)cols[c].row.size();
+            t += counts[c];
+        }
+        *total = t;
+        if (!rows || !absval || cap < t) {
+            small = true;
+            return;
+        }
+        int64_t w = 0;
+        for (int c = 0; c < ncol; c++)
+            for (size_t k = 0; k < cols[c].row.size(); k++, w++) {
+                rows[w] = cols[c].row[k];
+                absval[w] = cols[c].absval[k];
+            }
+    });
+    return rc != VHCI_OK ? rc : (small ? VHCI_E_ARG : VHCI_OK);
 }
 
 int vhci_hb_select(vhci_ctx* ctx, const double* cmax, int nopt, double eps, int64_t* nnew) {
@@ -504,7 +531,7 @@ int vhci_solve(const vhci_problem* p, int form, int device, double tol, double* 
     auto t0 = std::chrono::steady_clock::now();
     try {
         Engine eng(p->P, form, device);
-        eng.set_basis(p->P.basis0.data(), p->P.n0());
+        eng.set_initial_basis();
         RunOpts o;
         if (tol > 0) o.tol = tol;
         RunResult R;
@@ -731,7 +758,7 @@ int vhci_main(int argc, char** argv) {
             eng.set_vectors(C.data(), P.neig);
             std::cout << "done.\n  Basis functions: " << nb << "\n" << std::endl;
         } else {
-            eng.set_basis(P.basis0.data(), P.n0());
+            eng.set_initial_basis();
         }
         const bool pt = P.perturb;
         if (pt) {
@@ -773,13 +800,10 @@ int vhci_main(int argc, char** argv) {
             run(eng, P, o, R, log);
         }
         // PrintFreqs (VHCI.cpp:116-119)
-        const int64_t rows = eng.n();
         const int ncol = std::min<int>(eng.ritz_cols(), std::max(1, std::min<int>(P.neig, (int)R.E.size())));
-        std::vector<int32_t> q((size_t)rows * P.M);
-        std::vector<double> C((size_t)rows * ncol);
-        eng.get_basis(q.data());
-        eng.get_vectors(C.data(), ncol);
-        vhci::write_freqs(freqfile, P, q.data(), rows, R.E, C.data(), ncol);
+        std::vector<vhci::FreqColumn> cols;
+        eng.freq_assignments(ncol, cols);
+        vhci::write_freqs(freqfile, P, R.E, cols);
     } catch (const std::exception& e) {
         std::cout << "\n" << e.what() << "\n";
         return 1;

@@ -189,6 +189,32 @@ __global__ void k_fill(T* p, int64_t n, T v) {
         p[i] = v;
 }
 
+// Initial product basis (Input_Reader.cpp:308-364): every q with
+// q_m <= lim_m and sum q <= ntot, ordered with the last mode most
+// significant and mode 0 fastest.  Thread r unranks state r: cnt[m][s] =
+// number of choices for modes 0..m-1 with sum <= s, so taking the modes from
+// the last down, q_m is the first value whose block of completions holds r.
+__global__ void k_initial_basis(int64_t n0, int M, int ntot, int K, const int64_t* __restrict__ cnt,
+                                const int32_t* __restrict__ lim, uint8_t* __restrict__ keys) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r >= n0) return;
+    uint4* k4 = reinterpret_cast<uint4*>(keys + r * K);
+    for (int t = 0; t < K / 16; t++) k4[t] = make_uint4(0u, 0u, 0u, 0u);
+    int64_t rr = r;
+    int b = ntot;
+    for (int m = M - 1; m >= 0; m--) {
+        const int top = min(lim[m], b);
+        int q = 0;
+        for (; q < top; q++) {
+            const int64_t c = cnt[(int64_t)m * (ntot + 1) + (b - q)];
+            if (rr < c) break;
+            rr -= c;
+        }
+        keys[r * K + m] = (uint8_t)q;
+        b -= q;
+    }
+}
+
 // copy an r x c block between row-major arrays with different leading dims
 __global__ void k_copy2d(const double* __restrict__ src, int lds, double* __restrict__ dst, int ldd, int64_t rows,
                          int cols) {
@@ -1425,6 +1451,49 @@ __global__ void k_pt2_reduce(const double* __restrict__ part, int nw, int nopt, 
 }
 
 // ------------------------------------------------------------------------
+// PrintFreqs assignment scan (Ham.cpp:448-502) on the resident Ritz block X
+// (row-major, ld columns): per column c, half = max_m |X_mc| / 2 and the rows
+// with |X_mc| >= half.  Blocks take chunks of rows, threads take columns.
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_col_absmax(int64_t n, const double* __restrict__ X, int ld, int ncol,
+                                                    int64_t rpc, double* __restrict__ part) {
+    const int64_t r0 = (int64_t)blockIdx.x * rpc, r1 = min(n, r0 + rpc);
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+        double m = 0.0;
+        for (int64_t r = r0; r < r1; r++) m = fmax(m, fabs(X[r * ld + c]));
+        part[(int64_t)blockIdx.x * ncol + c] = m;
+    }
+}
+
+__global__ void k_col_half(const double* __restrict__ part, int nchunk, int ncol, double* __restrict__ half) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    double m = 0.0;
+    for (int k = 0; k < nchunk; k++) m = fmax(m, part[(int64_t)k * ncol + c]);
+    half[c] = m / 2;
+}
+
+__global__ __launch_bounds__(256) void k_col_select(int64_t n, const double* __restrict__ X, int ld, int ncol,
+                                                    int64_t rpc, const double* __restrict__ half,
+                                                    uint32_t* __restrict__ cnt, uint32_t cap,
+                                                    int64_t* __restrict__ rows, double* __restrict__ vals) {
+    const int64_t r0 = (int64_t)blockIdx.x * rpc, r1 = min(n, r0 + rpc);
+    for (int c = threadIdx.x; c < ncol; c += blockDim.x) {
+        const double h = half[c];
+        for (int64_t r = r0; r < r1; r++) {
+            const double v = fabs(X[r * ld + c]);
+            if (v >= h) {
+                const uint32_t slot = atomicAdd(&cnt[c], 1u);
+                if (slot < cap) {
+                    rows[(int64_t)c * cap + slot] = r;
+                    vals[(int64_t)c * cap + slot] = v;
+                }
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------------
 // PT2, push form: every variational state i probes its connection patterns
 // in the external-space hash; each hit emits (a, i, pattern) at offsets fixed
 // by a prefix sum over i and the lane order, so after a stable sort by a the
@@ -1757,6 +1826,51 @@ void Engine::set_basis(const int32_t* q, int64_t n) {
     xrows_ = 0;
     xcols_ = 0;
     rebuild_hash(n);
+    HIPCHK(hipStreamSynchronize(stream_));
+}
+
+void Engine::set_initial_basis() {
+    const int M = P_.M;
+    const int K = KW_ * 16;
+    if (M < 1) throw std::runtime_error("empty basis");
+    if (P_.restart) {
+        // the restart placeholder: one state of zero quanta (its basis comes from the checkpoint)
+        std::vector<int32_t> z(M, 0);
+        set_basis(z.data(), 1);
+        return;
+    }
+    const int nt = P_.ntot;
+    if (nt < 0) throw std::runtime_error("negative total quanta");
+    std::vector<int32_t> lim(M);
+    for (int m = 0; m < M; m++) lim[m] = std::min(P_.cap[m], nt);  // a negative cap leaves no state
+    // cnt[m][s]: choices for modes 0..m-1 with sum <= s
+    std::vector<int64_t> cnt((size_t)(M + 1) * (nt + 1), 0);
+    for (int s = 0; s <= nt; s++) cnt[s] = 1;
+    for (int m = 0; m < M; m++)
+        for (int s = 0; s <= nt; s++) {
+            int64_t c = 0;
+            for (int q = 0; q <= std::min(lim[m], s); q++) c += cnt[(size_t)m * (nt + 1) + (s - q)];
+            if (c > (int64_t)INT32_MAX) throw std::runtime_error("basis larger than 2^31-1 states");
+            cnt[(size_t)(m + 1) * (nt + 1) + s] = c;
+        }
+    const int64_t n0 = cnt[(size_t)M * (nt + 1) + nt];
+    if (n0 < 1) throw std::runtime_error("empty basis");
+    for (int m = 0; m < M; m++)
+        if (lim[m] > 255) throw std::runtime_error("quanta must lie in 0..255");
+    DevBuf dcnt, dlim;
+    dcnt.ensure(cnt.size() * sizeof(int64_t));
+    dlim.ensure(lim.size() * sizeof(int32_t));
+    HIPCHK(hipMemcpyAsync(dcnt.p, cnt.data(), cnt.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream_));
+    HIPCHK(hipMemcpyAsync(dlim.p, lim.data(), lim.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+    keys_.ensure((size_t)n0 * K);
+    k_initial_basis<<<grid_for(n0, 256), 256, 0, stream_>>>(n0, M, nt, K, dcnt.as<int64_t>(), dlim.as<int32_t>(),
+                                                           keys_.as<uint8_t>());
+    HIPCHK(hipGetLastError());
+    n_ = n0;
+    h_valid_ = false;
+    xrows_ = 0;
+    xcols_ = 0;
+    rebuild_hash(n0);
     HIPCHK(hipStreamSynchronize(stream_));
 }
 
@@ -2951,6 +3065,75 @@ void Engine::get_vectors(double* C, int ncol) const {
     HIPCHK(hipStreamSynchronize(stream_));
     for (int64_t i = 0; i < xrows_; i++)
         for (int c = 0; c < ncol; c++) C[i * ncol + c] = xo[i * xcols_ + c];
+}
+
+void Engine::freq_assignments(int ncol, std::vector<FreqColumn>& out) const {
+    HIPCHK(hipSetDevice(device_));
+    if (ncol < 1 || ncol > xcols_ || xrows_ < 1) throw std::runtime_error("no resident eigenvectors to assign");
+    const int64_t n = xrows_;
+    const int64_t rpc = std::max<int64_t>(64, (n + 1023) / 1024);
+    const int nchunk = (int)((n + rpc - 1) / rpc);
+    DevBuf part, half, cnt, rows, vals;
+    part.ensure((size_t)nchunk * ncol * sizeof(double));
+    half.ensure((size_t)ncol * sizeof(double));
+    cnt.ensure((size_t)ncol * sizeof(uint32_t));
+    const double* X = X_.as<double>();
+    k_col_absmax<<<nchunk, 256, 0, stream_>>>(n, X, xcols_, ncol, rpc, part.as<double>());
+    k_col_half<<<grid_for(ncol, 64), 64, 0, stream_>>>(part.as<double>(), nchunk, ncol, half.as<double>());
+    HIPCHK(hipGetLastError());
+    std::vector<uint32_t> c(ncol);
+    uint32_t cap = 64;
+    for (int pass = 0; pass < 2; pass++) {
+        rows.ensure((size_t)ncol * cap * sizeof(int64_t));
+        vals.ensure((size_t)ncol * cap * sizeof(double));
+        k_fill<uint32_t><<<grid_for(ncol, 256), 256, 0, stream_>>>(cnt.as<uint32_t>(), ncol, 0u);
+        k_col_select<<<nchunk, 256, 0, stream_>>>(n, X, xcols_, ncol, rpc, half.as<double>(), cnt.as<uint32_t>(), cap,
+                                                  rows.as<int64_t>(), vals.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c.data(), cnt.p, ncol * sizeof(uint32_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        const uint32_t most = *std::max_element(c.begin(), c.end());
+        if (most <= cap) break;
+        cap = most;  // a wider second pass holds every selected row
+    }
+    std::vector<int64_t> hr((size_t)ncol * cap);
+    std::vector<double> hv((size_t)ncol * cap);
+    HIPCHK(hipMemcpyAsync(hr.data(), rows.p, hr.size() * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(hv.data(), vals.p, hv.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    out.assign(ncol, FreqColumn{});
+    std::vector<uint32_t> want;
+    for (int k = 0; k < ncol; k++) {
+        std::vector<std::pair<int64_t, double>> sel;
+        for (uint32_t t = 0; t < c[k]; t++) sel.emplace_back(hr[(size_t)k * cap + t], hv[(size_t)k * cap + t]);
+        std::sort(sel.begin(), sel.end());  // ascending basis row, as the reference's scan
+        for (auto& e : sel) {
+            out[k].row.push_back(e.first);
+            out[k].absval.push_back(e.second);
+            want.push_back((uint32_t)e.first);
+        }
+    }
+    // quanta of the selected rows
+    std::vector<uint8_t> keys((size_t)std::max<size_t>(want.size(), 1) * KW_ * 16);
+    if (!want.empty()) {
+        DevBuf idx, kout;
+        idx.ensure(want.size() * sizeof(uint32_t));
+        kout.ensure(keys.size());
+        HIPCHK(hipMemcpyAsync(idx.p, want.data(), want.size() * sizeof(uint32_t), hipMemcpyHostToDevice, stream_));
+        k_gather_keys<<<grid_for((int64_t)want.size() * KW_, 256), 256, 0, stream_>>>(
+            keys_.as<uint4>(), idx.as<uint32_t>(), (int64_t)want.size(), KW_, kout.as<uint4>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(keys.data(), kout.p, keys.size(), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+    }
+    size_t w = 0;
+    const int M = P_.M, K = KW_ * 16;
+    for (int k = 0; k < ncol; k++)
+        for (size_t t = 0; t < out[k].row.size(); t++, w++) {
+            std::vector<int32_t> q(M);
+            for (int m = 0; m < M; m++) q[m] = keys[w * K + m];
+            out[k].quanta.push_back(std::move(q));
+        }
 }
 
 void Engine::set_vectors(const double* C, int ncol) {

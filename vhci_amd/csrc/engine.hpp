@@ -79,6 +79,9 @@ class Engine {
 
     // BasisSet (VCI_headers.h:149): replace the basis; quanta rows n x M.
     void set_basis(const int32_t* q, int64_t n);
+    // the initial product basis of the problem, enumerated on the device in
+    // the reference's order (equal to Problem::basis0)
+    void set_initial_basis();
     void get_basis(int32_t* q) const;
 
     // MakeHamSparse (Ham.cpp:328-344) on the current basis.
@@ -98,6 +101,10 @@ class Engine {
     // resident eigenvectors, row-major n x ncol
     int ritz_cols() const { return xcols_; }
     void get_vectors(double* C, int ncol) const;
+    // PrintFreqs assignment scan of the first ncol resident eigenvectors on
+    // the device: per column, the basis rows with |C| >= max |C| / 2
+    // (ascending), their |C| and quanta
+    void freq_assignments(int ncol, std::vector<FreqColumn>& out) const;
     void set_vectors(const double* C, int ncol);  // upload (restart)
 
     // AddStatesHB over every basis state (HB.cpp:221-223).  cmax may be null

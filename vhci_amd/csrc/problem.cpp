@@ -397,26 +397,18 @@ void build_tables(const Problem& P, Tables& T) {
     }
 }
 
-void write_freqs(std::ostream& os, const Problem& P, const int32_t* basis, int64_t rows,
-                 const std::vector<double>& E, const double* C, int ncol) {
+void write_freqs(std::ostream& os, const Problem& P, const std::vector<double>& E,
+                 const std::vector<FreqColumn>& cols) {
     const int M = P.M;
-    int nf = std::min<int>(P.neig, (int)E.size());
+    const int nf = std::min<int>(std::min<int>(P.neig, (int)E.size()), (int)cols.size());
     auto labels = [&](int col, const char* sep) {
-        double mx = -std::numeric_limits<double>::infinity(), mn = std::numeric_limits<double>::infinity();
-        for (int64_t m = 0; m < rows; m++) {
-            mx = std::max(mx, C[m * ncol + col]);
-            mn = std::min(mn, C[m * ncol + col]);
-        }
-        const double half = std::max(std::abs(mx), std::abs(mn)) / 2;
-        int written = 0;
-        for (int64_t m = 0; m < rows; m++) {
-            double c = std::abs(C[m * ncol + col]);
-            if (!(c >= half)) continue;
-            if (written++ > 0) os << sep;
-            os << std::fixed << std::setprecision(2) << c << " (";
+        const FreqColumn& fc = cols[col];
+        for (size_t t = 0; t < fc.row.size(); t++) {
+            if (t > 0) os << sep;
+            os << std::fixed << std::setprecision(2) << fc.absval[t] << " (";
             int nm = 0;
             for (int l = 0; l < M; l++) {
-                int q = basis[m * M + l];
+                const int q = fc.quanta[t][l];
                 if (q > 0) os << (nm++ ? "+" : "") << q << "w" << l;
             }
             os << ")";

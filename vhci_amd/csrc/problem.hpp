@@ -115,9 +115,16 @@ void build_ladder(int qmax, std::vector<double>& L);
 constexpr int kLadderW = 13;
 
 // Output files -----------------------------------------------------------
-// PrintFreqs (Ham.cpp:448-502): C row-major rows x ncol.
-void write_freqs(std::ostream& os, const Problem& P, const int32_t* basis, int64_t rows,
-                 const std::vector<double>& E, const double* C, int ncol);
+// One eigenvector's assignment for PrintFreqs: the basis rows with
+// |C| >= max |C| / 2 in ascending order, their |C| and their quanta.
+struct FreqColumn {
+    std::vector<int64_t> row;
+    std::vector<double> absval;
+    std::vector<std::vector<int32_t>> quanta;
+};
+// PrintFreqs (Ham.cpp:448-502) from the per-eigenvector assignments
+void write_freqs(std::ostream& os, const Problem& P, const std::vector<double>& E,
+                 const std::vector<FreqColumn>& cols);
 // WriteCheckpoint (Core_functions.cpp:94-117): the reference's text, byte for
 // byte (fixed, 17 decimals), with the rows formatted on `threads` host
 // threads.  binary = true writes the opt-in binary form instead (same

@@ -137,7 +137,7 @@ void Server::worker(int slot) {
             Tables T;
             build_tables(P, T);
             Engine eng(P, form, device_);
-            eng.set_basis(P.basis0.data(), P.n0());
+            eng.set_initial_basis();
             drv::RunOpts o;
             o.tol = tol;
             int64_t nprev = 0;
