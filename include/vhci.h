@@ -105,6 +105,12 @@ int vhci_get_basis(vhci_ctx* ctx, int32_t* q);
  * nonzeros, structural nonzeros of the reference's triplet list (-1 if not
  * computed), build time in microseconds. */
 int vhci_build_h(vhci_ctx* ctx, int64_t* stats);
+/* 1 when the built H keeps no stored entries and every product regenerates
+ * them from the basis hash (chosen when the stored matrix would not fit
+ * beside the eigensolver's blocks, or forced by VHCI_MATRIX_FREE=1), 0 for
+ * the stored CSR, negative on error.  No counterpart in the reference, which
+ * always stores H (src/Ham.cpp:328-344). */
+int vhci_matrix_free(const vhci_ctx* ctx);
 /* Structural nonzeros of the reference's triplet list for the current
  * basis: n plus every ordered pair whose quanta differ by an even L1
  * distance <= the highest FC order or an odd one <= that order - 1

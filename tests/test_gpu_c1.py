@@ -63,3 +63,21 @@ def test_c1_freq_assignments_match_full_scan(fp64_run):
         want = np.nonzero(col >= col.max() / 2)[0]
         assert np.array_equal(rows, want), c
         assert np.array_equal(vals, col[want]), c
+
+
+def test_c1_matrix_free_matches_oracle(ref, fp64_run, monkeypatch):
+    """Products that regenerate H from the basis hash instead of reading a
+    stored matrix (the form for bases whose CSR does not fit): same
+    per-iteration sizes and basis as the oracle, energies within 1e-6."""
+    monkeypatch.setenv("VHCI_MATRIX_FREE", "1")
+    p = vhci_amd.Problem(text=synth.config_input(1, pt2=False))
+    ctx = vhci_amd.Context(p, vhci_amd.FP64)
+    r = ctx.run()
+    assert ctx.matrix_free
+    sizes = ctx.iteration_sizes()
+    ctx.close()
+    print("c1 matrix-free sizes", sizes, r["timings"], flush=True)
+    assert sizes == ref["sizes"]
+    assert r["checksum"] == int(ref["checksum"])
+    assert np.max(np.abs(r["E_var"] - np.array(ref["E_var"]))) < 1e-6
+    assert np.max(np.abs(r["E_var"] - fp64_run["E_var"])) < 1e-6

@@ -328,3 +328,22 @@ def test_initial_basis_on_device(tmp_path):
         _say("[initial basis]", name, qh.shape)
         assert qd.shape == qh.shape and np.array_equal(qd, qh), name
         ctx.close()
+
+
+@pytest.mark.parametrize("name", ["clo2_pt2.inp", "clo2_vhci.inp"])
+def test_matrix_free_matches_stored(name, monkeypatch):
+    """ClO2 through matrix-free products (forced) against the stored CSR:
+    the dense first diagonalization sees the same matrix (H times identity
+    columns), so basis and energies agree; the stored run is the default."""
+    p = vhci_amd.Problem(os.path.join(GOLDEN, name))
+    ctx = vhci_amd.Context(p)
+    ref = ctx.run()
+    assert not ctx.matrix_free
+    ctx.close()
+    monkeypatch.setenv("VHCI_MATRIX_FREE", "1")
+    ctx = vhci_amd.Context(p)
+    r = ctx.run()
+    assert ctx.matrix_free
+    ctx.close()
+    assert r["nvar"] == ref["nvar"] and r["ns2"] == ref["ns2"] and r["checksum"] == ref["checksum"]
+    assert np.max(np.abs(r["E"] - ref["E"])) < 1e-8

@@ -50,6 +50,7 @@ _SIGS = {
     "vhci_basis_size": (ctypes.c_int64, [_vp]),
     "vhci_get_basis": (ctypes.c_int, [_vp, _i32p]),
     "vhci_build_h": (ctypes.c_int, [_vp, _i64p]),
+    "vhci_matrix_free": (ctypes.c_int, [_vp]),
     "vhci_structural_nnz": (ctypes.c_int64, [_vp]),
     "vhci_get_h": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _dp, _dp]),
     "vhci_eigsolve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]),
@@ -244,6 +245,14 @@ class Context:
         st = np.zeros(4, dtype=np.int64)
         self._check(lib().vhci_build_h(self.h, _ptr(st, _i64p)))
         return {"n": int(st[0]), "nnz_offdiag": int(st[1]), "nnz_struct": int(st[2]), "us": int(st[3])}
+
+    @property
+    def matrix_free(self):
+        """True when the built H keeps no stored entries (products regenerate them)."""
+        r = int(lib().vhci_matrix_free(self.h))
+        if r < 0:
+            self._check(r)
+        return bool(r)
 
     def structural_nnz(self):
         r = int(lib().vhci_structural_nnz(self.h))

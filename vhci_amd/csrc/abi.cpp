@@ -344,6 +344,11 @@ int vhci_build_h(vhci_ctx* ctx, int64_t* stats) {
     });
 }
 
+int vhci_matrix_free(const vhci_ctx* ctx) {
+    if (!ctx || !ctx->eng) return VHCI_E_ARG;
+    return ctx->eng->matrix_free() ? 1 : 0;
+}
+
 int64_t vhci_structural_nnz(vhci_ctx* ctx) {
     if (!ctx || !ctx->eng) return VHCI_E_ARG;
     int64_t r = -1;

@@ -215,6 +215,15 @@ __global__ void k_initial_basis(int64_t n0, int M, int ntot, int K, const int64_
     }
 }
 
+// row-major n x ld block of identity columns c0 .. c0+bc-1 (zero beyond bc)
+__global__ void k_identity_block(double* __restrict__ X, int64_t n, int ld, int64_t c0, int bc) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n * ld; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / ld;
+        const int c = (int)(t - r * ld);
+        X[t] = (c < bc && r == c0 + c) ? 1.0 : 0.0;
+    }
+}
+
 // copy an r x c block between row-major arrays with different leading dims
 __global__ void k_copy2d(const double* __restrict__ src, int lds, double* __restrict__ dst, int ldd, int64_t rows,
                          int cols) {
@@ -251,6 +260,30 @@ __global__ void k_hash_insert(const uint4* __restrict__ keys, int64_t i0, int64_
         if (prev == kEmpty) return;
         pos = (pos + 1) & mask;
     }
+}
+
+// Sum over the 64 lanes of a wave of each lane's acc[0..B): a transposing
+// butterfly over lane bits 5, 4, ... (each step keeps half of the columns and
+// trades the other half with the partner lane), then a sum over the R = 64/B
+// lanes that share a column.  Lane l (l % R == 0) returns column l / R.
+// Fixed order, so the result is reproducible bit for bit.
+template <int B, class AT>
+__device__ __forceinline__ AT lane_colsum(AT (&acc)[B], int lane) {
+#pragma unroll
+    for (int h = B / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+        const bool up = (lane & bit) != 0;
+#pragma unroll
+        for (int i = 0; i < h; i++) {
+            const AT send = up ? acc[i] : acc[i + h];
+            const AT keep = up ? acc[i + h] : acc[i];
+            acc[i] = keep + __shfl_xor(send, bit);
+        }
+    }
+    constexpr int R = 64 / B;  // lanes that share one column after the butterfly
+    AT s = acc[0];
+#pragma unroll
+    for (int off = R / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off);
+    return s;
 }
 
 // ------------------------------------------------------------------------
@@ -424,6 +457,94 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     }
     if (FILL && nq > 0) drain(nq);
     if (!FILL && lane == 0) a.rowcnt[row] = cnt;
+}
+
+// Matrix-free product Y = (D + H_off) X for B-column blocks (bcols <= B
+// used): each row regenerates its connections as the H build does (every
+// pattern probes the basis hash; hits queue in LDS and are evaluated 64 at
+// a time, one per lane) and multiplies them into the gathered rows of X
+// instead of storing them.  X and Y are row-major in stored-row order.
+struct HFreeArgs {
+    HBuildArgs h;
+    const double* D;
+    const double* X;
+    int ldx;
+    double* Y;
+    int ldy;
+    int bcols;
+};
+
+template <int NW, int B>
+__global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
+    const HBuildArgs& a = f.h;
+    __shared__ uint8_t sq[64];
+    __shared__ int32_t qj[2 * kWave];
+    __shared__ int32_t qp[2 * kWave];
+    const int lane = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    const int64_t src = a.perm ? a.perm[row] : row;
+    uint32_t w[NW];
+    key_load<NW>(a.keys, src, w);
+    if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+    __syncthreads();
+    double acc[B];
+#pragma unroll
+    for (int c = 0; c < B; c++) acc[c] = 0.0;
+    int nq = 0;
+    auto drain = [&](int count) {
+        if (lane < count) {
+            const int p = qp[lane];
+            const int64_t j = qj[lane];
+            const uint4 e = rec_load(a.deltas, p);
+            const GroupEnt ge = a.groups[delta_group(e)];
+            const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, delta_nz(e), src < j);
+            const int64_t jr = a.inv ? (int64_t)a.inv[j] : j;
+            const double* xr = f.X + jr * f.ldx;
+#pragma unroll
+            for (int c = 0; c < B; c++)
+                if (c < f.bcols) acc[c] = acc[c] + v * xr[c];
+        }
+    };
+    for (int p0 = 0; p0 < a.P; p0 += kWave) {
+        const int p = p0 + lane;
+        bool hit = false;
+        int64_t j = -1;
+        if (p < a.P) {
+            const uint4 e = rec_load(a.deltas, p);
+            uint32_t t[NW];
+#pragma unroll
+            for (int i = 0; i < NW; i++) t[i] = w[i];
+            if (apply_pattern<NW>(e, delta_nz(e), sq, t)) {
+                j = hash_find<NW>(a.slots, a.mask, a.keys, t);
+                hit = j >= 0;
+            }
+        }
+        const uint64_t bal = __ballot(hit);
+        if (hit) {
+            const int slot = nq + __popcll(bal & ((1ull << lane) - 1ull));
+            qj[slot] = (int32_t)j;
+            qp[slot] = p;
+        }
+        nq += __popcll(bal);
+        __syncthreads();
+        if (nq >= kWave) {
+            drain(kWave);
+            __syncthreads();
+            if (lane < nq - kWave) {
+                qj[lane] = qj[kWave + lane];
+                qp[lane] = qp[kWave + lane];
+            }
+            nq -= kWave;
+            __syncthreads();
+        }
+    }
+    if (nq > 0) drain(nq);
+    const double s = lane_colsum<B>(acc, lane);
+    constexpr int R = 64 / B;
+    if ((lane & (R - 1)) == 0) {
+        const int c = lane / R;
+        if (c < f.bcols) f.Y[row * f.ldy + c] = f.D[row] * f.X[row * f.ldx + c] + s;
+    }
 }
 
 // H_ii = E_i + V_ii (Ham.cpp:203-226 plus the zero-pattern FCs), in the
@@ -616,22 +737,7 @@ __device__ __forceinline__ typename SpAcc<MODE>::T sp_lane_dot(int64_t k0, int64
 #pragma unroll
         for (int c = 0; c < B; c++) acc[c] = sp_fma(v0, x0[c], acc[c]);
     }
-    // transposing butterfly over lane bits 5, 4, ...: keep half, trade half
-#pragma unroll
-    for (int h = B / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
-        const bool up = (lane & bit) != 0;
-#pragma unroll
-        for (int i = 0; i < h; i++) {
-            const AT send = up ? acc[i] : acc[i + h];
-            const AT keep = up ? acc[i + h] : acc[i];
-            acc[i] = keep + __shfl_xor(send, bit);
-        }
-    }
-    constexpr int R = 64 / B;  // lanes that share one column after the butterfly
-    AT s = acc[0];
-#pragma unroll
-    for (int off = R / 2; off >= 1; off >>= 1) s += __shfl_xor(s, off);
-    return s;
+    return lane_colsum<B>(acc, lane);
 }
 
 template <int B, int MODE>
@@ -1972,6 +2078,23 @@ void Engine::build_h_t(HStats& st) {
     HIPCHK(hipMemcpyAsync(&nnz, rowptr_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
     nnz_ = nnz;
+    matfree_ = use_matrix_free(nnz);
+    if (matfree_) {
+        // no stored entries: products regenerate them (k_hfree)
+        col_.release();
+        val_.release();
+        hi_.release();
+        lo_.release();
+        D_.ensure(n * sizeof(double));
+        k_diag<NW><<<grid_for(n, 128), 128, 0, stream_>>>(n, P_.M, a.keys, d_freq_.as<double>(),
+                                                          d_diag_.as<uint32_t>(), (int)T_.diag_fcs.size(), a.elem,
+                                                          a.ladder, 256, a.perm, D_.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream_));
+        st.n = n;
+        st.nnz_offdiag = nnz;
+        return;
+    }
     col_.ensure(std::max<int64_t>(1, nnz) * sizeof(int32_t));
     if (form_ == kFP64) {
         val_.ensure(std::max<int64_t>(1, nnz) * sizeof(double));
@@ -2079,6 +2202,7 @@ HStats Engine::build_h() {
 void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std::vector<double>& val,
                    std::vector<double>& diag) {
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    if (matfree_) throw std::runtime_error("the Hamiltonian is matrix-free: no stored entries to return");
     rowptr.resize(n_ + 1);
     col.resize(nnz_);
     val.resize(nnz_);
@@ -2164,8 +2288,69 @@ int64_t Engine::structural_nnz() {
 // ---------------------------------------------------------------- SpMM
 int Engine::spmm_launches(int) const { return 1; }
 
+// Stored CSR when it fits beside the eigensolver's blocks, else matrix-free
+// products.  VHCI_MATRIX_FREE=1 / 0 forces either.
+bool Engine::use_matrix_free(int64_t nnz) const {
+    if (const char* e = std::getenv("VHCI_MATRIX_FREE")) return std::string(e) == "1";
+    size_t free_b = 0, total_b = 0;
+    HIPCHK(hipMemGetInfo(&free_b, &total_b));
+    const double sv = form_ == kFP64 ? 8.0 : 8.0;  // FP64 value, or FP32 hi + lo
+    const double csr = (double)nnz * (sv + 4.0) * 2.0;  // entries, plus the sort's copy of a slice
+    // Krylov-Schur blocks (V, W: 3.5 nev + bs columns each; X, HX: 1.6 nev each)
+    const double nev = std::max(1, P_.neig);
+    const double eig = (double)n_ * 8.0 * (2.0 * (3.5 * nev + 8.0) + 2.0 * 1.6 * nev);
+    const double held = (double)(col_.cap + val_.cap + hi_.cap + lo_.cap);  // released first
+    return csr + eig > 0.9 * ((double)free_b + held);
+}
+
+template <int NW>
+void Engine::spmm_free_t(const double* X, int ldx, double* Y, int ldy, int b) {
+    HFreeArgs f{};
+    HBuildArgs& a = f.h;
+    a.n = n_;
+    a.M = P_.M;
+    a.P = P_deltas_;
+    a.nq = 256;
+    a.keys = keys_.as<uint4>();
+    a.slots = slots_.as<uint64_t>();
+    a.mask = hmask_;
+    a.deltas = d_deltas_.as<DeltaEnt>();
+    a.groups = d_groups_.as<GroupEnt>();
+    a.fclist = d_fclist_.as<uint32_t>();
+    a.elem = d_elem_.as<ElemFC>();
+    a.ladder = d_ladder_.as<double>();
+    a.perm = perm_on_ ? perm_.as<int32_t>() : nullptr;
+    a.inv = perm_on_ ? inv_.as<int32_t>() : nullptr;
+    f.D = D_.as<double>();
+    for (int c0 = 0; c0 < b; c0 += 16) {
+        f.X = X + c0;
+        f.ldx = ldx;
+        f.Y = Y + c0;
+        f.ldy = ldy;
+        f.bcols = std::min(16, b - c0);
+        if (f.bcols <= 8)
+            k_hfree<NW, 8><<<(unsigned)n_, 64, 0, stream_>>>(f);
+        else
+            k_hfree<NW, 16><<<(unsigned)n_, 64, 0, stream_>>>(f);
+        HIPCHK(hipGetLastError());
+    }
+}
+
 void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32, int ldx32) {
     const int64_t n = n_;
+    if (matfree_) {
+        // FP64 throughout (there are no stored FP32 values to use)
+        (void)low;
+        (void)X32;
+        (void)ldx32;
+        switch (KW_) {
+            case 1: spmm_free_t<4>(X, ldx, Y, ldy, b); break;
+            case 2: spmm_free_t<8>(X, ldx, Y, ldy, b); break;
+            case 3: spmm_free_t<12>(X, ldx, Y, ldy, b); break;
+            default: spmm_free_t<16>(X, ldx, Y, ldy, b); break;
+        }
+        return;
+    }
     const int64_t* rp = rowptr_.as<int64_t>();
     const int32_t* cl = col_.as<int32_t>();
     const double* D = D_.as<double>();
@@ -2356,11 +2541,27 @@ void Engine::dense_solve(int nev, std::vector<double>& E, EigStats& st) {
     w.ensure(n * sizeof(double));
     e.ensure(n * sizeof(double));
     info_.ensure(sizeof(int));
-    k_fill<double><<<grid_for(n * n, 256), 256, 0, stream_>>>(A.as<double>(), n * n, 0.0);
-    k_densify<<<(unsigned)n, 64, 0, stream_>>>(n, rowptr_.as<int64_t>(), col_.as<int32_t>(),
-                                               form_ == kFP64 ? val_.as<double>() : nullptr, hi_.as<float>(),
-                                               lo_.as<float>(), D_.as<double>(), A.as<double>());
-    HIPCHK(hipGetLastError());
+    if (matfree_) {
+        // H times identity columns, 16 at a time: each entry is one matrix
+        // element plus exact zeros, so A is the same as from stored entries
+        DevBuf xi, yb;
+        xi.ensure((size_t)n * 16 * sizeof(double));
+        yb.ensure((size_t)n * 16 * sizeof(double));
+        for (int64_t c0 = 0; c0 < n; c0 += 16) {
+            const int bc = (int)std::min<int64_t>(16, n - c0);
+            k_identity_block<<<grid_for(n * 16, 256), 256, 0, stream_>>>(xi.as<double>(), n, 16, c0, bc);
+            spmm(xi.as<double>(), 16, yb.as<double>(), 16, 16, false);
+            dim3 tg(grid_for(n, 32), 1);
+            k_rm_to_cm<<<tg, 256, 0, stream_>>>(yb.as<double>(), 16, n, bc, A.as<double>() + c0 * n, n, nullptr);
+            HIPCHK(hipGetLastError());
+        }
+    } else {
+        k_fill<double><<<grid_for(n * n, 256), 256, 0, stream_>>>(A.as<double>(), n * n, 0.0);
+        k_densify<<<(unsigned)n, 64, 0, stream_>>>(n, rowptr_.as<int64_t>(), col_.as<int32_t>(),
+                                                   form_ == kFP64 ? val_.as<double>() : nullptr, hi_.as<float>(),
+                                                   lo_.as<float>(), D_.as<double>(), A.as<double>());
+        HIPCHK(hipGetLastError());
+    }
     // The lower triangle is reduced from the first column, as Eigen's
     // SelfAdjointEigenSolver (src/Ham.cpp:372-379) and LAPACK's lower form
     // do.  For exactly degenerate eigenvalues (symmetry-equivalent modes)

@@ -99,6 +99,8 @@ class Engine {
     // DenseDiagonalize (Ham.cpp:372-379): all n eigenpairs.
     EigStats dense_all(std::vector<double>& E);
     // resident eigenvectors, row-major n x ncol
+    // products regenerate H (no stored entries): chosen when CSR would not fit
+    bool matrix_free() const { return matfree_; }
     int ritz_cols() const { return xcols_; }
     void get_vectors(double* C, int ncol) const;
     // PrintFreqs assignment scan of the first ncol resident eigenvectors on
@@ -192,6 +194,10 @@ class Engine {
     // Hamiltonian
     bool h_valid_ = false;
     int64_t nnz_ = 0;
+    bool matfree_ = false;  // products regenerate H from the hash (no CSR)
+    bool use_matrix_free(int64_t nnz) const;
+    template <int NW>
+    void spmm_free_t(const double* X, int ldx, double* Y, int ldy, int b);
     DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_;
     // solver order: H, D and the eigensolver's blocks index states in the
     // order perm_ (solver row -> basis state; inv_ the inverse); the
