@@ -667,77 +667,83 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
 }
 
 // Block product for exactly B contiguous row-major columns (ldx == B, rows
-// 16-byte aligned): one wave per row, each lane owns one stored entry per
-// step, so the column and value streams are read with full-width coalesced
-// loads (256 B of columns and 512 B of FP64 values per wave instruction)
-// and each lane gathers the whole B-wide row of X with 16-byte loads.  Two
-// entries per lane are in flight per step.  The B per-lane partial sums are
-// reduced by a transposing butterfly: at each of the first log2(B) steps a
-// lane keeps half of its columns and trades the other half with its
-// partner, so after 6 steps lane l holds column l / (64 / B).  The
-// summation order is fixed, so products are reproducible bit for bit.
+// 16-byte aligned): one wave per row.  The gathered row of X for a stored
+// entry is fetched cooperatively: P = B * (element bytes) / 16 lanes take
+// one 16-byte piece each, so one wave instruction brings E = 64 / P whole
+// rows (one cache line each) instead of a quarter of 64 rows.  Each lane
+// keeps U entries in flight and accumulates its W = B / P columns; the
+// wave then sums over the lanes that hold the same piece (fixed order, so
+// products are reproducible bit for bit).
 template <int B, int MODE>
-__device__ __forceinline__ void sp_xrow(const double* __restrict__ X, const float* __restrict__ X32, int64_t j,
-                                        typename SpAcc<MODE>::T (&x)[B]) {
+struct SpCoop {
+    static constexpr int XB = MODE == 1 ? 4 : 8;  // bytes per X element
+    static constexpr int W = 16 / XB;             // elements per lane piece
+    static constexpr int P = B / W;               // lanes per entry
+    static constexpr int E = 64 / P;              // entries per wave step
+    static constexpr int U = 8;                   // steps in flight per lane
+};
+
+// the W-element piece q of row j of X (FP64, or the FP32 copy for MODE 1)
+template <int B, int MODE>
+__device__ __forceinline__ void sp_piece(const double* __restrict__ X, const float* __restrict__ X32, int64_t j, int q,
+                                         typename SpAcc<MODE>::T (&x)[SpCoop<B, MODE>::W]) {
     if constexpr (MODE == 1) {
-        const float4* p = reinterpret_cast<const float4*>(X32 + j * B);
-#pragma unroll
-        for (int q = 0; q < B / 4; q++) {
-            const float4 t = p[q];
-            x[4 * q] = t.x;
-            x[4 * q + 1] = t.y;
-            x[4 * q + 2] = t.z;
-            x[4 * q + 3] = t.w;
-        }
+        const float4 t = *reinterpret_cast<const float4*>(X32 + j * B + 4 * q);
+        x[0] = t.x;
+        x[1] = t.y;
+        x[2] = t.z;
+        x[3] = t.w;
     } else {
-        const double2* p = reinterpret_cast<const double2*>(X + j * B);
-#pragma unroll
-        for (int q = 0; q < B / 2; q++) {
-            const double2 t = p[q];
-            x[2 * q] = t.x;
-            x[2 * q + 1] = t.y;
-        }
+        const double2 t = *reinterpret_cast<const double2*>(X + j * B + 2 * q);
+        x[0] = t.x;
+        x[1] = t.y;
     }
 }
 
-// dot products of one row segment [k0, k1) with the B columns of X, one
-// stored entry per lane per step (two in flight); returns, in lane l, the
-// full wave sum of column l / (64 / B) after the transposing butterfly
+// lane (q = lane % P, e = lane / P) sums entries k0 + e, k0 + e + E, ... of
+// its row against piece q; after the reduction lanes 0..P-1 hold the row's
+// products for columns q*W .. q*W+W-1
 template <int B, int MODE>
-__device__ __forceinline__ typename SpAcc<MODE>::T sp_lane_dot(int64_t k0, int64_t k1, int lane,
-                                                             const int32_t* __restrict__ col,
-                                                             const double* __restrict__ val,
-                                                             const float* __restrict__ hi,
-                                                             const float* __restrict__ lo,
-                                                             const float* __restrict__ X32,
-                                                             const double* __restrict__ X) {
+__device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, const int32_t* __restrict__ col,
+                                            const double* __restrict__ val, const float* __restrict__ hi,
+                                            const float* __restrict__ lo, const float* __restrict__ X32,
+                                            const double* __restrict__ X,
+                                            typename SpAcc<MODE>::T (&acc)[SpCoop<B, MODE>::W]) {
     using AT = typename SpAcc<MODE>::T;
-    static_assert(B == 4 || B == 8 || B == 16, "block width");
-    AT acc[B];
+    using C = SpCoop<B, MODE>;
+    constexpr int W = C::W, P = C::P, E = C::E, U = C::U;
+    const int q = lane % P, e = lane / P;
 #pragma unroll
-    for (int c = 0; c < B; c++) acc[c] = 0;
-    int64_t k = k0 + lane;
-    for (; k + 64 < k1; k += 128) {
-        const int64_t j0 = __builtin_nontemporal_load(col + k);
-        const int64_t j1 = __builtin_nontemporal_load(col + k + 64);
-        const AT v0 = sp_val<MODE>(val, hi, lo, k), v1 = sp_val<MODE>(val, hi, lo, k + 64);
-        AT x0[B], x1[B];
-        sp_xrow<B, MODE>(X, X32, j0, x0);
-        sp_xrow<B, MODE>(X, X32, j1, x1);
+    for (int c = 0; c < W; c++) acc[c] = 0;
+    int64_t k = k0 + e;
+    for (; k + (U - 1) * E < k1; k += U * E) {
+        int32_t j[U];
+        AT v[U];
+        AT x[U][W];
 #pragma unroll
-        for (int c = 0; c < B; c++) acc[c] = sp_fma(v0, x0[c], acc[c]);
+        for (int u = 0; u < U; u++) {
+            j[u] = __builtin_nontemporal_load(col + k + u * E);
+            v[u] = sp_val<MODE>(val, hi, lo, k + u * E);
+        }
 #pragma unroll
-        for (int c = 0; c < B; c++) acc[c] = sp_fma(v1, x1[c], acc[c]);
+        for (int u = 0; u < U; u++) sp_piece<B, MODE>(X, X32, j[u], q, x[u]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+#pragma unroll
+            for (int c = 0; c < W; c++) acc[c] = sp_fma(v[u], x[u][c], acc[c]);
     }
-    if (k < k1) {
-        const int64_t j0 = __builtin_nontemporal_load(col + k);
+    for (; k < k1; k += E) {
+        const int32_t j0 = __builtin_nontemporal_load(col + k);
         const AT v0 = sp_val<MODE>(val, hi, lo, k);
-        AT x0[B];
-        sp_xrow<B, MODE>(X, X32, j0, x0);
+        AT x0[W];
+        sp_piece<B, MODE>(X, X32, j0, q, x0);
 #pragma unroll
-        for (int c = 0; c < B; c++) acc[c] = sp_fma(v0, x0[c], acc[c]);
+        for (int c = 0; c < W; c++) acc[c] = sp_fma(v0, x0[c], acc[c]);
     }
-    return lane_colsum<B>(acc, lane);
+#pragma unroll
+    for (int off = P; off < 64; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < W; c++) acc[c] += __shfl_xor(acc[c], off);
 }
 
 template <int B, int MODE>
@@ -749,11 +755,16 @@ __global__ __launch_bounds__(256) void k_spmm_lane(int64_t n, const int64_t* __r
     const int lane = threadIdx.x & 63;
     const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (row >= n) return;
-    const auto s = sp_lane_dot<B, MODE>(rowptr[row], rowptr[row + 1], lane, col, val, hi, lo, X32, X);
-    constexpr int R = 64 / B;
-    if ((lane & (R - 1)) == 0) {
-        const int c = lane / R;
-        Y[row * ldy + c] = D[row] * X[row * B + c] + (double)s;
+    using C = SpCoop<B, MODE>;
+    typename SpAcc<MODE>::T acc[C::W];
+    sp_coop_dot<B, MODE>(rowptr[row], rowptr[row + 1], lane, col, val, hi, lo, X32, X, acc);
+    if (lane < C::P) {
+        const double d = D[row];
+#pragma unroll
+        for (int c = 0; c < C::W; c++) {
+            const int cc = lane * C::W + c;
+            Y[row * ldy + cc] = d * X[row * B + cc] + (double)acc[c];
+        }
     }
 }
 
