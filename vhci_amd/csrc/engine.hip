@@ -671,16 +671,15 @@ __global__ __launch_bounds__(256) void k_spmm(int64_t n, const int64_t* __restri
 // entry is fetched cooperatively: P = B * (element bytes) / 16 lanes take
 // one 16-byte piece each, so one wave instruction brings E = 64 / P whole
 // rows (one cache line each) instead of a quarter of 64 rows.  Each lane
-// keeps U entries in flight and accumulates its W = B / P columns; the
-// wave then sums over the lanes that hold the same piece (fixed order, so
-// products are reproducible bit for bit).
+// accumulates its W = B / P columns; the wave then sums over the lanes that
+// hold the same piece (fixed order, so products are reproducible bit for
+// bit).
 template <int B, int MODE>
 struct SpCoop {
     static constexpr int XB = MODE == 1 ? 4 : 8;  // bytes per X element
     static constexpr int W = 16 / XB;             // elements per lane piece
     static constexpr int P = B / W;               // lanes per entry
     static constexpr int E = 64 / P;              // entries per wave step
-    static constexpr int U = 8;                   // steps in flight per lane
 };
 
 // the W-element piece q of row j of X (FP64, or the FP32 copy for MODE 1)
@@ -702,7 +701,10 @@ __device__ __forceinline__ void sp_piece(const double* __restrict__ X, const flo
 
 // lane (q = lane % P, e = lane / P) sums entries k0 + e, k0 + e + E, ... of
 // its row against piece q; after the reduction lanes 0..P-1 hold the row's
-// products for columns q*W .. q*W+W-1
+// products for columns q*W .. q*W+W-1.  The column and value streams are
+// read once, one entry per lane (fully coalesced), in chunks of 64 entries
+// (C chunks in flight), and handed to the P lanes that gather each entry's
+// row by shuffles.
 template <int B, int MODE>
 __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, const int32_t* __restrict__ col,
                                             const double* __restrict__ val, const float* __restrict__ hi,
@@ -710,40 +712,50 @@ __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, co
                                             const double* __restrict__ X,
                                             typename SpAcc<MODE>::T (&acc)[SpCoop<B, MODE>::W]) {
     using AT = typename SpAcc<MODE>::T;
-    using C = SpCoop<B, MODE>;
-    constexpr int W = C::W, P = C::P, E = C::E, U = C::U;
+    using Cp = SpCoop<B, MODE>;
+    constexpr int W = Cp::W, P = Cp::P, E = Cp::E;
+    constexpr int C = 2;  // chunks of 64 entries in flight
     const int q = lane % P, e = lane / P;
 #pragma unroll
     for (int c = 0; c < W; c++) acc[c] = 0;
-    int64_t k = k0 + e;
-    for (; k + (U - 1) * E < k1; k += U * E) {
-        int32_t j[U];
-        AT v[U];
-        AT x[U][W];
+    int64_t k = k0;
+    for (; k + 64 * C <= k1; k += 64 * C) {
+        int32_t jl[C];
+        AT vl[C];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            j[u] = __builtin_nontemporal_load(col + k + u * E);
-            v[u] = sp_val<MODE>(val, hi, lo, k + u * E);
+        for (int c = 0; c < C; c++) {
+            jl[c] = __builtin_nontemporal_load(col + k + 64 * c + lane);
+            vl[c] = sp_val<MODE>(val, hi, lo, k + 64 * c + lane);
         }
+        AT v[C][P];
+        AT x[C][P][W];
 #pragma unroll
-        for (int u = 0; u < U; u++) sp_piece<B, MODE>(X, X32, j[u], q, x[u]);
+        for (int c = 0; c < C; c++)
 #pragma unroll
-        for (int u = 0; u < U; u++)
+            for (int u = 0; u < P; u++) {
+                const int32_t j = __shfl(jl[c], u * E + e);
+                v[c][u] = __shfl(vl[c], u * E + e);
+                sp_piece<B, MODE>(X, X32, j, q, x[c][u]);
+            }
 #pragma unroll
-            for (int c = 0; c < W; c++) acc[c] = sp_fma(v[u], x[u][c], acc[c]);
+        for (int c = 0; c < C; c++)
+#pragma unroll
+            for (int u = 0; u < P; u++)
+#pragma unroll
+                for (int w = 0; w < W; w++) acc[w] = sp_fma(v[c][u], x[c][u][w], acc[w]);
     }
-    for (; k < k1; k += E) {
+    for (k += e; k < k1; k += E) {
         const int32_t j0 = __builtin_nontemporal_load(col + k);
         const AT v0 = sp_val<MODE>(val, hi, lo, k);
         AT x0[W];
         sp_piece<B, MODE>(X, X32, j0, q, x0);
 #pragma unroll
-        for (int c = 0; c < W; c++) acc[c] = sp_fma(v0, x0[c], acc[c]);
+        for (int w = 0; w < W; w++) acc[w] = sp_fma(v0, x0[w], acc[w]);
     }
 #pragma unroll
     for (int off = P; off < 64; off <<= 1)
 #pragma unroll
-        for (int c = 0; c < W; c++) acc[c] += __shfl_xor(acc[c], off);
+        for (int w = 0; w < W; w++) acc[w] += __shfl_xor(acc[w], off);
 }
 
 template <int B, int MODE>
