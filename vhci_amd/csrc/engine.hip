@@ -1196,7 +1196,7 @@ __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double 
 // m0..m0+k.  *gate = 1 if any projection coefficient exceeds rel times the
 // smallest column norm of T, else 0 (one workgroup)
 __global__ __launch_bounds__(256) void k_gate_proj(const double* __restrict__ R, int m0, int k, int p, double rel,
-                                                   int* __restrict__ gate) {
+                                                   double* __restrict__ gate) {
     __shared__ int any;
     __shared__ double thr;
     if (threadIdx.x == 0) {
@@ -1213,7 +1213,7 @@ __global__ __launch_bounds__(256) void k_gate_proj(const double* __restrict__ R,
     }
     if (mine) atomicOr(&any, 1);
     __syncthreads();
-    if (threadIdx.x == 0) *gate = any;
+    if (threadIdx.x == 0) *gate = any ? 1.0 : 0.0;
 }
 
 // T[:, 0..kk) = T[:, 0..k) Z in place (T column-major n x k, ld; Z k x kk
@@ -1859,6 +1859,21 @@ thread_local int64_t t_dev_peak = 0;
 int64_t thread_device_bytes() { return t_dev_bytes; }
 int64_t thread_device_peak() { return t_dev_peak; }
 void thread_device_reset_peak() { t_dev_peak = t_dev_bytes; }
+
+HostBuf::~HostBuf() {
+    if (p) (void)hipHostFree(p);
+}
+void* HostBuf::ensure(size_t bytes) {
+    if (bytes <= cap && p) return p;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes, 4096);
+    if (hipHostMalloc(&p, want, hipHostMallocDefault) != hipSuccess || !p)
+        throw std::runtime_error("page-locked host allocation failed");
+    cap = want;
+    return p;
+}
 
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
@@ -2767,8 +2782,10 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
             BLASCHK(rocblas_dgemm(blas_, rocblas_operation_transpose, rocblas_operation_none, kk, kk, ni, &one, T, ld,
                                   T, ld, &zero, gram_.as<double>(), kk));
         g.resize((size_t)kk * kk);
-        HIPCHK(hipMemcpyAsync(g.data(), gram_.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        double* hg = static_cast<double*>(hr_.ensure(g.size() * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(hg, gram_.p, g.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
+        std::copy(hg, hg + g.size(), g.begin());
     };
     std::vector<double> g, before;
     auto cgs = [&]() {
@@ -2792,15 +2809,14 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
         // smallest column norm of T (it is 1e-16..1e-14 otherwise), and the
         // Gram matrix is then formed again.
         const int p = m0 + k;
-        S_.ensure((size_t)p * k * sizeof(double));
+        S_.ensure(((size_t)p * k + 1) * sizeof(double));
         tsm_tn(stream_, tsm_part_, n, p, k, V, ld, T, ld, S_.as<double>(), p);
-        int* gate = info_.as<int>() + 2;
-        k_gate_proj<<<1, 256, 0, stream_>>>(S_.as<double>(), m0, k, p, kOrthoSkip, gate);
-        std::vector<double> R((size_t)p * k);
-        int flag = 0;
-        HIPCHK(hipMemcpyAsync(R.data(), S_.p, R.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
-        HIPCHK(hipMemcpyAsync(&flag, gate, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        // the gate flag follows R in S_, so one copy brings both
+        k_gate_proj<<<1, 256, 0, stream_>>>(S_.as<double>(), m0, k, p, kOrthoSkip, S_.as<double>() + (size_t)p * k);
+        const double* R = static_cast<double*>(hr_.ensure(((size_t)p * k + 1) * sizeof(double)));
+        HIPCHK(hipMemcpyAsync(hr_.p, S_.p, ((size_t)p * k + 1) * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
+        const int flag = R[(size_t)p * k] != 0.0;
         g.assign((size_t)k * k, 0.0);
         for (int c = 0; c < k; c++)
             for (int i = 0; i < k; i++) g[(size_t)c * k + i] = R[(size_t)c * p + m0 + i];
@@ -2865,7 +2881,11 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
             for (int i = 0; i < k; i++) Z[(size_t)c * k + i] = dsc[i] * U[(size_t)src * k + i] * sc;
         }
         S_.ensure((size_t)std::max(std::max(1, m0) * k, k * kk) * sizeof(double));
-        HIPCHK(hipMemcpyAsync(S_.p, Z.data(), Z.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+        // Z goes up from page-locked memory that stays valid until the copy
+        // has run (it is rewritten only after a later synchronisation)
+        double* hz = static_cast<double*>(hz_.ensure(Z.size() * sizeof(double)));
+        std::copy(Z.begin(), Z.end(), hz);
+        HIPCHK(hipMemcpyAsync(S_.p, hz, Z.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
         if (small) {
             const unsigned rg = std::min<unsigned>(grid_for(n, 256), 2048);
             if (k <= 8) k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>());
@@ -2878,7 +2898,7 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
             HIPCHK(hipMemcpy2DAsync(T, ld * sizeof(double), otmp_.p, n * sizeof(double), n * sizeof(double), kk,
                                     hipMemcpyDeviceToDevice, stream_));
         }
-        HIPCHK(hipStreamSynchronize(stream_));
+        // no wait here: everything after this is ordered on the stream
         k = kk;
         // a well-conditioned block is done; otherwise repeat projection and
         // orthonormalisation once more on the new block

@@ -25,6 +25,20 @@ int64_t thread_device_bytes();
 int64_t thread_device_peak();
 void thread_device_reset_peak();
 
+// page-locked host buffer (hipHostMalloc) for small per-step transfers:
+// copies from and to it are DMA, asynchronous to the host
+struct HostBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf();
+    void* ensure(size_t bytes);  // contents are not kept when it grows
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -194,6 +208,7 @@ class Engine {
     // Hamiltonian
     bool h_valid_ = false;
     int64_t nnz_ = 0;
+    HostBuf hr_, hz_;  // block Gram-Schmidt: projections down, block mix up
     bool matfree_ = false;  // products regenerate H from the hash (no CSR)
     bool use_matrix_free(int64_t nnz) const;
     template <int NW>
