@@ -2334,9 +2334,9 @@ bool Engine::use_matrix_free(int64_t nnz) const {
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
     const double sv = form_ == kFP64 ? 8.0 : 8.0;  // FP64 value, or FP32 hi + lo
     const double csr = (double)nnz * (sv + 4.0) * 2.0;  // entries, plus the sort's copy of a slice
-    // Krylov-Schur blocks (V, W: 3.5 nev + bs columns each; X, HX: 1.6 nev each)
+    // Krylov-Schur blocks (V, W: 4 nev + bs columns each; X, HX: 1.6 nev each)
     const double nev = std::max(1, P_.neig);
-    const double eig = (double)n_ * 8.0 * (2.0 * (3.5 * nev + 8.0) + 2.0 * 1.6 * nev);
+    const double eig = (double)n_ * 8.0 * (2.0 * (4.0 * nev + 8.0) + 2.0 * 1.6 * nev);
     const double held = (double)(col_.cap + val_.cap + hi_.cap + lo_.cap);  // released first
     return csr + eig > 0.9 * ((double)free_b + held);
 }
@@ -3073,12 +3073,15 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     const int ni = (int)n;
     int bs = 8;
     if (const char* e = std::getenv("VHCI_KS_BLOCK")) bs = std::max(1, std::min(32, std::atoi(e)));
-    // basis of m = 3.5 nev vectors (a multiple of bs, at least nev + 6
+    // basis of m = 4 nev vectors (a multiple of bs, at least nev + 6
     // blocks), restarted from the keep = 1.6 nev lowest Ritz vectors: each
-    // cycle adds about 1.9 nev new vectors per Rayleigh-Ritz step (on
-    // config 1 this is 12 % faster than m = 3 nev, keep = 2.1 nev)
+    // cycle adds about 2.4 nev new vectors per Rayleigh-Ritz step (on the
+    // 15-mode, 100-state workload 7 % faster than m = 3.5 nev, and the
+    // 8-36-mode request mix slightly faster too)
     const auto up = [&](double x) { return ((int)std::ceil(x) + bs - 1) / bs * bs; };
-    int m = std::max(up(3.5 * nev), up((double)nev + 6 * bs));
+    double mfac = 4.0;
+    if (const char* e = std::getenv("VHCI_KS_MFAC")) mfac = std::max(2.0, std::atof(e));
+    int m = std::max(up(mfac * nev), up((double)nev + 6 * bs));
     if (const char* e = std::getenv("VHCI_KS_DIM")) m = std::max(nev + 2 * bs, (std::atoi(e) / bs) * bs);
     while (m + bs > n) m -= bs;
     if (m < nev + 2 * bs) throw std::runtime_error("Krylov-Schur: basis too small for the requested states");
