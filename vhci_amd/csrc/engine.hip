@@ -1134,6 +1134,69 @@ __global__ __launch_bounds__(256) void k_tsm_tn(int64_t n, int p, int q, const d
         }
 }
 
+// The same partial products for wide B (q > 16): a 64 x 64 output tile per
+// block, 4 x 4 outputs per thread, so A is read q/64 times instead of q/16.
+__global__ __launch_bounds__(256) void k_tsm_tn_wide(int64_t n, int p, int q, const double* __restrict__ A,
+                                                     int64_t lda, const double* __restrict__ B, int64_t ldb,
+                                                     int64_t rows_per_chunk, double* __restrict__ part) {
+    __shared__ double As[32][64 + 2];
+    __shared__ double Bs[32][64 + 2];
+    const int t = threadIdx.x;
+    const int a0 = blockIdx.y * 64, c0 = blockIdx.z * 64;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_chunk;
+    const int64_t r1 = min(n, r0 + rows_per_chunk);
+    const int ta = t & 15, tc = t >> 4;  // outputs (4 ta + u, 4 tc + v)
+    const int ai = t & 31, aj = t >> 5;  // staging: rows ai, columns aj + 8 s
+    double ra[8], rb[8];
+    auto fetch = [&](int64_t rt) {
+        const int64_t r = rt + ai;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = aj + 8 * u;
+            ra[u] = (r < r1 && a0 + j < p) ? A[r + (int64_t)(a0 + j) * lda] : 0.0;
+            rb[u] = (r < r1 && c0 + j < q) ? B[r + (int64_t)(c0 + j) * ldb] : 0.0;
+        }
+    };
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
+    if (r0 < r1) fetch(r0);
+    for (int64_t rt = r0; rt < r1; rt += 32) {
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            As[ai][aj + 8 * u] = ra[u];
+            Bs[ai][aj + 8 * u] = rb[u];
+        }
+        __syncthreads();
+        if (rt + 32 < r1) fetch(rt + 32);
+#pragma unroll 4
+        for (int i = 0; i < 32; i++) {
+            double x[4], y[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) x[u] = As[i][4 * ta + u];
+#pragma unroll
+            for (int v = 0; v < 4; v++) y[v] = Bs[i][4 * tc + v];
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) acc[u][v] = fma(x[u], y[v], acc[u][v]);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int a = a0 + 4 * ta + u;
+        if (a >= p) continue;
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int c = c0 + 4 * tc + v;
+            if (c < q) part[((int64_t)blockIdx.x * q + c) * p + a] = acc[u][v];
+        }
+    }
+}
+
 // C[a + c*ldc] = sum over chunks: 16 outputs per block, the chunks split
 // over 16 lanes groups (chunk w, w + 16, ...), combined in a fixed order
 // (deterministic)
@@ -2646,6 +2709,9 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
     if (q <= 8) {
         dim3 g(nchunk, (p + 127) / 128, 1);
         k_tsm_tn<128, 8><<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
+    } else if (q > 16 && p > 16) {
+        dim3 g(nchunk, (p + 63) / 64, (q + 63) / 64);
+        k_tsm_tn_wide<<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
     } else {
         dim3 g(nchunk, (p + 63) / 64, (q + 15) / 16);
         k_tsm_tn<64, 16><<<g, 256, 0, s>>>(n, p, q, A, lda, B, ldb, rpc, part.as<double>());
