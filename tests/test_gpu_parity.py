@@ -347,3 +347,32 @@ def test_matrix_free_matches_stored(name, monkeypatch):
     ctx.close()
     assert r["nvar"] == ref["nvar"] and r["ns2"] == ref["ns2"] and r["checksum"] == ref["checksum"]
     assert np.max(np.abs(r["E"] - ref["E"])) < 1e-8
+
+
+SERVE_MIX = dict(m_range=(8, 14), eps1_set=(1.0, 2.0), nstate_range=(20, 200), eps2_set=(0.0, 0.01, 0.1))
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_incremental_build_matches_full(form, monkeypatch):
+    """Each heat-bath iteration rebuilds H from the previous one (new rows
+    probed, old rows carried over with the transposes of the new entries);
+    the matrix must be the one a full build gives, bit for bit.  The
+    request grows past 16384 states, so the solver order appears midway."""
+    text = synth.request_input(5, **SERVE_MIX)
+    out = {}
+    for full in (False, True):
+        if full:
+            monkeypatch.setenv("VHCI_FULL_BUILD", "1")
+        else:
+            monkeypatch.delenv("VHCI_FULL_BUILD", raising=False)
+        ctx = vhci_amd.Context(vhci_amd.Problem(text=text), form)
+        r = ctx.run()
+        out[full] = (r, ctx.iteration_sizes(), ctx.get_h())
+        ctx.close()
+    (ri, si, hi), (rf, sf, hf) = out[False], out[True]
+    _say("[incremental build] sizes", si)
+    assert si == sf and si[-1] > 16384
+    assert ri["checksum"] == rf["checksum"] and ri["ns2"] == rf["ns2"]
+    for a, b in zip(hi, hf):
+        assert np.array_equal(a, b)
+    assert np.array_equal(ri["E"], rf["E"])

@@ -311,6 +311,15 @@ struct HBuildArgs {
     float* hi;
     float* lo;
     int form;
+    // incremental build (n_old > 0): block b builds the row of basis state
+    // first + b; a hit on an old state j < n_old also counts (extra[j]) or
+    // stores (after oldcnt[j] entries, at cursor[j]) the transposed entry in
+    // j's row
+    int64_t first;
+    int64_t n_old;
+    unsigned long long* extra;
+    const int64_t* oldcnt;
+    unsigned long long* cursor;
 };
 
 // sum over the group's FCs of fc * prod_t L[k][q_start][q_end], in the
@@ -388,8 +397,14 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     __shared__ int32_t qj[2 * kWave];
     __shared__ int32_t qp[2 * kWave];
     const int lane = threadIdx.x;
-    const int64_t row = blockIdx.x;                      // stored row
-    const int64_t src = a.perm ? a.perm[row] : row;      // its basis state
+    int64_t row, src;  // stored row and its basis state
+    if (a.n_old > 0) {
+        src = a.first + blockIdx.x;
+        row = a.inv ? a.inv[src] : src;
+    } else {
+        row = blockIdx.x;
+        src = a.perm ? a.perm[row] : row;
+    }
     uint32_t w[NW];
     key_load<NW>(a.keys, src, w);
     if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
@@ -408,12 +423,24 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
             const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, delta_nz(e), src < j);
             const int64_t pos = base + lane;
             a.col[pos] = a.inv ? a.inv[j] : (int32_t)j;
+            // an old state's row gets the transposed entry (same value: the
+            // element starts from the lower basis index either way)
+            const bool tr = j < a.n_old;
+            const int64_t pt =
+                tr ? a.rowptr[a.inv ? a.inv[j] : j] + a.oldcnt[j] + (int64_t)atomicAdd(&a.cursor[j], 1ull) : 0;
+            if (tr) a.col[pt] = (int32_t)row;
             if (a.form == kFP64) {
                 a.val[pos] = v;
+                if (tr) a.val[pt] = v;
             } else {
                 const float h = (float)v;
+                const float l = (float)(v - (double)h);
                 a.hi[pos] = h;
-                a.lo[pos] = (float)(v - (double)h);
+                a.lo[pos] = l;
+                if (tr) {
+                    a.hi[pt] = h;
+                    a.lo[pt] = l;
+                }
             }
         }
         base += count;
@@ -435,6 +462,7 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         const uint64_t bal = __ballot(hit);
         if (!FILL) {
             cnt += __popcll(bal);
+            if (hit && j < a.n_old) atomicAdd(&a.extra[j], 1ull);
         } else {
             if (hit) {
                 const int slot = nq + __popcll(bal & ((1ull << lane) - 1ull));
@@ -544,6 +572,51 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
     if ((lane & (R - 1)) == 0) {
         const int c = lane / R;
         if (c < f.bcols) f.Y[row * f.ldy + c] = f.D[row] * f.X[row * f.ldx + c] + s;
+    }
+}
+
+// Incremental build: rows of the n_old states of the previous build.  Each
+// keeps its old entries (old solver order -> basis state -> new solver
+// order; both orders sort the same keys, so the columns stay ascending) and
+// gains extra[x] entries for the new states.
+__global__ void k_old_counts(int64_t n_old, const int32_t* __restrict__ inv, const int32_t* __restrict__ inv_o,
+                             const int64_t* __restrict__ rowptr_o, const unsigned long long* __restrict__ extra,
+                             int64_t* __restrict__ oldcnt, int64_t* __restrict__ rowcnt) {
+    const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (x >= n_old) return;
+    const int64_t ro = inv_o ? inv_o[x] : x;
+    const int64_t oc = rowptr_o[ro + 1] - rowptr_o[ro];
+    oldcnt[x] = oc;
+    rowcnt[inv ? inv[x] : x] = oc + (int64_t)extra[x];
+}
+
+// one wave per old state
+__global__ __launch_bounds__(256) void k_copy_old_rows(int64_t n_old, const int32_t* __restrict__ inv,
+                                                       const int32_t* __restrict__ inv_o,
+                                                       const int32_t* __restrict__ perm_o,
+                                                       const int64_t* __restrict__ rowptr_o,
+                                                       const int32_t* __restrict__ col_o,
+                                                       const double* __restrict__ val_o,
+                                                       const float* __restrict__ hi_o, const float* __restrict__ lo_o,
+                                                       const int64_t* __restrict__ rowptr, int32_t* __restrict__ col,
+                                                       double* __restrict__ val, float* __restrict__ hi,
+                                                       float* __restrict__ lo, int form) {
+    const int lane = threadIdx.x & 63;
+    const int64_t x = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (x >= n_old) return;
+    const int64_t ro = inv_o ? inv_o[x] : x;
+    const int64_t k0 = rowptr_o[ro], k1 = rowptr_o[ro + 1];
+    const int64_t d = rowptr[inv ? inv[x] : x] - k0;
+    for (int64_t k = k0 + lane; k < k1; k += 64) {
+        const int64_t c = col_o[k];
+        const int64_t b = perm_o ? perm_o[c] : c;
+        col[k + d] = inv ? inv[b] : (int32_t)b;
+        if (form == kFP64) {
+            val[k + d] = val_o[k];
+        } else {
+            hi[k + d] = hi_o[k];
+            lo[k + d] = lo_o[k];
+        }
     }
 }
 
@@ -2029,6 +2102,7 @@ void Engine::set_basis(const int32_t* q, int64_t n) {
     keys_.ensure(packed.size());
     HIPCHK(hipMemcpyAsync(keys_.p, packed.data(), packed.size(), hipMemcpyHostToDevice, stream_));
     n_ = n;
+    built_n_ = 0;
     h_valid_ = false;
     xrows_ = 0;
     xcols_ = 0;
@@ -2074,6 +2148,7 @@ void Engine::set_initial_basis() {
                                                            keys_.as<uint8_t>());
     HIPCHK(hipGetLastError());
     n_ = n0;
+    built_n_ = 0;
     h_valid_ = false;
     xrows_ = 0;
     xcols_ = 0;
@@ -2156,13 +2231,53 @@ void Engine::build_h_t(HStats& st) {
     a.elem = d_elem_.as<ElemFC>();
     a.ladder = d_ladder_.as<double>();
     a.form = form_;
+    // Incremental build when the previous stored H covers the first n_old
+    // states (the basis only grew since) and fewer than half are new: only
+    // the new states' rows are probed; old rows keep their entries and gain
+    // the transposes of the new rows' entries to them.
+    const int64_t n_old = built_n_;
+    const char* fb = std::getenv("VHCI_FULL_BUILD");
+    const bool incr = n_old > 0 && n_old < n && 2 * (n - n_old) < n && a.P > 0 && !matfree_ &&
+                      !(fb && std::string(fb) == "1");
+    built_n_ = 0;
+    const bool old_perm_on = perm_on_;
+    auto swap_buf = [](DevBuf& x, DevBuf& y) {
+        std::swap(x.p, y.p);
+        std::swap(x.cap, y.cap);
+    };
+    if (incr) {
+        swap_buf(perm_, perm_o_);
+        swap_buf(inv_, inv_o_);
+        swap_buf(rowptr_, rowptr_o_);
+        swap_buf(col_, col_o_);
+        swap_buf(val_, val_o_);
+        swap_buf(hi_, hi_o_);
+        swap_buf(lo_, lo_o_);
+    }
     make_solver_order();
     a.perm = perm_on_ ? perm_.as<int32_t>() : nullptr;
     a.inv = perm_on_ ? inv_.as<int32_t>() : nullptr;
     rowcnt_.ensure((n + 1) * sizeof(int64_t));
     rowptr_.ensure((n + 1) * sizeof(int64_t));
     a.rowcnt = rowcnt_.as<int64_t>();
-    if (a.P > 0) {
+    if (incr) {
+        extra_.ensure(n_old * sizeof(uint64_t));
+        oldcnt_.ensure(n_old * sizeof(int64_t));
+        cursor_.ensure(n_old * sizeof(uint64_t));
+        k_fill<uint64_t><<<grid_for(n_old, 256), 256, 0, stream_>>>(extra_.as<uint64_t>(), n_old, 0ull);
+        k_fill<uint64_t><<<grid_for(n_old, 256), 256, 0, stream_>>>(cursor_.as<uint64_t>(), n_old, 0ull);
+        a.first = n_old;
+        a.n_old = n_old;
+        a.extra = extra_.as<unsigned long long>();
+        a.oldcnt = oldcnt_.as<int64_t>();
+        a.cursor = cursor_.as<unsigned long long>();
+        k_hbuild<NW, false><<<(unsigned)(n - n_old), 64, 0, stream_>>>(a);
+        HIPCHK(hipGetLastError());
+        k_old_counts<<<grid_for(n_old, 256), 256, 0, stream_>>>(
+            n_old, a.inv, old_perm_on ? inv_o_.as<int32_t>() : nullptr, rowptr_o_.as<int64_t>(),
+            extra_.as<unsigned long long>(), oldcnt_.as<int64_t>(), a.rowcnt);
+        HIPCHK(hipGetLastError());
+    } else if (a.P > 0) {
         k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     } else {
@@ -2186,6 +2301,10 @@ void Engine::build_h_t(HStats& st) {
         val_.release();
         hi_.release();
         lo_.release();
+        col_o_.release();
+        val_o_.release();
+        hi_o_.release();
+        lo_o_.release();
         D_.ensure(n * sizeof(double));
         k_diag<NW><<<grid_for(n, 128), 128, 0, stream_>>>(n, P_.M, a.keys, d_freq_.as<double>(),
                                                           d_diag_.as<uint32_t>(), (int)T_.diag_fcs.size(), a.elem,
@@ -2208,7 +2327,17 @@ void Engine::build_h_t(HStats& st) {
     a.val = val_.as<double>();
     a.hi = hi_.as<float>();
     a.lo = lo_.as<float>();
-    if (a.P > 0 && nnz > 0) {
+    if (incr) {
+        k_copy_old_rows<<<grid_for(n_old, 4), 256, 0, stream_>>>(
+            n_old, a.inv, old_perm_on ? inv_o_.as<int32_t>() : nullptr, old_perm_on ? perm_o_.as<int32_t>() : nullptr,
+            rowptr_o_.as<int64_t>(), col_o_.as<int32_t>(), val_o_.as<double>(), hi_o_.as<float>(), lo_o_.as<float>(),
+            a.rowptr, a.col, a.val, a.hi, a.lo, form_);
+        HIPCHK(hipGetLastError());
+        if (nnz > 0) {
+            k_hbuild<NW, true><<<(unsigned)(n - n_old), 64, 0, stream_>>>(a);
+            HIPCHK(hipGetLastError());
+        }
+    } else if (a.P > 0 && nnz > 0) {
         k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     }
@@ -2279,6 +2408,13 @@ void Engine::build_h_t(HStats& st) {
                                                       D_.as<double>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
+    if (incr) {
+        col_o_.release();
+        val_o_.release();
+        hi_o_.release();
+        lo_o_.release();
+    }
+    built_n_ = n;
     st.n = n;
     st.nnz_offdiag = nnz;
 }

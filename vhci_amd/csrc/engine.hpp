@@ -209,6 +209,11 @@ class Engine {
     bool h_valid_ = false;
     int64_t nnz_ = 0;
     HostBuf hr_, hz_;  // block Gram-Schmidt: projections down, block mix up
+    // incremental H build: basis size of the last stored build whose states
+    // are still the first ones of the basis (0: none), and the previous
+    // order and matrix while the next one is assembled
+    int64_t built_n_ = 0;
+    DevBuf perm_o_, inv_o_, rowptr_o_, col_o_, val_o_, hi_o_, lo_o_, extra_, oldcnt_, cursor_;
     bool matfree_ = false;  // products regenerate H from the hash (no CSR)
     bool use_matrix_free(int64_t nnz) const;
     template <int NW>
