@@ -92,12 +92,16 @@ def phase_rooflines(ph, sp):
     K = ph["key_bytes"]
     n, nnz, P = ph["n"], ph["nnz"], ph["patterns"]
     sv = 8.0  # FP64 value, or FP32 hi + FP32 lo
-    probes = n * P
-    hb = probes * (K + 4) + nnz * (sv + 4) + 8 * (n + 1) + n * K
+    probes = n * P  # every variational state's connections (PT2 probes them all)
+    # an incremental build probes only the new rows and reads the carried entries once more
+    build_probes = ph.get("probed_rows", n) * P
+    hb = build_probes * (K + 4) + nnz * (sv + 4) + ph.get("carried", 0) * (sv + 4) + 8 * (n + 1) + n * K
     sel = n * (K + 8) + ph["hb_candidates"] * (K + 4) + ph["hb_unique"] * (2 * K + 4)
     ns = ph["nstate"]
     pt2 = n * (K + 8 * ns) + probes * (K + 4) + ph["pairs"] * 8 * ns * 2 + ph["ns2"] * (2 * K + 4 + 8 * ns)
-    out = {"h_build": {"n": n, "stored_entries": nnz, "probes": probes, "bytes": hb, "ms": ph["build_ms"],
+    out = {"h_build": {"n": n, "stored_entries": nnz, "probed_rows": ph.get("probed_rows", n),
+                       "carried_entries": ph.get("carried", 0), "probes": build_probes, "bytes": hb,
+                       "ms": ph["build_ms"],
                        "GBs": hb / ph["build_ms"] / 1e6 if ph["build_ms"] > 0 else None},
            "pt2_select": {"candidates": ph["hb_candidates"], "unique": ph["hb_unique"], "bytes": sel,
                           "ms": ph["hb_ms"], "GBs": sel / ph["hb_ms"] / 1e6 if ph["hb_ms"] > 0 else None},

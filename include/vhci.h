@@ -182,10 +182,12 @@ int vhci_spmm_profile_read(vhci_ctx* ctx, double* out);
  * support; no counterpart in the reference. */
 int vhci_gemm_profile_read(vhci_ctx* ctx, double* out);
 /* Counters of the last phases (measurement support; no counterpart in the
- * reference), out[12]: last H build n, stored entries, connection
+ * reference), out[14]: last H build n, stored entries, connection
  * patterns, build ms; last heat-bath selection (after vhci_pt2: the PT2
  * one) candidates, unique states, ms; last PT2 external states,
- * (external, variational) pairs, sum ms; key bytes; PT2 states. */
+ * (external, variational) pairs, sum ms; key bytes; PT2 states; rows the
+ * last H build probed (all, or only the new states' rows of an incremental
+ * build); entries it carried over from the previous build. */
 int vhci_phase_stats(vhci_ctx* ctx, double* out);
 
 /* ---- whole solve (Perform_HCI + DoPT2 of one input) --------------------

@@ -2313,6 +2313,7 @@ void Engine::build_h_t(HStats& st) {
         HIPCHK(hipStreamSynchronize(stream_));
         st.n = n;
         st.nnz_offdiag = nnz;
+        st.probed_rows = incr ? n - n_old : n;
         return;
     }
     col_.ensure(std::max<int64_t>(1, nnz) * sizeof(int32_t));
@@ -2417,6 +2418,12 @@ void Engine::build_h_t(HStats& st) {
     built_n_ = n;
     st.n = n;
     st.nnz_offdiag = nnz;
+    st.probed_rows = incr ? n - n_old : n;
+    if (incr) {
+        int64_t oe = 0;
+        HIPCHK(hipMemcpy(&oe, rowptr_o_.as<int64_t>() + n_old, sizeof(int64_t), hipMemcpyDeviceToHost));
+        st.carried = oe;
+    }
 }
 
 HStats Engine::build_h() {
@@ -2742,7 +2749,7 @@ void Engine::gemm_profile_read(double out[3]) {
     out[2] = (double)prof_.gemm_calls;
 }
 
-void Engine::phase_stats(double out[12]) const {
+void Engine::phase_stats(double out[14]) const {
     out[0] = (double)last_h_.n;
     out[1] = (double)last_h_.nnz_offdiag;
     out[2] = (double)P_deltas_;
@@ -2755,6 +2762,8 @@ void Engine::phase_stats(double out[12]) const {
     out[9] = last_pt2_.sum_ms;
     out[10] = 16.0 * KW_;
     out[11] = (double)last_pt2_nopt_;
+    out[12] = (double)last_h_.probed_rows;
+    out[13] = (double)last_h_.carried;
 }
 
 void Engine::spmm_profile_read(double out[3]) {

@@ -57,6 +57,8 @@ struct HStats {
     int64_t nnz_offdiag = 0;     // stored off-diagonal entries (both triangles)
     int64_t nnz_struct = -1;     // reference's structural count, if computed
     double build_ms = 0;
+    int64_t probed_rows = 0;     // rows whose connections were probed (all, or the new ones)
+    int64_t carried = 0;         // entries carried over from the previous build
 };
 
 struct EigStats {
@@ -157,7 +159,7 @@ class Engine {
     // heat-bath selection (the PT2 one after pt2()): candidates, unique, ms;
     // last PT2: external states, (external, variational) pairs, sum ms; key
     // bytes; PT2 states
-    void phase_stats(double out[12]) const;
+    void phase_stats(double out[14]) const;
     // algorithmic bytes of one product of b vectors: stored entries (value +
     // column), row pointers, the diagonal, X read once and Y written once
     double spmm_bytes(int b, int mode) const;
