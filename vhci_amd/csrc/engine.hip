@@ -817,13 +817,22 @@ __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, co
 #pragma unroll
                 for (int w = 0; w < W; w++) acc[w] = sp_fma(v[c][u], x[c][u][w], acc[w]);
     }
-    for (k += e; k < k1; k += E) {
-        const int32_t j0 = __builtin_nontemporal_load(col + k);
-        const AT v0 = sp_val<MODE>(val, hi, lo, k);
-        AT x0[W];
-        sp_piece<B, MODE>(X, X32, j0, q, x0);
+    // the rest of the row: 64-entry chunks, masked, handed out the same way
+    for (; k < k1; k += 64) {
+        const bool in = k + lane < k1;
+        const int32_t jl = in ? __builtin_nontemporal_load(col + k + lane) : 0;
+        const AT vl = in ? sp_val<MODE>(val, hi, lo, k + lane) : AT(0);
 #pragma unroll
-        for (int w = 0; w < W; w++) acc[w] = sp_fma(v0, x0[w], acc[w]);
+        for (int u = 0; u < P; u++) {
+            const int32_t j = __shfl(jl, u * E + e);
+            const AT v = __shfl(vl, u * E + e);
+            if (k + u * E + e < k1) {
+                AT x[W];
+                sp_piece<B, MODE>(X, X32, j, q, x);
+#pragma unroll
+                for (int w = 0; w < W; w++) acc[w] = sp_fma(v, x[w], acc[w]);
+            }
+        }
     }
 #pragma unroll
     for (int off = P; off < 64; off <<= 1)
