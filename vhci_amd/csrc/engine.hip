@@ -787,7 +787,7 @@ __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, co
     using AT = typename SpAcc<MODE>::T;
     using Cp = SpCoop<B, MODE>;
     constexpr int W = Cp::W, P = Cp::P, E = Cp::E;
-    constexpr int C = 2;  // chunks of 64 entries in flight
+    constexpr int C = P <= 4 ? 4 : 2;  // chunks of 64 entries in flight (fewer for wide rows: registers)
     const int q = lane % P, e = lane / P;
 #pragma unroll
     for (int c = 0; c < W; c++) acc[c] = 0;
