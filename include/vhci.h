@@ -159,6 +159,13 @@ int vhci_append_new_states(vhci_ctx* ctx);
  * for n < nopt.  *ns2 (nullable) receives the external-space size. */
 int vhci_pt2(vhci_ctx* ctx, const double* E, int nopt, double eps2, double* dE, int64_t* ns2);
 
+/* Y = H X with the resident H for b vectors (1 <= b <= 64): X and Y are
+ * host row-major n x b in basis order.  Replaces the product operator the
+ * reference hands to its eigensolver (SparseSymMatProd, src/Ham.cpp:349,
+ * the whole H applied to a block).  fp32 != 0 selects the mixed form's FP32
+ * product (FP32 values and vectors); otherwise the product is FP64 (FP64
+ * values, or hi + lo in the mixed form).  Runs the solver's own kernels. */
+int vhci_apply_h(vhci_ctx* ctx, int b, int fp32, const double* X, double* Y);
 /* Time b-vector SpMMs with the resident H (reps products, HIP events on
  * the context stream).  out[0] = ms per product, out[1] = algorithmic
  * bytes per product, out[2] = kernel launches per product.  fp32 != 0

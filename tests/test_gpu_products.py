@@ -1,0 +1,110 @@
+"""GPU parity of the product kernels: H X through the solver's own kernels
+(vhci_apply_h) against a float64 product of the stored H copied to the host,
+for every block-width class the dispatcher distinguishes, in each precision
+form, in basis order and in the solver's row order, and in matrix-free mode.
+
+The error of each output element is measured against sum_j |H_ij| |X_jc|,
+the scale its rounding error is proportional to:
+- FP64 products (and the mixed form's hi + lo product, whose values get_h
+  returns exactly as hi + lo): below 1e-12 of that scale;
+- the mixed form's FP32 product (FP32 values, FP32 vectors, FP32 sums):
+  below 1e-4 of it, and above 1e-9 (it must really be the FP32 product).
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from helpers import GOLDEN
+
+import vhci_amd
+from vhci_amd import synth
+
+pytestmark = pytest.mark.gpu
+
+MIX = dict(m_range=(8, 14), eps1_set=(1.0, 2.0), nstate_range=(20, 200), eps2_set=(0.0, 0.01, 0.1))
+WIDTHS = (1, 2, 3, 4, 5, 8, 12, 16, 24, 40, 64)
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _stored(ctx):
+    rp, col, val, diag = ctx.get_h()
+    n = len(diag)
+    return sp.csr_matrix((val, col, rp), shape=(n, n)), diag
+
+
+def _errors(ctx, H, diag, fp32, widths=WIDTHS):
+    rng = np.random.default_rng(11)
+    Habs = abs(H)
+    out = {}
+    for b in widths:
+        X = rng.standard_normal((H.shape[0], b))
+        want = H @ X + diag[:, None] * X
+        scale = Habs @ np.abs(X) + np.abs(diag)[:, None] * np.abs(X)
+        got = ctx.apply_h(X if b > 1 else X[:, 0], fp32=fp32).reshape(-1, b)
+        out[b] = float(np.max(np.abs(got - want) / scale))
+        _say("  b=%d fp32=%s max scaled error %.2e" % (b, fp32, out[b]))
+    return out
+
+
+@pytest.fixture(scope="module")
+def grown_text():
+    # a request that grows past 16384 states, so its final H is stored in
+    # the solver's row order
+    return synth.request_input(5, **MIX)
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_products_solver_order(grown_text, form):
+    _say("products, serving-mix request 5, form", form)
+    ctx = vhci_amd.Context(vhci_amd.Problem(text=grown_text), form)
+    ctx.run()
+    assert ctx.n >= 16384
+    H, diag = _stored(ctx)
+    for b, e in _errors(ctx, H, diag, fp32=False).items():
+        assert e < 1e-12, (b, e)
+    if form == vhci_amd.MIXED:
+        for b, e in _errors(ctx, H, diag, fp32=True).items():
+            assert 1e-9 < e < 1e-4, (b, e)
+    ctx.close()
+
+
+def test_products_basis_order():
+    _say("products, ClO2 VCI (basis order)")
+    ctx = vhci_amd.Context(vhci_amd.Problem(os.path.join(GOLDEN, "clo2_vci.inp")))
+    ctx.run()
+    assert ctx.n < 16384
+    H, diag = _stored(ctx)
+    for b, e in _errors(ctx, H, diag, fp32=False).items():
+        assert e < 1e-12, (b, e)
+    ctx.close()
+
+
+def test_products_matrix_free(grown_text, monkeypatch):
+    _say("products, serving-mix request 5, matrix-free against stored")
+    p = vhci_amd.Problem(text=grown_text)
+    ref = vhci_amd.Context(p)
+    r0 = ref.run()
+    H, diag = _stored(ref)
+    ref.close()
+    monkeypatch.setenv("VHCI_MATRIX_FREE", "1")
+    ctx = vhci_amd.Context(p)
+    r1 = ctx.run()
+    assert ctx.matrix_free and r1["checksum"] == r0["checksum"]
+    for b, e in _errors(ctx, H, diag, fp32=False, widths=(1, 3, 8, 16, 40)).items():
+        assert e < 1e-12, (b, e)
+    ctx.close()
+
+
+def test_apply_h_rejects_bad_arguments():
+    ctx = vhci_amd.Context(vhci_amd.Problem(os.path.join(GOLDEN, "clo2_vci.inp")))
+    ctx.run()
+    with pytest.raises(RuntimeError, match="block width"):
+        ctx.apply_h(np.zeros((ctx.n, 65)))
+    with pytest.raises(RuntimeError, match="mixed form only"):
+        ctx.apply_h(np.zeros((ctx.n, 2)), fp32=True)
+    ctx.close()

@@ -63,6 +63,7 @@ _SIGS = {
     "vhci_get_new_states": (ctypes.c_int, [_vp, _i32p]),
     "vhci_append_new_states": (ctypes.c_int, [_vp]),
     "vhci_pt2": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _dp, _i64p]),
+    "vhci_apply_h": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "vhci_time_spmm": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "vhci_copy_probe": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
     "vhci_spmm_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -272,6 +273,16 @@ class Context:
         self._check(L.vhci_get_h(self.h, ctypes.byref(nnz), _ptr(rp, _i64p), _ptr(col, _i32p), _ptr(val, _dp),
                                  _ptr(diag, _dp)))
         return rp, col[: nnz.value], val[: nnz.value], diag
+
+    def apply_h(self, X, fp32=False):
+        """H X for an n x b block (or an n-vector) in basis order, through
+        the solver's product kernels (fp32: the mixed form's FP32 product)."""
+        X = np.asarray(X, dtype=np.float64)
+        vec = X.ndim == 1
+        X = np.ascontiguousarray(X.reshape(self.n, -1))
+        Y = np.zeros_like(X)
+        self._check(lib().vhci_apply_h(self.h, X.shape[1], int(fp32), _ptr(X, _dp), _ptr(Y, _dp)))
+        return Y[:, 0] if vec else Y
 
     def eigsolve(self, nev, tol=1e-10, maxit=500, vectors=True):
         n = self.n

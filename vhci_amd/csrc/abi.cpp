@@ -474,6 +474,13 @@ int vhci_phase_stats(vhci_ctx* ctx, double* out) {
     return guard(ctx, [&] { ctx->eng->phase_stats(out); });
 }
 
+int vhci_apply_h(vhci_ctx* ctx, int b, int fp32, const double* X, double* Y) {
+    return guard(ctx, [&] {
+        if (!X || !Y) throw std::runtime_error("vhci_apply_h: null vector");
+        ctx->eng->apply_h(X, Y, b, fp32 != 0);
+    });
+}
+
 int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
     return guard(ctx, [&] {
         double bytes = 0;

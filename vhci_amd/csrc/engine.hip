@@ -42,6 +42,12 @@ namespace {
 
 constexpr uint64_t kEmpty = ~0ull;
 constexpr int kWave = 64;
+// entries of padding after the last stored entry of col/val/hi/lo, so that a
+// 16-byte load of 4 entries that starts inside the matrix stays in bounds
+constexpr int64_t kCsrPad = 4;
+typedef int32_t sp_i4 __attribute__((ext_vector_type(4)));
+typedef double sp_d2 __attribute__((ext_vector_type(2)));
+typedef float sp_f4 __attribute__((ext_vector_type(4)));
 
 using clk = std::chrono::steady_clock;
 double ms_since(clk::time_point t0) { return std::chrono::duration<double, std::milli>(clk::now() - t0).count(); }
@@ -862,8 +868,33 @@ __global__ __launch_bounds__(256) void k_spmm_lane(int64_t n, const int64_t* __r
     }
 }
 
-// single vector: 32 lanes per row, 8 rows per 256-thread block, four
-// entries per lane in flight
+// entries kk .. kk+3 (kk a multiple of 4) as one or two 16-byte loads
+template <int MODE>
+__device__ __forceinline__ void sp_val4(const double* __restrict__ val, const float* __restrict__ hi,
+                                        const float* __restrict__ lo, int64_t kk,
+                                        typename SpAcc<MODE>::T (&v)[4]) {
+    if constexpr (MODE == 0) {
+        const sp_d2 a = __builtin_nontemporal_load(reinterpret_cast<const sp_d2*>(val + kk));
+        const sp_d2 b = __builtin_nontemporal_load(reinterpret_cast<const sp_d2*>(val + kk + 2));
+        v[0] = a.x, v[1] = a.y, v[2] = b.x, v[3] = b.y;
+    } else if constexpr (MODE == 1) {
+        const sp_f4 a = __builtin_nontemporal_load(reinterpret_cast<const sp_f4*>(hi + kk));
+        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w;
+    } else {
+        const sp_f4 a = __builtin_nontemporal_load(reinterpret_cast<const sp_f4*>(hi + kk));
+        const sp_f4 b = __builtin_nontemporal_load(reinterpret_cast<const sp_f4*>(lo + kk));
+        v[0] = (double)a.x + (double)b.x, v[1] = (double)a.y + (double)b.y;
+        v[2] = (double)a.z + (double)b.z, v[3] = (double)a.w + (double)b.w;
+    }
+}
+
+// single vector: 16 lanes per row, 16 rows per 256-thread block.  Each lane
+// takes groups of 4 consecutive entries (a 16-byte column load and 16-byte
+// value loads), 8 groups in flight.  The groups start at the row's first
+// entry rounded down to a multiple of 4; entries outside the row are loaded
+// (the arrays carry kCsrPad entries of padding) but not used.  The gathers
+// of x bound this kernel, not the streams: one 8-byte element per cache
+// line fetched from L2.
 template <int MODE>
 __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t* __restrict__ rowptr,
                                               const int32_t* __restrict__ col, const double* __restrict__ val,
@@ -871,31 +902,42 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t* __restri
                                               const double* __restrict__ D, const float* __restrict__ x32,
                                               const double* __restrict__ x, double* __restrict__ y) {
     using AT = typename SpAcc<MODE>::T;
-    const int sub = threadIdx.x & 31;
-    const int64_t row = (int64_t)blockIdx.x * 8 + (threadIdx.x >> 5);
+    constexpr int G = 16, U = 8;
+    const int sub = threadIdx.x % G;
+    const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
     const bool live = row < n;
-    AT a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    AT acc = 0;
     if (live) {
-        const int64_t k1 = rowptr[row + 1];
-        int64_t k = rowptr[row] + sub;
-        for (; k + 96 < k1; k += 128) {
-            const int64_t j0 = __builtin_nontemporal_load(col + k);
-            const int64_t j1 = __builtin_nontemporal_load(col + k + 32);
-            const int64_t j2 = __builtin_nontemporal_load(col + k + 64);
-            const int64_t j3 = __builtin_nontemporal_load(col + k + 96);
-            a0 = sp_fma(sp_val<MODE>(val, hi, lo, k), sp_x<MODE>(x, x32, j0, j0), a0);
-            a1 = sp_fma(sp_val<MODE>(val, hi, lo, k + 32), sp_x<MODE>(x, x32, j1, j1), a1);
-            a2 = sp_fma(sp_val<MODE>(val, hi, lo, k + 64), sp_x<MODE>(x, x32, j2, j2), a2);
-            a3 = sp_fma(sp_val<MODE>(val, hi, lo, k + 96), sp_x<MODE>(x, x32, j3, j3), a3);
-        }
-        for (; k < k1; k += 32) {
-            const int64_t j0 = __builtin_nontemporal_load(col + k);
-            a0 = sp_fma(sp_val<MODE>(val, hi, lo, k), sp_x<MODE>(x, x32, j0, j0), a0);
+        const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+        for (int64_t k = (k0 & ~int64_t(3)) + 4 * sub; k < k1; k += 4 * G * U) {
+            sp_i4 c[U];
+            AT v[U][4];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t kk = k + 4 * G * u;
+                if (kk < k1) {
+                    c[u] = __builtin_nontemporal_load(reinterpret_cast<const sp_i4*>(col + kk));
+                    sp_val4<MODE>(val, hi, lo, kk, v[u]);
+                } else {
+                    c[u] = sp_i4{0, 0, 0, 0};
+#pragma unroll
+                    for (int t = 0; t < 4; t++) v[u][t] = 0;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int64_t kk = k + 4 * G * u;
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (kk + t >= k0 && kk + t < k1) {
+                        const int64_t j = c[u][t];
+                        acc = sp_fma(v[u][t], sp_x<MODE>(x, x32, j, j), acc);
+                    }
+            }
         }
     }
-    AT acc = (a0 + a1) + (a2 + a3);
 #pragma unroll
-    for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 32);
+    for (int off = G / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, G);
     if (live && sub == 0) y[row] = D[row] * x[row] + (double)acc;
 }
 
@@ -2325,12 +2367,12 @@ void Engine::build_h_t(HStats& st) {
         st.probed_rows = incr ? n - n_old : n;
         return;
     }
-    col_.ensure(std::max<int64_t>(1, nnz) * sizeof(int32_t));
+    col_.ensure((nnz + kCsrPad) * sizeof(int32_t));
     if (form_ == kFP64) {
-        val_.ensure(std::max<int64_t>(1, nnz) * sizeof(double));
+        val_.ensure((nnz + kCsrPad) * sizeof(double));
     } else {
-        hi_.ensure(std::max<int64_t>(1, nnz) * sizeof(float));
-        lo_.ensure(std::max<int64_t>(1, nnz) * sizeof(float));
+        hi_.ensure((nnz + kCsrPad) * sizeof(float));
+        lo_.ensure((nnz + kCsrPad) * sizeof(float));
     }
     a.rowptr = rowptr_.as<int64_t>();
     a.col = col_.as<int32_t>();
@@ -2618,7 +2660,7 @@ void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low,
     const float* h = hi_.as<float>();
     const float* l = lo_.as<float>();
     if (b == 1) {
-        const unsigned grid = grid_for(n, 8);
+        const unsigned grid = grid_for(n, 16);
         if (mode == 0) k_spmv<0><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
         else if (mode == 1) k_spmv<1><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
         else k_spmv<2><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y);
@@ -2707,6 +2749,37 @@ double Engine::copy_probe(size_t bytes) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return ms;
+}
+
+void Engine::apply_h(const double* X, double* Y, int b, bool fp32) {
+    HIPCHK(hipSetDevice(device_));
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    if (b < 1 || b > 64) throw std::runtime_error("apply_h: the block width must be 1 to 64");
+    if (fp32 && form_ != kMixed) throw std::runtime_error("apply_h: the FP32 product exists in the mixed form only");
+    const int64_t n = n_;
+    const size_t cnt = (size_t)n * b;
+    std::vector<int32_t> perm;
+    if (perm_on_) {
+        perm.resize(n);
+        HIPCHK(hipMemcpy(perm.data(), perm_.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    }
+    // stored row r is basis state perm[r]
+    std::vector<double> xs(cnt), ys(cnt);
+    for (int64_t r = 0; r < n; r++) {
+        const int64_t i = perm_on_ ? perm[r] : r;
+        std::copy(X + i * b, X + (i + 1) * b, xs.begin() + r * b);
+    }
+    DevBuf xd, yd;
+    xd.ensure(cnt * sizeof(double));
+    yd.ensure(cnt * sizeof(double));
+    HIPCHK(hipMemcpyAsync(xd.p, xs.data(), cnt * sizeof(double), hipMemcpyHostToDevice, stream_));
+    spmm(xd.as<double>(), b, yd.as<double>(), b, b, fp32);
+    HIPCHK(hipMemcpyAsync(ys.data(), yd.p, cnt * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
+    for (int64_t r = 0; r < n; r++) {
+        const int64_t i = perm_on_ ? perm[r] : r;
+        std::copy(ys.begin() + r * b, ys.begin() + (r + 1) * b, Y + i * b);
+    }
 }
 
 double Engine::time_spmm(int b, int reps, bool fp32, double* bytes_per_call) {

@@ -137,6 +137,9 @@ class Engine {
 
     hipStream_t stream() const { return stream_; }
     double last_spmm_ms() const { return last_spmm_ms_; }
+    // Y = H X for b host vectors (row-major n x b, basis order) through the
+    // solver's product kernels; fp32 selects the mixed form's FP32 product
+    void apply_h(const double* X, double* Y, int b, bool fp32);
     // time one SpMM of b vectors, averaged over reps (for the roofline figure)
     double time_spmm(int b, int reps, bool fp32, double* bytes_per_call);
     // a plain copy of `bytes` bytes on the context stream; returns ms
