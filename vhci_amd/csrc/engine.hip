@@ -1180,15 +1180,18 @@ __global__ void k_diag_block(double* G, int m, int ld, const double* __restrict_
     if (i < m && j < m) G[i + (int64_t)j * ld] = (i == j) ? theta[i] : 0.0;
 }
 
-// start block: columns [0, k0) from the resident row-major Ritz block (rows
-// beyond xrows are zero), columns [k0, b) unit vectors on idx[]
+// Davidson start block: columns c < k0 from the resident row-major vectors
+// Xr (basis order; stored row r is basis state perm[r] when perm is given),
+// the others unit vectors on the stored rows idx.
 __global__ void k_start_block(int64_t n, int b, int k0, const double* __restrict__ Xr, int64_t xrows, int xcols,
-                              const int64_t* __restrict__ idx, double* __restrict__ V, int64_t ldv) {
+                              const int64_t* __restrict__ idx, double* __restrict__ V, int64_t ldv,
+                              const int32_t* __restrict__ perm) {
     const int c = blockIdx.y;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         double v = 0.0;
+        const int64_t br = perm ? perm[r] : r;
         if (c < k0)
-            v = r < xrows ? Xr[r * xcols + c] : 0.0;
+            v = Xr && br < xrows ? Xr[br * xcols + c] : 0.0;
         else
             v = (idx[c - k0] == r) ? 1.0 : 0.0;
         V[r + c * ldv] = v;
@@ -3247,12 +3250,22 @@ void Engine::davidson(int nev, double tol, int maxit, std::vector<double>& E, Ei
     HIPCHK(hipMemcpyAsync(Dh.data(), D_.p, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
     int k0 = 0;
-    if (xcols_ > 0 && xrows_ > 0 && xrows_ <= n) k0 = std::min(xcols_, b);
+    const bool resident = xcols_ > 0 && xrows_ > 0 && xrows_ <= n;
+    if (resident) k0 = std::min(xcols_, b);
     {
+        std::vector<int32_t> perm;
+        if (perm_on_) {
+            perm.resize(n);
+            HIPCHK(hipMemcpy(perm.data(), perm_.p, n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        }
+        // unit vectors go on the lowest diagonal entries of the states new
+        // since the resident vectors were computed (of all states otherwise)
         std::vector<int64_t> idx(n);
         std::iota(idx.begin(), idx.end(), 0);
-        const int64_t first = (k0 > 0) ? xrows_ : 0;
-        std::vector<int64_t> cand(idx.begin() + first, idx.end());
+        const int64_t first = resident ? xrows_ : 0;
+        std::vector<int64_t> cand;
+        for (int64_t r = 0; r < n; r++)
+            if ((perm_on_ ? perm[r] : r) >= first) cand.push_back(r);
         const int need = b - k0;
         if ((int64_t)cand.size() < need) cand = idx;
         auto cmp = [&](int64_t a, int64_t c) { return Dh[a] < Dh[c] || (Dh[a] == Dh[c] && a < c); };
@@ -3262,8 +3275,8 @@ void Engine::davidson(int nev, double tol, int maxit, std::vector<double>& E, Ei
         di.ensure(cand.size() * sizeof(int64_t));
         HIPCHK(hipMemcpyAsync(di.p, cand.data(), cand.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream_));
         dim3 g(grid_for(n, 256), b);
-        k_start_block<<<g, 256, 0, stream_>>>(n, b, k0, k0 ? X_.as<double>() : nullptr, xrows_, xcols_,
-                                             di.as<int64_t>(), V, ld);
+        k_start_block<<<g, 256, 0, stream_>>>(n, b, k0, resident ? X_.as<double>() : nullptr, xrows_, xcols_,
+                                             di.as<int64_t>(), V, ld, perm_on_ ? perm_.as<int32_t>() : nullptr);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream_));
     }
