@@ -1180,6 +1180,23 @@ __global__ void k_diag_block(double* G, int m, int ld, const double* __restrict_
     if (i < m && j < m) G[i + (int64_t)j * ld] = (i == j) ? theta[i] : 0.0;
 }
 
+// Ritz residual norms from the Krylov-Schur relation: out[j] =
+// || B s_j(last bs rows) ||^2 with B[c * bs + a] = (V_m^T W_last)[a][c] and
+// s_j column j of the eigenvector matrix Gw (ld m)
+__global__ void k_relation_resid(const double* __restrict__ B, int bs, const double* __restrict__ Gw, int m, int nev,
+                                 double* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nev) return;
+    const double* sj = Gw + (int64_t)j * m + (m - bs);
+    double sq = 0.0;
+    for (int a = 0; a < bs; a++) {
+        double v = 0.0;
+        for (int c = 0; c < bs; c++) v += B[c * bs + a] * sj[c];
+        sq += v * v;
+    }
+    out[j] = sq;
+}
+
 // Davidson start block: columns c < k0 from the resident row-major vectors
 // Xr (basis order; stored row r is basis state perm[r] when perm is given),
 // the others unit vectors on the stored rows idx.
@@ -3471,6 +3488,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     int stall = 0;
     for (int it = 0; it < maxit; it++) {
         st.iterations = it + 1;
+        bool refilled = false;
         for (int j = k; j < m; j += bs) {
             // W_j = H V_j, also written to V_(j+bs) as the next block's start
             apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st, V + (int64_t)(j + bs) * ld);
@@ -3488,6 +3506,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
                 tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
             }
             int kept = ortho_cm(V, ld, j + bs, bs, j > k);
+            if (kept < bs) refilled = true;
             for (int tries = 0; kept < bs && tries < 4; tries++) {
                 // invariant subspace reached for some directions: refill
                 random_cm(V + (int64_t)(j + bs + kept) * ld, ld, bs - kept, seed + 977 * (it * m + j + tries + 1));
@@ -3510,25 +3529,45 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
             }
             HIPCHK(hipEventRecord(prof_.gev[0], stream_));
         }
+        // FP64 cycles: W = H V holds column for column, and every block but
+        // the last maps into the basis, so H V = V G + V_m B E^T with
+        // B = V_m^T W_last (V_m: the block after the last, columns m..m+bs).
+        // The Ritz residual H x_i - theta_i x_i is then V_m B s_i(last bs
+        // rows): its norm comes from bs x bs and bs x nev numbers, and H X
+        // is not formed.  The FP32 cycles' products carry FP32 rounding in
+        // every column, so they form H X and measure the residuals directly;
+        // so does a cycle in which a block lost directions and was refilled
+        // at random (the relation then misses what was dropped), with H X
+        // from the product since W's kept columns are not carried in FP64.
+        const bool relation = !low && !refilled;
         BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, V, ld, Gw, m,
                               &zero, X, ld));
-        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, W, ld, Gw, m,
-                              &zero, HX, ld));
+        if (low)
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, W, ld, Gw,
+                                  m, &zero, HX, ld));
         if (gprof) {
             HIPCHK(hipEventRecord(prof_.gev[1], stream_));
             HIPCHK(hipEventSynchronize(prof_.gev[1]));
             float t = 0;
             HIPCHK(hipEventElapsedTime(&t, prof_.gev[0], prof_.gev[1]));
             prof_.gemm_ms += t;
-            prof_.gemm_flops += 2.0 * 2.0 * (double)n * m * keep;
-            prof_.gemm_calls += 2;
+            prof_.gemm_flops += (low ? 2.0 : 1.0) * 2.0 * (double)n * m * keep;
+            prof_.gemm_calls += low ? 2 : 1;
         }
-        dim3 rg(nblk, nev);
-        k_resid_cm<<<rg, 256, 0, stream_>>>(n, HX, X, ld, th, rpb, part_.as<double>());
-        k_sum_parts<<<grid_for(nev, 64), 64, 0, stream_>>>(part_.as<double>(), nblk, nev, work_.as<double>());
+        if (!low && !relation) apply_cm(X, HX, ld, nev, false, st);
+        if (relation) {
+            S_.ensure((size_t)std::max<size_t>(S_.cap / sizeof(double), (size_t)bs * bs) * sizeof(double));
+            tsm_tn(stream_, tsm_part_, n, bs, bs, V + (int64_t)m * ld, ld, W + (int64_t)(m - bs) * ld, ld,
+                   S_.as<double>(), bs);
+            k_relation_resid<<<grid_for(nev, 64), 64, 0, stream_>>>(S_.as<double>(), bs, Gw, m, nev, work_.as<double>());
+        } else {
+            dim3 rg(nblk, nev);
+            k_resid_cm<<<rg, 256, 0, stream_>>>(n, HX, X, ld, th, rpb, part_.as<double>());
+            k_sum_parts<<<grid_for(nev, 64), 64, 0, stream_>>>(part_.as<double>(), nblk, nev, work_.as<double>());
+        }
         HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(thh.data(), th, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipMemcpyAsync(rn.data(), work_.p, nev * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(thh.data(), th, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
         double maxrel = 0;
         int unconv = 0;
@@ -3589,7 +3628,10 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
             k = 0;
             continue;
         } else {
-            HIPCHK(hipMemcpyAsync(W, HX, (size_t)n * keep * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+            // W[:, :keep] = H X is read only by the next cycle's H X product,
+            // which FP64 cycles do not form
+            if (low)
+                HIPCHK(hipMemcpyAsync(W, HX, (size_t)n * keep * sizeof(double), hipMemcpyDeviceToDevice, stream_));
             dim3 dg(grid_for(keep, 64), keep);
             k_diag_block<<<dg, 64, 0, stream_>>>(G, keep, m, th);
         }
