@@ -90,7 +90,8 @@ FP64_PEAK_TFLOPS = 78.6
 def phase_rooflines(ph, sp):
     """Algorithmic-bytes bandwidth of the H build, the PT2 selection and the
     PT2 sum of the last timed solve (wall times of the phases, syncs
-    included), and the Ritz rotation GEMMs' FP64 rate."""
+    included), and the FP64 rate of the Rayleigh-Ritz GEMMs (the projection
+    V^T W and the rotation X = V S, rocBLAS MFMA kernels)."""
     K = ph["key_bytes"]
     n, nnz, P = ph["n"], ph["nnz"], ph["patterns"]
     sv = 8.0  # FP64 value, or FP32 hi + FP32 lo

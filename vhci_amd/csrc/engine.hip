@@ -3532,6 +3532,16 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         // shape; split into 8192-row chunks of one batched product (MFMA
         // tiles) with the partial products summed in a fixed order, it is a
         // little faster than the vector-ALU tall-skinny kernel.
+        // GEMM timing (bench profiling): events around the projection and
+        // the rotation, read after the rotation's
+        const bool gprof = prof_.b > 0;
+        if (gprof) {
+            if (prof_.gev.empty()) {
+                prof_.gev.resize(4);
+                for (auto& e : prof_.gev) HIPCHK(hipEventCreate(&e));
+            }
+            HIPCHK(hipEventRecord(prof_.gev[2], stream_));
+        }
         {
             const int64_t chunk = 8192;
             const int nfull = (int)(n / chunk);
@@ -3552,17 +3562,11 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
             k_sum_batches<<<grid_for((int64_t)m * q, 256), 256, 0, stream_>>>(pp, nb, m, q, G + (int64_t)k * m, m);
             HIPCHK(hipGetLastError());
         }
+        if (gprof) HIPCHK(hipEventRecord(prof_.gev[3], stream_));
         HIPCHK(hipMemcpyAsync(Gw, G, (size_t)m * m * sizeof(double), hipMemcpyDeviceToDevice, stream_));
         BLASCHK(rocsolver_dsyevd(blas_, rocblas_evect_original, rocblas_fill_upper, m, Gw, m, th, work_.as<double>(),
                                  info_.as<rocblas_int>()));
-        const bool gprof = prof_.b > 0;
-        if (gprof) {
-            if (prof_.gev.empty()) {
-                prof_.gev.resize(2);
-                for (auto& e : prof_.gev) HIPCHK(hipEventCreate(&e));
-            }
-            HIPCHK(hipEventRecord(prof_.gev[0], stream_));
-        }
+        if (gprof) HIPCHK(hipEventRecord(prof_.gev[0], stream_));
         // FP64 cycles: W = H V holds column for column, and every block but
         // the last maps into the basis, so H V = V G + V_m B E^T with
         // B = V_m^T W_last (V_m: the block after the last, columns m..m+bs).
@@ -3582,11 +3586,12 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         if (gprof) {
             HIPCHK(hipEventRecord(prof_.gev[1], stream_));
             HIPCHK(hipEventSynchronize(prof_.gev[1]));
-            float t = 0;
+            float t = 0, tp = 0;
             HIPCHK(hipEventElapsedTime(&t, prof_.gev[0], prof_.gev[1]));
-            prof_.gemm_ms += t;
-            prof_.gemm_flops += (low ? 2.0 : 1.0) * 2.0 * (double)n * m * keep;
-            prof_.gemm_calls += low ? 2 : 1;
+            HIPCHK(hipEventElapsedTime(&tp, prof_.gev[2], prof_.gev[3]));
+            prof_.gemm_ms += t + tp;
+            prof_.gemm_flops += (low ? 2.0 : 1.0) * 2.0 * (double)n * m * keep + 2.0 * (double)n * m * (m - k);
+            prof_.gemm_calls += (low ? 2 : 1) + 1;
         }
         if (!low && !relation) apply_cm(X, HX, ld, nev, false, st);
         if (relation) {
