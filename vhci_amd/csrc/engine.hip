@@ -3404,15 +3404,17 @@ void Engine::davidson(int nev, double tol, int maxit, std::vector<double>& E, Ei
 }
 
 // Block Krylov-Schur (thick-restart block Lanczos with full
-// re-orthogonalisation).  V holds m + bs column-major basis vectors, W = H V
-// for the first m; after each cycle the Rayleigh-Ritz pairs of G = V^T W are
-// tested (||H x - theta x|| < tol max(eps^(2/3), |theta|) for the nev
-// lowest), and the space restarts from the keep lowest Ritz vectors plus the
-// last residual block, which is orthogonal to all of them.  The start block
-// mixes the resident Ritz vectors of the previous basis (zero-padded) with a
-// small random component.  Mixed form: the cycles run with the FP32 product
-// until the residuals stop improving; then W of the kept Ritz vectors is
-// recomputed in FP64 and the cycles continue in FP64.
+// re-orthogonalisation).  V holds m + bs column-major basis vectors and W =
+// H V for the columns formed in the current cycle (FP32 cycles also carry
+// W = H X for the kept Ritz vectors); after each cycle the Rayleigh-Ritz
+// pairs of G = V^T W are tested (||H x - theta x|| < tol max(eps^(2/3),
+// |theta|) for the nev lowest), and the space restarts from the keep lowest
+// Ritz vectors plus the last residual block, which is orthogonal to all of
+// them.  The start block mixes the resident Ritz vectors of the previous
+// basis (zero-padded) with a small random component.  Mixed form: the cycles
+// run with the FP32 product until the residuals stop improving; then the
+// FP64 cycles start afresh from random combinations of the kept Ritz
+// vectors.
 void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st) {
     const int64_t n = n_;
     const int ni = (int)n;
