@@ -98,6 +98,12 @@ def test_products_matrix_free(grown_text, monkeypatch):
     for b, e in _errors(ctx, H, diag, fp32=False, widths=(1, 3, 8, 16, 40)).items():
         assert e < 1e-12, (b, e)
     ctx.close()
+    ctx = vhci_amd.Context(p, vhci_amd.MIXED)
+    ctx.run()
+    assert ctx.matrix_free
+    with pytest.raises(RuntimeError, match="FP64 only"):
+        ctx.apply_h(np.zeros((ctx.n, 2)), fp32=True)
+    ctx.close()
 
 
 def test_apply_h_rejects_bad_arguments():

@@ -2788,6 +2788,7 @@ void Engine::apply_h(const double* X, double* Y, int b, bool fp32) {
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
     if (b < 1 || b > 64) throw std::runtime_error("apply_h: the block width must be 1 to 64");
     if (fp32 && form_ != kMixed) throw std::runtime_error("apply_h: the FP32 product exists in the mixed form only");
+    if (fp32 && matfree_) throw std::runtime_error("apply_h: matrix-free products are FP64 only");
     const int64_t n = n_;
     const size_t cnt = (size_t)n * b;
     std::vector<int32_t> perm;
@@ -3531,9 +3532,11 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         }
         // G[:, k:m] = V[:, :m]^T W[:, k:m] (upper triangle of the new columns).
         // One rocBLAS product over all n rows runs at 2.3 TFLOP/s for this
-        // shape; split into 8192-row chunks of one batched product (MFMA
-        // tiles) with the partial products summed in a fixed order, it is a
-        // little faster than the vector-ALU tall-skinny kernel.
+        // shape; split into row chunks of one batched product (MFMA tiles)
+        // with the partial products summed in a fixed order, it is a little
+        // faster than the vector-ALU tall-skinny kernel.  Chunks of 8192
+        // rows, longer for bases beyond 2^21 states so that at most 256
+        // partial products are held.
         // GEMM timing (bench profiling): events around the projection and
         // the rotation, read after the rotation's
         const bool gprof = prof_.b > 0;
@@ -3545,7 +3548,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
             HIPCHK(hipEventRecord(prof_.gev[2], stream_));
         }
         {
-            const int64_t chunk = 8192;
+            const int64_t chunk = std::max<int64_t>(8192, (n + 255) / 256);
             const int nfull = (int)(n / chunk);
             const int64_t tail = n - (int64_t)nfull * chunk;
             const int q = m - k;
