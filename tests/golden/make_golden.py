@@ -5,7 +5,9 @@ are the example runs under examples/ (input, expected freqs.dat, terminal log)
 and the force fields under input/.  This script copies the inputs and the
 expected freqs.dat files verbatim and extracts the integer pins of the logs
 (initial basis, basis size per heat-bath iteration, PT2 space size) into
-manifest.json.  It reads the reference checkout, so it runs only where that
+manifest.json.  It also copies the published runs under results/ (the
+freqs.dat of each force field at the eps1, Nstate, PT2 and eps2 that its
+file name states, total quanta 2).  It reads the reference checkout, so it runs only where that
 checkout exists; the tests use the committed copies.
 """
 import json
@@ -60,6 +62,11 @@ def main():
     os.makedirs(os.path.join(OUT, "fields"), exist_ok=True)
     for f in FIELDS:
         shutil.copyfile(os.path.join(REF, "input", f), os.path.join(OUT, "fields", f))
+    # the published runs on the force fields above (freqs.dat of each)
+    os.makedirs(os.path.join(OUT, "results"), exist_ok=True)
+    for f in sorted(os.listdir(os.path.join(REF, "results"))):
+        if f.endswith(".dat"):
+            shutil.copyfile(os.path.join(REF, "results", f), os.path.join(OUT, "results", f))
     with open(os.path.join(OUT, "manifest.json"), "w") as f:
         json.dump(manifest, f, indent=1, sort_keys=True)
 
