@@ -22,6 +22,10 @@ precision form, with p50/p99 latency from submission to result and a check
 that every result equals its solo run; then the capacity: all requests at
 once through four- and eight-slot FP64 servers.
 
+The reference-runs entry solves the inputs of the reference's own logged
+runs (ClO2 and acetonitrile examples, the ethylene oxide run) and reports
+each whole-program time beside the wall time its log prints.
+
 The roofline entry is for the FP64 eigensolver's block SpMM (k_spmm_lane),
 every launch of the timed solves bracketed by HIP events on the context's
 stream; with --traffic its HBM bytes per launch come from two rocprofv3
@@ -136,7 +140,7 @@ def traffic_pass(counters, kernel, timeout_s=170):
     cmd = ["rocprofv3", "--pmc"] + counters + ["--output-format", "csv", "-d", d, "-o", "pmc", "--",
                                                sys.executable, "-u", os.path.abspath(__file__), "--steps", "1",
                                                "--warmup", "0", "--forms", "fp64", "--no-cpu-baseline",
-                                               "--serve", "0"]
+                                               "--serve", "0", "--no-reference-runs"]
     print("[bench] counter pass: %s" % " ".join(cmd), file=sys.stderr, flush=True)
     log = os.path.join(d, "pass.log")
     with open(log, "w") as lf:
@@ -324,6 +328,49 @@ def serving(nreq, slots, load, seed=7):
     return out
 
 
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+# The reference's own logged runs (8 OpenMP threads; the CPU is not stated):
+# (workload, input under tests/golden, header values for a force field or
+# None, wall seconds the log prints, where the log says it).
+REFERENCE_RUNS = [
+    ("ClO2 conventional VCI (N = 3276, 20 states)", "clo2_vci.inp", None, 2.00,
+     "examples/CLO2/VCI/output.log:38"),
+    ("acetonitrile conventional VCI (N = 50,388, 70 states)", "acn_vci.inp", None, 2.60 * 60,
+     "examples/Acetonitrile/VCI/output.log:38"),
+    ("acetonitrile VHCI, eps1 1, 70 states", "acn_vhci.inp", None, 3.05 * 60,
+     "examples/Acetonitrile/VHCI/output.log:66"),
+    ("acetonitrile VHCI + PT2, eps1 1, eps2 0, 70 states", "acn_pt2.inp", None, 28.92 * 60,
+     "examples/Acetonitrile/VHCI+PT2/output.log:74"),
+    ("ethylene oxide VHCI + PT2, eps1 1, eps2 0.01, 200 states", "fields/ethylene_oxide.inp",
+     ("1", "200", "1", "0.01", "2"), 6.80 * 3600, "the vim swap file of its run log, .ethylene_oxide_eps1-1.log.swp"),
+]
+
+
+def reference_runs():
+    """The reference's logged runs solved here, whole program (input parse,
+    context, heat-bath loop, PT2), FP64, against the wall time each log
+    prints."""
+    out = []
+    for name, rel, header, ref_s, where in REFERENCE_RUNS:
+        text = open(os.path.join(GOLDEN, rel)).read()
+        if header:
+            lines = text.split("\n")
+            # header lines 2-6: HCI_Eps, Nstate, PT2, PT2_Eps, Max_quanta
+            for i, v in enumerate(header):
+                lines[1 + i] = lines[1 + i].split()[0] + " " + v
+            text = "\n".join(lines)
+        t0 = time.perf_counter()
+        ctx = vhci_amd.Context(vhci_amd.Problem(text=text), vhci_amd.FP64)
+        r = ctx.run()
+        t = time.perf_counter() - t0
+        ctx.close()
+        out.append({"workload": name, "seconds": t, "reference_seconds": ref_s, "speedup": ref_s / t,
+                    "reference_log": where, "nvar": r["nvar"], "ns2": r["ns2"]})
+        print("[bench] %s: %.2f s (reference log %.0f s, %.0fx)" % (name, t, ref_s, ref_s / t), file=sys.stderr,
+              flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -334,6 +381,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--serve", type=int, default=32, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--no-reference-runs", action="store_true",
+                    help="skip the reference's logged runs (acetonitrile, ethylene oxide, ClO2)")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=1.2, help="offered load / solo FP64 rate")
     ap.add_argument("--traffic", action="store_true",
@@ -438,6 +487,8 @@ def main():
                        "GBs": 2.0 * (1 << 30) / cms / 1e6}
     if args.serve > 0:
         out["serving"] = serving(args.serve, args.serve_slots, args.serve_load)
+    if not args.no_reference_runs:
+        out["reference_runs"] = reference_runs()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,
                                            args.cpu_threads)
