@@ -24,7 +24,9 @@ once through four- and eight-slot FP64 servers.
 
 The reference-runs entry solves the inputs of the reference's own logged
 runs (ClO2 and acetonitrile examples, the ethylene oxide run) and reports
-each whole-program time beside the wall time its log prints.
+each whole-program time beside the wall time its log prints; the
+large-field entry runs the 48-mode naphthalene field at the large
+workload's parameters (a few million states).
 
 The roofline entry is for the FP64 eigensolver's block SpMM (k_spmm_lane),
 every launch of the timed solves bracketed by HIP events on the context's
@@ -371,6 +373,33 @@ def reference_runs():
     return out
 
 
+def large_field_run():
+    """The 48-mode naphthalene field at the large workload's parameters
+    (eps1 1, Nstate 128, PT2 with eps2 0.2, total quanta 2): one FP64 solve
+    with the block SpMM launches timed, for the behaviour at a few million
+    states (the reference's published output of this run is a parity
+    fixture of the tests)."""
+    lines = open(os.path.join(GOLDEN, "fields", "naphthalene.inp")).read().split("\n")
+    for i, v in enumerate(("1", "128", "1", "0.2", "2")):
+        lines[1 + i] = lines[1 + i].split()[0] + " " + v
+    t0 = time.perf_counter()
+    ctx = vhci_amd.Context(vhci_amd.Problem(text="\n".join(lines)), vhci_amd.FP64)
+    ctx.spmm_profile(8)
+    r = ctx.run()
+    t = time.perf_counter() - t0
+    ms, by, nl = ctx.spmm_profile_read()
+    ph = ctx.phase_stats()
+    ctx.close()
+    out = {"workload": "naphthalene (48 modes) VHCI + PT2, eps1 1, Nstate 128, eps2 0.2", "seconds": t,
+           "nvar": r["nvar"], "ns2": r["ns2"], "stored_entries": ph["nnz"], "timings_ms": r["timings"],
+           "spmm": {"kernel": spmm_kernel_name(8, 0), "launches": nl, "avg_launch_ms": ms / max(nl, 1),
+                    "GBs": by / ms / 1e6 if ms > 0 else None,
+                    "frac_of_hbm": by / ms / 1e6 / HBM_PEAK_GBS if ms > 0 else None}}
+    print("[bench] %s: %.2f s, %d states, %d PT2 states, %.3g entries, SpMM %.0f GB/s" %
+          (out["workload"], t, r["nvar"], r["ns2"], ph["nnz"], out["spmm"]["GBs"] or 0), file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -489,6 +518,7 @@ def main():
         out["serving"] = serving(args.serve, args.serve_slots, args.serve_load)
     if not args.no_reference_runs:
         out["reference_runs"] = reference_runs()
+        out["large_field"] = large_field_run()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,
                                            args.cpu_threads)

@@ -778,13 +778,15 @@ __device__ __forceinline__ void sp_piece(const double* __restrict__ X, const flo
     }
 }
 
-// lane (q = lane % P, e = lane / P) sums entries k0 + e, k0 + e + E, ... of
-// its row against piece q; after the reduction lanes 0..P-1 hold the row's
-// products for columns q*W .. q*W+W-1.  The column and value streams are
-// read once, one entry per lane (fully coalesced), in chunks of 64 entries
-// (C chunks in flight), and handed to the P lanes that gather each entry's
-// row by shuffles.
-template <int B, int MODE>
+// A row is summed by a group of G lanes (the wave, or half of it for short
+// rows).  Lane (q = lane % P, e = lane / P) of the group sums entries
+// k0 + e, k0 + e + E, ... (E = G / P) of its row against piece q; after the
+// reduction lanes 0..P-1 of the group hold the row's products for columns
+// q*W .. q*W+W-1.  The column and value streams are read once, one entry
+// per lane (fully coalesced), in chunks of G entries (C chunks in flight),
+// and handed to the P lanes that gather each entry's row by shuffles within
+// the group.
+template <int B, int MODE, int G>
 __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, const int32_t* __restrict__ col,
                                             const double* __restrict__ val, const float* __restrict__ hi,
                                             const float* __restrict__ lo, const float* __restrict__ X32,
@@ -792,19 +794,19 @@ __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, co
                                             typename SpAcc<MODE>::T (&acc)[SpCoop<B, MODE>::W]) {
     using AT = typename SpAcc<MODE>::T;
     using Cp = SpCoop<B, MODE>;
-    constexpr int W = Cp::W, P = Cp::P, E = Cp::E;
-    constexpr int C = P <= 4 ? 4 : 2;  // chunks of 64 entries in flight (fewer for wide rows: registers)
+    constexpr int W = Cp::W, P = Cp::P, E = G / P;
+    constexpr int C = P <= 4 ? 4 : 2;  // chunks of G entries in flight (fewer for wide rows: registers)
     const int q = lane % P, e = lane / P;
 #pragma unroll
     for (int c = 0; c < W; c++) acc[c] = 0;
     int64_t k = k0;
-    for (; k + 64 * C <= k1; k += 64 * C) {
+    for (; k + G * C <= k1; k += G * C) {
         int32_t jl[C];
         AT vl[C];
 #pragma unroll
         for (int c = 0; c < C; c++) {
-            jl[c] = __builtin_nontemporal_load(col + k + 64 * c + lane);
-            vl[c] = sp_val<MODE>(val, hi, lo, k + 64 * c + lane);
+            jl[c] = __builtin_nontemporal_load(col + k + G * c + lane);
+            vl[c] = sp_val<MODE>(val, hi, lo, k + G * c + lane);
         }
         AT v[C][P];
         AT x[C][P][W];
@@ -812,8 +814,8 @@ __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, co
         for (int c = 0; c < C; c++)
 #pragma unroll
             for (int u = 0; u < P; u++) {
-                const int32_t j = __shfl(jl[c], u * E + e);
-                v[c][u] = __shfl(vl[c], u * E + e);
+                const int32_t j = __shfl(jl[c], u * E + e, G);
+                v[c][u] = __shfl(vl[c], u * E + e, G);
                 sp_piece<B, MODE>(X, X32, j, q, x[c][u]);
             }
 #pragma unroll
@@ -823,15 +825,15 @@ __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, co
 #pragma unroll
                 for (int w = 0; w < W; w++) acc[w] = sp_fma(v[c][u], x[c][u][w], acc[w]);
     }
-    // the rest of the row: 64-entry chunks, masked, handed out the same way
-    for (; k < k1; k += 64) {
+    // the rest of the row: G-entry chunks, masked, handed out the same way
+    for (; k < k1; k += G) {
         const bool in = k + lane < k1;
         const int32_t jl = in ? __builtin_nontemporal_load(col + k + lane) : 0;
         const AT vl = in ? sp_val<MODE>(val, hi, lo, k + lane) : AT(0);
 #pragma unroll
         for (int u = 0; u < P; u++) {
-            const int32_t j = __shfl(jl, u * E + e);
-            const AT v = __shfl(vl, u * E + e);
+            const int32_t j = __shfl(jl, u * E + e, G);
+            const AT v = __shfl(vl, u * E + e, G);
             if (k + u * E + e < k1) {
                 AT x[W];
                 sp_piece<B, MODE>(X, X32, j, q, x);
@@ -841,23 +843,23 @@ __device__ __forceinline__ void sp_coop_dot(int64_t k0, int64_t k1, int lane, co
         }
     }
 #pragma unroll
-    for (int off = P; off < 64; off <<= 1)
+    for (int off = P; off < G; off <<= 1)
 #pragma unroll
-        for (int w = 0; w < W; w++) acc[w] += __shfl_xor(acc[w], off);
+        for (int w = 0; w < W; w++) acc[w] += __shfl_xor(acc[w], off, G);
 }
 
-template <int B, int MODE>
+template <int B, int MODE, int G = 64>
 __global__ __launch_bounds__(256) void k_spmm_lane(int64_t n, const int64_t* __restrict__ rowptr,
                                                    const int32_t* __restrict__ col, const double* __restrict__ val,
                                                    const float* __restrict__ hi, const float* __restrict__ lo,
                                                    const double* __restrict__ D, const float* __restrict__ X32,
                                                    const double* __restrict__ X, double* __restrict__ Y, int ldy) {
-    const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
     if (row >= n) return;
     using C = SpCoop<B, MODE>;
     typename SpAcc<MODE>::T acc[C::W];
-    sp_coop_dot<B, MODE>(rowptr[row], rowptr[row + 1], lane, col, val, hi, lo, X32, X, acc);
+    sp_coop_dot<B, MODE, G>(rowptr[row], rowptr[row + 1], lane, col, val, hi, lo, X32, X, acc);
     if (lane < C::P) {
         const double d = D[row];
 #pragma unroll
@@ -2725,8 +2727,21 @@ void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low,
     const bool dense_x = ldx == b && ((uintptr_t)X & 15) == 0 &&
                          (mode != 1 || (ldx32 == b && ((uintptr_t)X32 & 15) == 0));
     if (dense_x && (b == 4 || b == 8 || b == 16)) {
+        // rows of fewer than 128 entries on average go two to a wave
+        bool half = nnz_ < 128 * n;
+        if (const char* e = std::getenv("VHCI_SPMM_ROW_LANES")) half = std::atoi(e) == 32;
         auto go = [&](auto tag) {
             constexpr int B = decltype(tag)::value;
+            if (half) {
+                const unsigned g2 = grid_for(n, 8);
+                if (mode == 0)
+                    k_spmm_lane<B, 0, 32><<<g2, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
+                else if (mode == 1)
+                    k_spmm_lane<B, 1, 32><<<g2, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
+                else
+                    k_spmm_lane<B, 2, 32><<<g2, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
+                return;
+            }
             if (mode == 0) k_spmm_lane<B, 0><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
             else if (mode == 1) k_spmm_lane<B, 1><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
             else k_spmm_lane<B, 2><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy);
