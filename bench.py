@@ -123,8 +123,10 @@ def phase_rooflines(ph, sp):
     return out
 
 
-def spmm_kernel_name(bs, mode):
-    return "k_spmm_lane<%d, %d>" % (bs, mode)
+def spmm_kernel_name(bs, mode, lanes=64):
+    """The block SpMM kernel as profilers name it (lanes per row: 64, or 32
+    for matrices whose rows average fewer than 128 entries)."""
+    return "k_spmm_lane<%d, %d, %d>" % (bs, mode, lanes)
 
 
 def traffic_pass(counters, kernel, timeout_s=170):
@@ -392,7 +394,7 @@ def large_field_run():
     ctx.close()
     out = {"workload": "naphthalene (48 modes) VHCI + PT2, eps1 1, Nstate 128, eps2 0.2", "seconds": t,
            "nvar": r["nvar"], "ns2": r["ns2"], "stored_entries": ph["nnz"], "timings_ms": r["timings"],
-           "spmm": {"kernel": spmm_kernel_name(8, 0), "launches": nl, "avg_launch_ms": ms / max(nl, 1),
+           "spmm": {"kernel": spmm_kernel_name(8, 0, 32 if ph["nnz"] < 128 * ph["n"] else 64), "launches": nl, "avg_launch_ms": ms / max(nl, 1),
                     "GBs": by / ms / 1e6 if ms > 0 else None,
                     "frac_of_hbm": by / ms / 1e6 / HBM_PEAK_GBS if ms > 0 else None}}
     print("[bench] %s: %.2f s, %d states, %d PT2 states, %.3g entries, SpMM %.0f GB/s" %
