@@ -420,6 +420,8 @@ def main():
                     help="measure the SpMM kernel's HBM traffic (the default; rocprofv3 counter passes in child "
                          "processes, run before this process uses the GPU)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the counter passes (traffic is then null)")
+    ap.add_argument("--large-field-only", action="store_true",
+                    help="only the 48-mode naphthalene run at the large workload's parameters (one JSON line)")
     ap.add_argument("--profile-spmm", action="store_true",
                     help="only build the final H of the workload (FP64), then time --spmm-reps block SpMMs and "
                          "one 1 GiB calibration copy")
@@ -428,6 +430,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.large_field_only:
+        print(json.dumps(large_field_run()), flush=True)
+        return
     text = synth.config_input(1)
     problem = vhci_amd.Problem(text=text)
     bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
