@@ -200,3 +200,18 @@ def test_checkpoint_errors(ck_problem, tmp_path):
     P.write_checkpoint(str(tmp_path / "bb.ck"), q, E, C, binary=True)
     bb = open(str(tmp_path / "bb.ck"), "rb").read()
     bad("bt.ck", bb[: len(bb) - 8], "truncated")
+
+
+def test_published_run_inputs():
+    """Each published run's input (the committed force field with the header
+    its file name states) parses with that eps1, Nstate, PT2 flag and eps2,
+    total quanta 2, and an initial basis of C(M + 2, 2) states."""
+    from test_gpu_results import CASES, NAME, _input_text
+
+    assert len(CASES) == 14
+    for name in CASES:
+        mol, eps1, nstate, pt2, eps2 = NAME.match(name).groups()
+        p = vhci_amd.Problem(text=_input_text(name))
+        assert p.eps1 == float(eps1) and p.neig == int(nstate) and p.perturb == (pt2 == "1")
+        assert p.eps2 == (float(eps2) if pt2 == "1" else 0.0)
+        assert p.ntot == 2 and p.n0 == math.comb(p.M + 2, 2)
