@@ -31,8 +31,15 @@ std::string sparse_line(const vhci::HStats& h) {
 
 void save_checkpoint(Engine& eng, const Problem& P, const std::vector<double>& E, const RunOpts& o,
                      const Logger& log) {
+    // The reference opens the file before it tests for the name "none"
+    // (src/HB.cpp:253-256), which leaves an empty file called
+    // "none" in the working directory; the name is tested first here.
+    if (o.cpname == "none") {
+        log("\n Error: checkpoint file could not be found or no name was given.\n\n");
+        return;
+    }
     std::ofstream cf(o.cpname, std::ios::out | std::ios::trunc | std::ios::binary);
-    if (!cf.good() || o.cpname == "none") {
+    if (!cf.good()) {
         log("\n Error: checkpoint file could not be found or no name was given.\n\n");
         return;
     }
