@@ -196,6 +196,13 @@ int vhci_gemm_profile_read(vhci_ctx* ctx, double* out);
  * last H build probed (all, or only the new states' rows of an incremental
  * build); entries it carried over from the previous build. */
 int vhci_phase_stats(vhci_ctx* ctx, double* out);
+/* Device-memory pool of the context, out[5]: hipMalloc calls, hipFree
+ * calls, blocks handed out again from the idle set, bytes held (in use +
+ * idle), idle bytes.  Blocks return to the device only at heat-bath
+ * iteration boundaries (vhci_append_new_states, vhci_set_basis), on a
+ * failed allocation and at vhci_ctx_destroy.  Measurement support; no
+ * counterpart in the reference. */
+int vhci_pool_stats(vhci_ctx* ctx, int64_t* out);
 
 /* ---- whole solve (Perform_HCI + DoPT2 of one input) --------------------
  * Runs the reference's control flow (src/VHCI.cpp:39-114) on a device

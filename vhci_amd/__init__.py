@@ -70,6 +70,7 @@ _SIGS = {
     "vhci_spmm_profile_read": (ctypes.c_int, [_vp, _dp]),
     "vhci_gemm_profile_read": (ctypes.c_int, [_vp, _dp]),
     "vhci_phase_stats": (ctypes.c_int, [_vp, _dp]),
+    "vhci_pool_stats": (ctypes.c_int, [_vp, _i64p]),
     "vhci_solve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, _dp, _ip, _i64p, _i64p, _dp]),
     "vhci_run_ctx": (ctypes.c_int, [_vp, ctypes.c_double, ctypes.c_int, _dp, _dp, _ip, _i64p, _dp]),
     "vhci_iteration_sizes": (ctypes.c_int, [_vp, _i64p, ctypes.c_int]),
@@ -399,6 +400,13 @@ class Context:
         keys = ["n", "nnz", "patterns", "build_ms", "hb_candidates", "hb_unique", "hb_ms", "ns2", "pairs",
                 "pt2_sum_ms", "key_bytes", "nstate", "probed_rows", "carried"]
         return dict(zip(keys, out.tolist()))
+
+    def pool_stats(self):
+        """device-memory pool of the context: hipMalloc calls, hipFree calls,
+        idle blocks handed out again, bytes held, idle bytes"""
+        out = np.zeros(5, dtype=np.int64)
+        self._check(lib().vhci_pool_stats(self.h, _ptr(out, _i64p)))
+        return dict(zip(["mallocs", "frees", "reuses", "held", "idle"], out.tolist()))
 
     def copy_probe(self, nbytes):
         ms = ctypes.c_double()

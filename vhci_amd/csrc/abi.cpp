@@ -474,6 +474,11 @@ int vhci_phase_stats(vhci_ctx* ctx, double* out) {
     return guard(ctx, [&] { ctx->eng->phase_stats(out); });
 }
 
+int vhci_pool_stats(vhci_ctx* ctx, int64_t* out) {
+    if (!out) return VHCI_E_ARG;
+    return guard(ctx, [&] { ctx->eng->pool_stats(out); });
+}
+
 int vhci_apply_h(vhci_ctx* ctx, int b, int fp32, const double* X, double* Y) {
     return guard(ctx, [&] {
         if (!X || !Y) throw std::runtime_error("vhci_apply_h: null vector");
@@ -745,6 +750,7 @@ static int main_body(int argc, char** argv) {
     RunOpts o;
     if (const char* t = std::getenv("VHCI_TOL")) o.tol = std::atof(t);
     if (const char* t = std::getenv("VHCI_STOP_AFTER_ITERATION")) o.stop_after = std::atoi(t);
+    if (const char* t = std::getenv("VHCI_MAX_STATES")) o.max_states = std::atoll(t);
     if (const char* t = std::getenv("VHCI_CHECKPOINT_FORMAT")) o.binary_checkpoint = std::string(t) == "binary";
     o.threads = ncpus;
     o.cpname = cpname;

@@ -129,6 +129,10 @@ void run(Engine& eng, const Problem& P, const RunOpts& o, RunResult& R, const Lo
                     std::to_string(n) + " product states.\n");
                 break;
             }
+            if (o.max_states > 0 && n + eng.new_count() > o.max_states)
+                throw std::runtime_error("heat-bath iteration " + std::to_string(iter) + " would grow the basis to " +
+                                         std::to_string(n + eng.new_count()) + " states, beyond the limit of " +
+                                         std::to_string(o.max_states) + " (VHCI_MAX_STATES)");
             eng.append_new_states();
             R.sizes.push_back(eng.n());
             if (o.on_grow) o.on_grow(eng.n());

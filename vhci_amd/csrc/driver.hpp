@@ -22,6 +22,9 @@ struct RunOpts {
     int stop_after = 0;  // failure injection: end the run after this HCI iteration
     int threads = 1;     // host threads for checkpoint formatting (-n)
     bool binary_checkpoint = false;  // opt-in binary checkpoint form
+    // end the run with an error when a heat-bath iteration would grow the
+    // basis beyond this many states (0: no limit)
+    int64_t max_states = 0;
     // called after every basis growth with the new basis size (serving
     // loop: memory reservation updates)
     std::function<void(int64_t)> on_grow;

@@ -2096,11 +2096,84 @@ void* HostBuf::ensure(size_t bytes) {
     return p;
 }
 
+namespace {
+thread_local DevPool* t_pool = nullptr;
+
+// capacity class of a request: multiples of 256 B below 64 KiB, above that
+// eight classes per power of two (at most 12.5 % over the request)
+size_t size_class(size_t bytes) {
+    bytes = std::max<size_t>(bytes, 256);
+    if (bytes <= (size_t(1) << 16)) return (bytes + 255) & ~size_t(255);
+    int top = 63 - __builtin_clzll((unsigned long long)bytes);
+    const size_t step = size_t(1) << (top - 3);
+    return (bytes + step - 1) & ~(step - 1);
+}
+}  // namespace
+
+DevPool* current_pool() { return t_pool; }
+PoolScope::PoolScope(DevPool* p) : prev(t_pool) { t_pool = p; }
+PoolScope::~PoolScope() { t_pool = prev; }
+
+void* DevPool::take(size_t bytes, size_t& cap) {
+    const size_t want = size_class(bytes);
+    auto it = idle.lower_bound(want);
+    if (it != idle.end() && it->first <= 2 * want) {
+        void* p = it->second;
+        cap = it->first;
+        idle_bytes -= (int64_t)cap;
+        idle.erase(it);
+        reuses++;
+        return p;
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess && !idle.empty()) {
+        (void)hipGetLastError();
+        trim();
+        e = hipMalloc(&p, want);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw std::runtime_error("device allocation of " + std::to_string(want) + " bytes failed: " +
+                                 hipGetErrorString(e));
+    }
+    mallocs++;
+    cap = want;
+    held += (int64_t)want;
+    t_dev_bytes += (int64_t)want;
+    t_dev_peak = std::max(t_dev_peak, t_dev_bytes);
+    return p;
+}
+
+void DevPool::give(void* p, size_t cap) {
+    idle.emplace(cap, p);
+    idle_bytes += (int64_t)cap;
+}
+
+void DevPool::trim() {
+    if (idle.empty()) return;
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto& b : idle) {
+        (void)hipFree(b.second);
+        frees++;
+        held -= (int64_t)b.first;
+        t_dev_bytes -= (int64_t)b.first;
+    }
+    idle.clear();
+    idle_bytes = 0;
+}
+
+DevPool::~DevPool() { trim(); }
+
 DevBuf::~DevBuf() { release(); }
 void DevBuf::release() {
     if (p) {
-        (void)hipFree(p);
-        t_dev_bytes -= (int64_t)cap;
+        if (pool) {
+            pool->give(p, cap);
+        } else {
+            (void)hipFree(p);
+            t_dev_bytes -= (int64_t)cap;
+        }
     }
     p = nullptr;
     cap = 0;
@@ -2108,6 +2181,13 @@ void DevBuf::release() {
 void* DevBuf::ensure(size_t bytes) {
     if (bytes <= cap && p) return p;
     release();
+    if (!pool) pool = t_pool;
+    if (pool) {
+        size_t c = 0;
+        p = pool->take(bytes, c);
+        cap = c;
+        return p;
+    }
     size_t want = std::max<size_t>(bytes, 256);
     hipError_t e = hipMalloc(&p, want);
     if (e != hipSuccess) {
@@ -2132,8 +2212,10 @@ Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), dev
     if (ndev < 1) throw std::runtime_error("no HIP device available");
     if (device < 0 || device >= ndev) throw std::runtime_error("device index out of range");
     HIPCHK(hipSetDevice(device));
+    PoolScope ps(&pool_);
     KW_ = (P_.M + 15) / 16;
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    pool_.stream = stream_;
     BLASCHK(rocblas_create_handle(&blas_));
     BLASCHK(rocblas_set_stream(blas_, stream_));
     // no atomics-based algorithms: the same inputs give the same bits on every run
@@ -2143,6 +2225,10 @@ Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), dev
 }
 
 Engine::~Engine() {
+    // the buffers go back to the pool after this body, and the pool frees
+    // them without waiting for the stream, which is finished here
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    pool_.stream = nullptr;
     for (hipEvent_t e : prof_.ev) (void)hipEventDestroy(e);
     for (hipEvent_t e : prof_.gev) (void)hipEventDestroy(e);
     if (blas_) rocblas_destroy_handle(blas_);
@@ -2173,6 +2259,8 @@ void Engine::upload_tables() {
 }
 
 void Engine::set_basis(const int32_t* q, int64_t n) {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     if (n < 1) throw std::runtime_error("empty basis");
     if (n >= (int64_t)INT32_MAX) throw std::runtime_error("basis larger than 2^31-1 states");
     const int M = P_.M;
@@ -2193,9 +2281,12 @@ void Engine::set_basis(const int32_t* q, int64_t n) {
     xcols_ = 0;
     rebuild_hash(n);
     HIPCHK(hipStreamSynchronize(stream_));
+    pool_.trim();
 }
 
 void Engine::set_initial_basis() {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     const int M = P_.M;
     const int K = KW_ * 16;
     if (M < 1) throw std::runtime_error("empty basis");
@@ -2242,6 +2333,8 @@ void Engine::set_initial_basis() {
 }
 
 void Engine::get_basis(int32_t* q) const {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     const int M = P_.M, K = KW_ * 16;
     std::vector<uint8_t> packed((size_t)n_ * K);
     HIPCHK(hipMemcpyAsync(packed.data(), keys_.p, packed.size(), hipMemcpyDeviceToHost, stream_));
@@ -2326,10 +2419,7 @@ void Engine::build_h_t(HStats& st) {
                       !(fb && std::string(fb) == "1");
     built_n_ = 0;
     const bool old_perm_on = perm_on_;
-    auto swap_buf = [](DevBuf& x, DevBuf& y) {
-        std::swap(x.p, y.p);
-        std::swap(x.cap, y.cap);
-    };
+    auto swap_buf = [](DevBuf& x, DevBuf& y) { x.swap(y); };
     if (incr) {
         swap_buf(perm_, perm_o_);
         swap_buf(inv_, inv_o_);
@@ -2512,6 +2602,7 @@ void Engine::build_h_t(HStats& st) {
 }
 
 HStats Engine::build_h() {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     if (n_ < 1) throw std::runtime_error("build_h: the basis is empty");
     auto t0 = clk::now();
@@ -2530,6 +2621,8 @@ HStats Engine::build_h() {
 
 void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std::vector<double>& val,
                    std::vector<double>& diag) {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
     if (matfree_) throw std::runtime_error("the Hamiltonian is matrix-free: no stored entries to return");
     rowptr.resize(n_ + 1);
@@ -2586,6 +2679,8 @@ void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std:
 // one <= fcmax-1 gets a triplet, zero or not; after H + H^T that is n plus
 // twice the off-diagonal pairs.  Counted by probing the L1 ball.
 int64_t Engine::structural_nnz() {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     if (T_.ball.empty() || n_ == 0) return -1;
     HIPCHK(hipSetDevice(device_));
     if (!d_ball_.p) upload(d_ball_, T_.ball, stream_);
@@ -2778,6 +2873,7 @@ __global__ void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, int
 }
 
 double Engine::copy_probe(size_t bytes) {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     const int64_t nw = (int64_t)(bytes / 16);
     DevBuf a, b;
@@ -2799,6 +2895,7 @@ double Engine::copy_probe(size_t bytes) {
 }
 
 void Engine::apply_h(const double* X, double* Y, int b, bool fp32) {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
     if (b < 1 || b > 64) throw std::runtime_error("apply_h: the block width must be 1 to 64");
@@ -2831,6 +2928,8 @@ void Engine::apply_h(const double* X, double* Y, int b, bool fp32) {
 }
 
 double Engine::time_spmm(int b, int reps, bool fp32, double* bytes_per_call) {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
     const int64_t n = n_;
     DevBuf xb, yb;
@@ -2964,6 +3063,7 @@ void Engine::dense_solve(int nev, std::vector<double>& E, EigStats& st) {
 }
 
 EigStats Engine::dense_all(std::vector<double>& E) {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
     auto t0 = clk::now();
@@ -3701,6 +3801,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
 }
 
 EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E) {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
     auto t0 = clk::now();
@@ -3723,6 +3824,8 @@ EigStats Engine::eigsolve(int nev, double tol, int maxit, std::vector<double>& E
 }
 
 void Engine::get_vectors(double* C, int ncol) const {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     if (ncol > xcols_) throw std::runtime_error("requested more eigenvector columns than resident");
     std::vector<double> xo((size_t)xrows_ * xcols_);
     HIPCHK(hipMemcpyAsync(xo.data(), X_.p, xo.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
@@ -3732,6 +3835,7 @@ void Engine::get_vectors(double* C, int ncol) const {
 }
 
 void Engine::freq_assignments(int ncol, std::vector<FreqColumn>& out) const {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     if (ncol < 1 || ncol > xcols_ || xrows_ < 1) throw std::runtime_error("no resident eigenvectors to assign");
     const int64_t n = xrows_;
@@ -3801,6 +3905,8 @@ void Engine::freq_assignments(int ncol, std::vector<FreqColumn>& out) const {
 }
 
 void Engine::set_vectors(const double* C, int ncol) {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     X_.ensure((size_t)n_ * ncol * sizeof(double));
     HIPCHK(hipMemcpyAsync(X_.p, C, (size_t)n_ * ncol * sizeof(double), hipMemcpyHostToDevice, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
@@ -3907,6 +4013,7 @@ void Engine::hb_select_t(double eps, HbStats& st) {
 }
 
 HbStats Engine::hb_select(const double* cmax_host, int nopt, double eps) {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     auto t0 = clk::now();
     HbStats st;
@@ -3935,6 +4042,8 @@ HbStats Engine::hb_select(const double* cmax_host, int nopt, double eps) {
 }
 
 void Engine::get_new_states(int32_t* q) const {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
     const int M = P_.M, K = KW_ * 16;
     std::vector<uint8_t> packed((size_t)nnew_ * K);
     if (nnew_)
@@ -3945,6 +4054,7 @@ void Engine::get_new_states(int32_t* q) const {
 }
 
 void Engine::append_new_states() {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     if (nnew_ == 0) return;
     const int64_t n1 = n_ + nnew_;
@@ -3955,8 +4065,7 @@ void Engine::append_new_states() {
         nk.ensure(std::max<size_t>(n1 * K, keys_.cap * 3 / 2));
         HIPCHK(hipMemcpyAsync(nk.p, keys_.p, n_ * K, hipMemcpyDeviceToDevice, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
-        std::swap(nk.p, keys_.p);
-        std::swap(nk.cap, keys_.cap);
+        nk.swap(keys_);
     }
     HIPCHK(hipMemcpyAsync(keys_.as<uint8_t>() + n_ * K, newkeys_.p, nnew_ * K, hipMemcpyDeviceToDevice, stream_));
     const int64_t n0 = n_;
@@ -3976,6 +4085,17 @@ void Engine::append_new_states() {
         HIPCHK(hipGetLastError());
     }
     HIPCHK(hipStreamSynchronize(stream_));
+    // a heat-bath iteration boundary: the blocks the last iteration's phases
+    // gave back are sized for the smaller basis and go back to the device
+    pool_.trim();
+}
+
+void Engine::pool_stats(int64_t out[5]) const {
+    out[0] = pool_.mallocs;
+    out[1] = pool_.frees;
+    out[2] = pool_.reuses;
+    out[3] = pool_.held;
+    out[4] = pool_.idle_bytes;
 }
 
 // ---------------------------------------------------------------- PT2
@@ -4012,6 +4132,9 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st) {
     st.pairs = npairs;
     size_t freeb = 0, totb = 0;
     HIPCHK(hipMemGetInfo(&freeb, &totb));
+    if (std::getenv("VHCI_DEBUG"))
+        std::fprintf(stderr, "[pt2] push form: %lld external states, %lld pairs, %.1f GB free\n", (long long)ns2,
+                     (long long)npairs, freeb / 1e9);
     if ((double)npairs * 32.0 > 0.25 * (double)freeb) return false;
     DevBuf pk, pv, sk, sv, seg;
     pk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
@@ -4134,6 +4257,7 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::ve
 }
 
 Pt2Stats Engine::pt2(const std::vector<double>& E, int nopt, double eps2, std::vector<double>& dE) {
+    PoolScope ps(&pool_);
     HIPCHK(hipSetDevice(device_));
     if (nopt < 1 || nopt > xcols_ || (int)E.size() < nopt) throw std::runtime_error("PT2: bad number of states");
     if (nopt > 256) throw std::runtime_error("PT2: at most 256 states");

@@ -9,7 +9,9 @@
 #include <rocblas/rocblas.h>
 
 #include <cstdint>
+#include <map>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "problem.hpp"
@@ -39,15 +41,57 @@ struct HostBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Device memory of one context.  Blocks that the context's buffers give
+// back are kept idle and handed out again to later requests of up to twice
+// their size, so the phases of a solve call neither hipFree (which waits
+// for the whole device) nor, once the context has held blocks of the sizes
+// they need, hipMalloc.  Idle blocks go back to the device at heat-bath
+// iteration boundaries (trim), when an allocation fails, and when the
+// context ends.  Every block is used on the context's stream only, so a
+// block handed out again is ordered after its previous use.
+struct DevPool {
+    hipStream_t stream = nullptr;
+    std::multimap<size_t, void*> idle;  // capacity -> block
+    int64_t held = 0;                   // bytes taken from the device (in use + idle)
+    int64_t idle_bytes = 0;
+    int64_t mallocs = 0, frees = 0, reuses = 0;  // hipMalloc / hipFree calls, idle blocks handed out
+    DevPool() = default;
+    DevPool(const DevPool&) = delete;
+    DevPool& operator=(const DevPool&) = delete;
+    ~DevPool();
+    // a block of at least `bytes` (its capacity in cap); throws on failure
+    void* take(size_t bytes, size_t& cap);
+    void give(void* p, size_t cap);
+    // waits for the stream, then returns every idle block to the device
+    void trim();
+};
+
+// The pool that buffers first sized on this thread draw from: set for the
+// duration of each Engine call (PoolScope), none outside.
+DevPool* current_pool();
+struct PoolScope {
+    DevPool* prev;
+    explicit PoolScope(DevPool* p);
+    ~PoolScope();
+    PoolScope(const PoolScope&) = delete;
+    PoolScope& operator=(const PoolScope&) = delete;
+};
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    DevPool* pool = nullptr;  // taken from current_pool() at the first ensure; none: plain hipMalloc/hipFree
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf();
     void* ensure(size_t bytes);  // throws std::runtime_error on failure
     void release();
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(cap, o.cap);
+        std::swap(pool, o.pool);
+    }
     template <class T>
     T* as() const { return static_cast<T*>(p); }
 };
@@ -167,7 +211,12 @@ class Engine {
     // column), row pointers, the diagonal, X read once and Y written once
     double spmm_bytes(int b, int mode) const;
 
+    // device-memory pool of this context: calls made, bytes held
+    void pool_stats(int64_t out[5]) const;
+
   private:
+    // declared first so that it outlives every buffer below
+    mutable DevPool pool_;
     void upload_tables();
     void make_solver_order();
     void rebuild_hash(int64_t cap_hint);

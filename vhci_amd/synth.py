@@ -11,8 +11,21 @@ Workloads (seed = 0x5648434900 + config):
   c1  heat-bath VCI + PT2: 15 modes, all 680 cubic, the 330 semi-diagonal
       quartics plus 270 distinct f_iijk (600 quartic); Ntot 2, caps 6,
       eps1 = 1.0, Nstate = 100, eps2 = 0.01.
-  c2  48 modes: cubic and quartic (semi-diagonal + iijk) pools sampled with
-      |f| >= 0.5 kept; Ntot 2, caps 6, eps1 = 1, Nstate = 128, eps2 = 0.2.
+  c2  48 modes, frequencies in [150, 3200]: all 19,600 cubic f_ijk with |f|
+      log-uniform in [0.5, 60 w/3200], and the 3,432 semi-diagonal plus
+      51,888 f_iijk quartics with |f| log-uniform in [0.25, 12 w/3200], w
+      the geometric mean frequency of the constant's modes; |f| < 0.5 is
+      dropped (42,212 quartics kept) and f_iiii > 0.  Ntot 2, caps 6,
+      eps1 = 1, Nstate = 128, eps2 = 0.2.  The workload fixes the counts
+      (about 2e4 cubic, 4e4 quartic), the 0.5 cutoff and log-uniform
+      magnitudes, not the upper ends.  With config 0's ends (300 and 60 for
+      every mode) the 48-mode heat bath does not converge (the basis passes
+      16 M states): every mode with a few hundred cm^-1 is then coupled as
+      strongly as the 3000 cm^-1 ones.  Ends proportional to the modes'
+      frequencies keep each mode about as anharmonic relative to its own
+      frequency, as in real force fields, and the basis converges after
+      three iterations at 1,598,434 states (1225 -> 1,003,305 ->
+      1,548,210 -> 1,598,434) with about 1.7e9 nonzeros.
   c3  serving stream: request r (seed 0x5648434900 + 1000 + r) draws
       M uniform in [8, 36], eps1 from {0.5, 1, 2}, Nstate uniform in
       [20, 200] and eps2 from {0, 0.01, 0.1}; the field has all cubic and
@@ -126,17 +139,31 @@ def write_input(path_or_none, comment, eps1, nstate, pt2, eps2, ntot, freqs, cap
 
 
 def field(M, flo, fhi, seed, cubic=(1.0, 300.0), quartic=(0.1, 60.0), n_extra_iijk=0, cutoff=0.0,
-          cubic_pool=None, quartic_pool=None, positive_iiii=False):
+          cubic_pool=None, quartic_pool=None, positive_iiii=False, freq_scale=0.0):
+    """Frequencies uniform in [flo, fhi]; one log-uniform magnitude with a
+    random sign per constant of the pools.  freq_scale = p > 0 scales the
+    upper end of each constant's range by (w / fhi)^p, w the geometric mean
+    frequency of its modes (a range that would end below its lower end ends
+    just above it)."""
+    import math
     rng = Rng(seed)
     freqs = [rng.uniform(flo, fhi) for _ in range(M)]
     fcs = []
+
+    def draw(t, rng_ab):
+        a, b = rng_ab
+        if freq_scale:
+            w = math.exp(sum(math.log(freqs[i]) for i in t) / len(t))
+            b = max(a * 1.0001, b * (w / fhi) ** freq_scale)
+        return rng.logmag(a, b)
+
     for t in (cubic_pool if cubic_pool is not None else _cubic_all(M)):
-        v = rng.logmag(*cubic)
+        v = draw(t, cubic)
         if abs(v) >= cutoff:
             fcs.append((t, v))
     qp = quartic_pool if quartic_pool is not None else _semidiag_quartic(M)
     for t in qp:
-        v = rng.logmag(*quartic)
+        v = draw(t, quartic)
         if positive_iiii and t[0] == t[1] == t[2] == t[3]:
             v = abs(v)
         if abs(v) >= cutoff:
@@ -164,8 +191,8 @@ def config_input(c, path=None, **override):
         args = dict(eps1=1.0, nstate=100, pt2=True, eps2=0.01, ntot=2, caps=[6] * M)
     elif c == 2:
         M = 48
-        freqs, fcs = field(M, 150.0, 3200.0, seed, cubic=(0.05, 300.0), quartic=(0.05, 60.0), cutoff=0.5,
-                           quartic_pool=_semidiag_quartic(M) + _iijk_pool(M)[::8])
+        freqs, fcs = field(M, 150.0, 3200.0, seed, cubic=(0.5, 60.0), quartic=(0.25, 12.0), cutoff=0.5,
+                           quartic_pool=_semidiag_quartic(M) + _iijk_pool(M), positive_iiii=True, freq_scale=1.0)
         args = dict(eps1=1.0, nstate=128, pt2=True, eps2=0.2, ntot=2, caps=[6] * M)
     else:
         raise ValueError("unknown workload %r" % c)
