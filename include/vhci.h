@@ -123,6 +123,13 @@ int64_t vhci_structural_nnz(vhci_ctx* ctx);
  * rows in basis order, columns ascending), diag[n].  Pass rowptr=NULL to
  * query nnz via *nnz. */
 int vhci_get_h(vhci_ctx* ctx, int64_t* nnz, int64_t* rowptr, int32_t* col, double* val, double* diag);
+/* The same for the nr basis rows rows[] only: rowptr[nr+1], col/val[nnz]
+ * (each row's off-diagonal entries, columns in basis order ascending),
+ * diag[nr].  Pass rowptr=NULL to query nnz via *nnz.  Test support for
+ * bases whose whole matrix is too large to copy; no counterpart in the
+ * reference. */
+int vhci_get_h_rows(vhci_ctx* ctx, const int64_t* rows, int64_t nr, int64_t* nnz, int64_t* rowptr, int32_t* col,
+                    double* val, double* diag);
 
 /* SparseDiagonalize (src/Ham.cpp:347-370): the nev lowest eigenpairs,
  * converged to ||H x - theta x|| < tol * max(eps^(2/3), |theta|).  E[nev]
@@ -156,7 +163,13 @@ int vhci_append_new_states(vhci_ctx* ctx);
 
 /* DoPT2 (src/PT2.cpp:16-154): external space by heat bath with eps2 from
  * the resident vectors, then dE[n] = sum_a (sum_i H_ai C_in)^2 / (E_n - E_a)
- * for n < nopt.  *ns2 (nullable) receives the external-space size. */
+ * for n < nopt.  *ns2 (nullable) receives the external-space size.  The
+ * external space is processed in chunks (ranges of its states' hash) so
+ * that at most 2^28 external states and 2^30 (a, i) pairs are held at once
+ * (VHCI_PT2_CHUNK_STATES / VHCI_PT2_CHUNK_PAIRS lower the budgets);
+ * afterwards vhci_get_new_states returns the last chunk's states, which are
+ * the whole external space when it fit one chunk.  The eigensolver's work
+ * blocks are released first (the resident eigenvectors stay). */
 int vhci_pt2(vhci_ctx* ctx, const double* E, int nopt, double eps2, double* dE, int64_t* ns2);
 
 /* Y = H X with the resident H for b vectors (1 <= b <= 64): X and Y are
@@ -189,12 +202,13 @@ int vhci_spmm_profile_read(vhci_ctx* ctx, double* out);
  * support; no counterpart in the reference. */
 int vhci_gemm_profile_read(vhci_ctx* ctx, double* out);
 /* Counters of the last phases (measurement support; no counterpart in the
- * reference), out[14]: last H build n, stored entries, connection
+ * reference), out[15]: last H build n, stored entries, connection
  * patterns, build ms; last heat-bath selection (after vhci_pt2: the PT2
- * one) candidates, unique states, ms; last PT2 external states,
+ * one, all chunks) candidates, unique states, ms; last PT2 external states,
  * (external, variational) pairs, sum ms; key bytes; PT2 states; rows the
  * last H build probed (all, or only the new states' rows of an incremental
- * build); entries it carried over from the previous build. */
+ * build); entries it carried over from the previous build; chunks the last
+ * PT2 summed its external space in. */
 int vhci_phase_stats(vhci_ctx* ctx, double* out);
 /* Device-memory pool of the context, out[5]: hipMalloc calls, hipFree
  * calls, blocks handed out again from the idle set, bytes held (in use +

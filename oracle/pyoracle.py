@@ -46,6 +46,8 @@ def lib():
             "orc_anharm_pot": (ctypes.c_double, [_vp, ctypes.c_int, ctypes.c_int, _i32p, _i32p]),
             "orc_build_h": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _i64p, _i64p, _i32p, _dp]),
             "orc_build_h_dense": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _dp]),
+            "orc_h_rows": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p, _i64p, _i32p,
+                                          _dp]),
             "orc_eigh": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp]),
             "orc_lanczos": (ctypes.c_int, [ctypes.c_int64, _i64p, _i32p, _dp, ctypes.c_int, ctypes.c_double,
                                            ctypes.c_int, _dp, _dp]),
@@ -118,6 +120,21 @@ class OProblem:
         col = np.zeros(nnz.value, dtype=np.int32)
         val = np.zeros(nnz.value)
         lib().orc_build_h(self.h, _p(q, _i32p), n, ctypes.byref(nnz), _p(rp, _i64p), _p(col, _i32p), _p(val, _dp))
+        return rp, col, val
+
+    def h_rows(self, q, rows):
+        """rows of the all-pairs H over the whole basis q: (rowptr, col, val)"""
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        nnz = ctypes.c_int64()
+        if lib().orc_h_rows(self.h, _p(q, _i32p), q.shape[0], _p(rows, _i64p), len(rows), ctypes.byref(nnz), None,
+                            None, None):
+            raise RuntimeError(lib().orc_error().decode())
+        rp = np.zeros(len(rows) + 1, dtype=np.int64)
+        col = np.zeros(nnz.value, dtype=np.int32)
+        val = np.zeros(nnz.value)
+        lib().orc_h_rows(self.h, _p(q, _i32p), q.shape[0], _p(rows, _i64p), len(rows), ctypes.byref(nnz),
+                         _p(rp, _i64p), _p(col, _i32p), _p(val, _dp))
         return rp, col, val
 
     def build_h_dense(self, q):

@@ -1077,6 +1077,51 @@ int orc_build_h(const orc_problem* p, const int32_t* q, int64_t n, int64_t* nnz,
     return 0;
 }
 
+/* Selected rows of the all-pairs H (Ham.cpp:237-326): for row r every
+ * column j of the basis, the element computed from the lower index as the
+ * reference's i <= j loop does, diagonal = harmonic + V_rr; structural
+ * entries (explicit zeros) included, columns ascending. */
+int orc_h_rows(const orc_problem* p, const int32_t* q, int64_t n, const int64_t* rows, int64_t nr, int64_t* nnz,
+               int64_t* rowptr, int32_t* col, double* val) {
+    const Problem& P = p->P;
+    const int M = P.M;
+    int nt = g_threads > 0 ? g_threads : omp_get_max_threads();
+    std::vector<std::vector<std::pair<int32_t, double>>> out(nr);
+    for (int64_t k = 0; k < nr; k++) {
+        const int64_t r = rows[k];
+        if (r < 0 || r >= n) return fail("orc_h_rows: row out of range");
+        const int64_t nb = (n + 4095) / 4096;
+        std::vector<std::vector<std::pair<int32_t, double>>> part(nb);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+        for (int64_t b = 0; b < nb; b++) {
+            std::vector<int> dv(M);
+            for (int64_t j = b * 4096; j < std::min(n, (b + 1) * 4096); j++) {
+                const int64_t lo = std::min(r, j), hi = std::max(r, j);
+                double v;
+                if (pair_element(P, q + lo * M, q + hi * M, dv.data(), v)) {
+                    if (j == r) v += harmonic(P, q + r * M);
+                    part[b].push_back({(int32_t)j, v});
+                }
+            }
+        }
+        for (auto& x : part) out[k].insert(out[k].end(), x.begin(), x.end());
+    }
+    int64_t tot = 0;
+    for (auto& x : out) tot += (int64_t)x.size();
+    *nnz = tot;
+    if (rowptr) {
+        rowptr[0] = 0;
+        for (int64_t k = 0; k < nr; k++) {
+            rowptr[k + 1] = rowptr[k] + (int64_t)out[k].size();
+            for (size_t t = 0; t < out[k].size(); t++) {
+                col[rowptr[k] + t] = out[k][t].first;
+                val[rowptr[k] + t] = out[k][t].second;
+            }
+        }
+    }
+    return 0;
+}
+
 int orc_build_h_dense(const orc_problem* p, const int32_t* q, int64_t n, double* H) {
     std::vector<double> A;
     build_h_dense(p->P, q, n, A);

@@ -58,6 +58,11 @@ double orc_anharm_pot(const orc_problem* p, int which, int idx,
  * Output: full symmetric CSR, columns ascending. */
 int orc_build_h(const orc_problem* p, const int32_t* q, int64_t n,
                 int64_t* nnz, int64_t* rowptr, int32_t* col, double* val);
+/* Selected rows of the all-pairs Hamiltonian over the whole basis (every
+ * column, element from the lower index, diagonal included, structural zeros
+ * kept, columns ascending).  Two-call pattern as orc_build_h. */
+int orc_h_rows(const orc_problem* p, const int32_t* q, int64_t n, const int64_t* rows, int64_t nr, int64_t* nnz,
+               int64_t* rowptr, int32_t* col, double* val);
 /* Dense H (Ham.cpp:107-200) as used for the first HCI iteration */
 int orc_build_h_dense(const orc_problem* p, const int32_t* q, int64_t n, double* H);
 

@@ -53,6 +53,7 @@ _SIGS = {
     "vhci_matrix_free": (ctypes.c_int, [_vp]),
     "vhci_structural_nnz": (ctypes.c_int64, [_vp]),
     "vhci_get_h": (ctypes.c_int, [_vp, _i64p, _i64p, _i32p, _dp, _dp]),
+    "vhci_get_h_rows": (ctypes.c_int, [_vp, _i64p, ctypes.c_int64, _i64p, _i64p, _i32p, _dp, _dp]),
     "vhci_eigsolve": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp]),
     "vhci_dense_diag": (ctypes.c_int, [_vp, _dp, _dp]),
     "vhci_ritz_cols": (ctypes.c_int, [_vp]),
@@ -275,6 +276,21 @@ class Context:
                                  _ptr(diag, _dp)))
         return rp, col[: nnz.value], val[: nnz.value], diag
 
+    def get_h_rows(self, rows):
+        """(rowptr, col, val, diag) of the given basis rows (off-diagonal
+        entries, columns in basis order)"""
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        nnz = ctypes.c_int64()
+        self._check(lib().vhci_get_h_rows(self.h, _ptr(rows, _i64p), len(rows), ctypes.byref(nnz), None, None, None,
+                                          None))
+        rp = np.zeros(len(rows) + 1, dtype=np.int64)
+        col = np.zeros(nnz.value, dtype=np.int32)
+        val = np.zeros(nnz.value)
+        diag = np.zeros(len(rows))
+        self._check(lib().vhci_get_h_rows(self.h, _ptr(rows, _i64p), len(rows), ctypes.byref(nnz), _ptr(rp, _i64p),
+                                          _ptr(col, _i32p), _ptr(val, _dp), _ptr(diag, _dp)))
+        return rp, col, val, diag
+
     def apply_h(self, X, fp32=False):
         """H X for an n x b block (or an n-vector) in basis order, through
         the solver's product kernels (fp32: the mixed form's FP32 product)."""
@@ -395,10 +411,10 @@ class Context:
         return float(out[0]), float(out[1]), int(out[2])
 
     def phase_stats(self):
-        out = np.zeros(14)
+        out = np.zeros(15)
         self._check(lib().vhci_phase_stats(self.h, _ptr(out, _dp)))
         keys = ["n", "nnz", "patterns", "build_ms", "hb_candidates", "hb_unique", "hb_ms", "ns2", "pairs",
-                "pt2_sum_ms", "key_bytes", "nstate", "probed_rows", "carried"]
+                "pt2_sum_ms", "key_bytes", "nstate", "probed_rows", "carried", "pt2_chunks"]
         return dict(zip(keys, out.tolist()))
 
     def pool_stats(self):

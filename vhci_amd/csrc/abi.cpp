@@ -372,6 +372,24 @@ int vhci_get_h(vhci_ctx* ctx, int64_t* nnz, int64_t* rowptr, int32_t* col, doubl
     });
 }
 
+int vhci_get_h_rows(vhci_ctx* ctx, const int64_t* rows, int64_t nr, int64_t* nnz, int64_t* rowptr, int32_t* col,
+                    double* val, double* diag) {
+    if (!rows || nr < 0 || !nnz) return VHCI_E_ARG;
+    return guard(ctx, [&] {
+        std::vector<int64_t> rp;
+        std::vector<int32_t> cl;
+        std::vector<double> vl, dg;
+        ctx->eng->get_h_rows(rows, nr, rp, cl, vl, dg);
+        *nnz = (int64_t)cl.size();
+        if (rowptr) {
+            std::memcpy(rowptr, rp.data(), rp.size() * sizeof(int64_t));
+            std::memcpy(col, cl.data(), cl.size() * sizeof(int32_t));
+            std::memcpy(val, vl.data(), vl.size() * sizeof(double));
+            std::memcpy(diag, dg.data(), dg.size() * sizeof(double));
+        }
+    });
+}
+
 int vhci_eigsolve(vhci_ctx* ctx, int nev, double tol, int maxit, double* E, double* C) {
     return guard(ctx, [&] {
         std::vector<double> e;

@@ -15,6 +15,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -139,10 +142,10 @@ __device__ __forceinline__ void key_add(uint32_t (&w)[NW], int m, int d) {
     for (int t = 0; t < NW; t++) w[t] += add & (0u - (uint32_t)(t == wi));
 }
 
+// probe with the key's hash h already computed
 template <int NW>
-__device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots, uint64_t mask,
-                                             const uint4* __restrict__ keys, const uint32_t (&w)[NW]) {
-    const uint64_t h = key_hash<NW>(w);
+__device__ __forceinline__ int64_t hash_find_h(const uint64_t* __restrict__ slots, uint64_t mask,
+                                               const uint4* __restrict__ keys, const uint32_t (&w)[NW], uint64_t h) {
     uint64_t pos = h & mask;
     const uint32_t tag = (uint32_t)(h >> 32);
     while (true) {
@@ -151,6 +154,12 @@ __device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots,
         if ((uint32_t)(s >> 32) == tag && key_eq<NW>(keys, (uint32_t)s, w)) return (int64_t)(uint32_t)s;
         pos = (pos + 1) & mask;
     }
+}
+
+template <int NW>
+__device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots, uint64_t mask,
+                                             const uint4* __restrict__ keys, const uint32_t (&w)[NW]) {
+    return hash_find_h<NW>(slots, mask, keys, w, key_hash<NW>(w));
 }
 
 // ------------------------------------------------------------------------
@@ -304,6 +313,7 @@ struct HBuildArgs {
     const uint64_t* slots;
     uint64_t mask;
     const DeltaEnt* deltas;
+    const uint32_t* poff;  // M + 2 block offsets of deltas
     const GroupEnt* groups;
     const uint32_t* fclist;
     const ElemFC* elem;
@@ -375,6 +385,26 @@ __device__ __forceinline__ bool apply_pattern(const uint4 e, int nz, const uint8
         }
     }
     return ok;
+}
+
+// Walk of the pattern blocks a row can use (the table is ordered so that
+// block 0 lowers no mode and block 1 + m has m as its lowest lowered mode):
+// block 0, then the block of every mode with quanta.  b = -1 starts the
+// walk; false when no block is left.  sq is the row's quanta in LDS, so the
+// walk is the same for every lane.
+__device__ __forceinline__ bool next_block(const uint32_t* __restrict__ off, int M, const uint8_t* __restrict__ sq,
+                                           int& b, int& p0, int& p1) {
+    if (b < 0) {
+        b = 0;
+    } else {
+        do {
+            ++b;
+        } while (b <= M && sq[b - 1] == 0);
+        if (b > M) return false;
+    }
+    p0 = (int)off[b];
+    p1 = (int)off[b + 1];
+    return true;
 }
 
 // sum over a list of zero-pattern FCs of fc * prod_t L[k][q][q] (in list order)
@@ -451,11 +481,13 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         }
         base += count;
     };
-    for (int p0 = 0; p0 < a.P; p0 += kWave) {
+    int blk = -1, pb0 = 0, pb1 = 0;
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
         bool hit = false;
         int64_t j = -1;
-        if (p < a.P) {
+        if (p < pb1) {
             const uint4 e = rec_load(a.deltas, p);
             uint32_t t[NW];
 #pragma unroll
@@ -539,11 +571,13 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
                 if (c < f.bcols) acc[c] = acc[c] + v * xr[c];
         }
     };
-    for (int p0 = 0; p0 < a.P; p0 += kWave) {
+    int blk = -1, pb0 = 0, pb1 = 0;
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
         bool hit = false;
         int64_t j = -1;
-        if (p < a.P) {
+        if (p < pb1) {
             const uint4 e = rec_load(a.deltas, p);
             uint32_t t[NW];
 #pragma unroll
@@ -1511,6 +1545,7 @@ struct HbGenArgs {
     uint64_t* pool_rank;
     uint32_t* counters;  // [0] pool fill, [1] overflow flags
     uint32_t pool_cap;
+    uint64_t tlo, thi;   // keep only candidates whose hash tag lies in [tlo, thi)
 };
 
 template <int NW>
@@ -1527,8 +1562,7 @@ __device__ __forceinline__ bool pool_key_eq(const uint4* pool, uint32_t idx, con
 
 // insert a candidate into the new-state hash; keeps the smallest rank per key
 template <int NW>
-__device__ void new_insert(const HbGenArgs& a, const uint32_t (&w)[NW], uint64_t rank) {
-    const uint64_t h = key_hash<NW>(w);
+__device__ void new_insert(const HbGenArgs& a, const uint32_t (&w)[NW], uint64_t h, uint64_t rank) {
     uint64_t pos = h & a.nmask;
     const uint32_t tag = (uint32_t)(h >> 32);
     int64_t mine = -1;
@@ -1604,8 +1638,11 @@ __global__ __launch_bounds__(256) void k_hb_gen(HbGenArgs a) {
 #pragma unroll
             for (int t = 0; t < kMaxD; t++)
                 if (t < d) key_add<NW>(w, (int)rec_u8(v, 1 + t), rec_i8(v, 7 + t));
-            if (hash_find<NW>(a.bslots, a.bmask, a.keys, w) >= 0) continue;
-            new_insert<NW>(a, w, (uint64_t)(g0 + r));
+            const uint64_t h = key_hash<NW>(w);
+            const uint64_t tag = h >> 32;
+            if (tag < a.tlo || tag >= a.thi) continue;
+            if (hash_find_h<NW>(a.bslots, a.bmask, a.keys, w, h) >= 0) continue;
+            new_insert<NW>(a, w, h, (uint64_t)(g0 + r));
         }
     }
 }
@@ -1648,6 +1685,7 @@ struct Pt2Args {
     const uint64_t* vslots;
     uint64_t vmask;
     const DeltaEnt* deltas;
+    const uint32_t* poff;  // M + 2 block offsets of deltas
     const GroupEnt* groups;
     const uint32_t* fclist;
     const ElemFC* elem;
@@ -1660,6 +1698,7 @@ struct Pt2Args {
     int ldc;
     const double* E;
     double* part;  // nwaves x nopt x 2
+    int carry;     // 1: continue the sums already in part (a later chunk)
 };
 
 constexpr int kHitBuf = 512;
@@ -1674,7 +1713,12 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
     constexpr int R = 4;  // up to 256 columns
     double ks[R], kc[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) ks[r] = kc[r] = 0.0;
+    for (int r = 0; r < R; r++) {
+        const int c = lane + 64 * r;
+        const bool on = a.carry && c < a.nopt;
+        ks[r] = on ? a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 0] : 0.0;
+        kc[r] = on ? a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 1] : 0.0;
+    }
     for (int64_t ia = blockIdx.x; ia < a.ns2; ia += gridDim.x) {
         uint32_t w[NW];
         key_load<NW>(a.akeys, ia, w);
@@ -1688,13 +1732,34 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
             num32[r] = 0.0f;
         }
         int nh = 0;
-        for (int p0 = 0; p0 < a.P; p0 += kWave) {
+        auto flush = [&]() {
+            __syncthreads();
+            for (int h = 0; h < nh; h++) {
+                const int64_t i = hidx[h];
+                const double hv = hval[h];
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int c = lane + 64 * r;
+                    if (c < a.nopt) {
+                        if (MIXED)
+                            num32[r] = fmaf((float)hv, a.C32[i * a.ldc + c], num32[r]);
+                        else
+                            num[r] = fma(hv, a.C[i * a.ldc + c], num[r]);
+                    }
+                }
+            }
+            __syncthreads();
+            nh = 0;
+        };
+        int blk = -1, pb0 = 0, pb1 = 0;
+        while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+        for (int p0 = pb0; p0 < pb1; p0 += kWave) {
             const int p = p0 + lane;
             bool hit = false;
             int64_t j = -1;
             uint4 e = make_uint4(0u, 0u, 0u, 0u);
             int nz = 0;
-            if (p < a.P) {
+            if (p < pb1) {
                 e = rec_load(a.deltas, p);
                 nz = delta_nz(e);
                 uint32_t t[NW];
@@ -1714,26 +1779,9 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
                 hval[pos] = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, nz, true);
             }
             nh += __popcll(bal);
-            if (nh > kHitBuf - kWave || p0 + kWave >= a.P) {
-                __syncthreads();
-                for (int h = 0; h < nh; h++) {
-                    const int64_t i = hidx[h];
-                    const double hv = hval[h];
-#pragma unroll
-                    for (int r = 0; r < R; r++) {
-                        const int c = lane + 64 * r;
-                        if (c < a.nopt) {
-                            if (MIXED)
-                                num32[r] = fmaf((float)hv, a.C32[i * a.ldc + c], num32[r]);
-                            else
-                                num[r] = fma(hv, a.C[i * a.ldc + c], num[r]);
-                        }
-                    }
-                }
-                __syncthreads();
-                nh = 0;
-            }
+            if (nh > kHitBuf - kWave) flush();
         }
+        flush();
         // E_a (PT2.cpp:119-144): harmonic sum, then each zero-pattern FC
         // term added one at a time in list order
         double ea = 0.;
@@ -1859,6 +1907,8 @@ struct Pt2PushArgs {
     const uint4* apool;
     const uint32_t* apos;
     const DeltaEnt* deltas;
+    const uint32_t* poff;  // M + 2 block offsets of deltas
+    uint64_t tlo, thi;     // external states whose hash tag lies in [tlo, thi)
     int64_t* cnt;
     const int64_t* off;
     uint32_t* pkey;
@@ -1876,18 +1926,24 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
     __syncthreads();
     int64_t base = FILL ? a.off[row] : 0;
     int64_t cnt = 0;
-    for (int p0 = 0; p0 < a.P; p0 += kWave) {
+    int blk = -1, pb0 = 0, pb1 = 0;
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
         bool hit = false;
         int64_t idx = -1;
-        if (p < a.P) {
+        if (p < pb1) {
             const uint4 e = rec_load(a.deltas, p);
             uint32_t t[NW];
 #pragma unroll
             for (int i = 0; i < NW; i++) t[i] = w[i];
             if (apply_pattern<NW>(e, delta_nz(e), sq, t)) {
-                idx = hash_find<NW>(a.aslots, a.amask, a.apool, t);
-                hit = idx >= 0;
+                const uint64_t h = key_hash<NW>(t);
+                const uint64_t tag = h >> 32;
+                if (tag >= a.tlo && tag < a.thi) {
+                    idx = hash_find_h<NW>(a.aslots, a.amask, a.apool, t, h);
+                    hit = idx >= 0;
+                }
             }
         }
         const uint64_t bal = __ballot(hit);
@@ -1947,6 +2003,7 @@ struct Pt2NumArgs {
     int ldc;
     const double* E;
     double* part;
+    int carry;  // 1: continue the sums already in part (a later chunk)
 };
 
 // one wave per external state: numerators sum_i H_ai C_in over its pairs in
@@ -1964,7 +2021,12 @@ __global__ __launch_bounds__(64) void k_pt2_num(Pt2NumArgs a) {
     constexpr int U = 8;
     double ks[R], kc[R];
 #pragma unroll
-    for (int r = 0; r < R; r++) ks[r] = kc[r] = 0.0;
+    for (int r = 0; r < R; r++) {
+        const int c = lane + 64 * r;
+        const bool on = a.carry && c < a.nopt;
+        ks[r] = on ? a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 0] : 0.0;
+        kc[r] = on ? a.part[((int64_t)blockIdx.x * a.nopt + c) * 2 + 1] : 0.0;
+    }
     for (int64_t ia = blockIdx.x; ia < a.ns2; ia += gridDim.x) {
         uint32_t w[NW];
         key_load<NW>(a.akeys, ia, w);
@@ -2247,6 +2309,7 @@ void Engine::upload_tables() {
     upload(d_ladder_, L, stream_);
     upload(d_elem_, T_.elem, stream_);
     upload(d_deltas_, T_.deltas, stream_);
+    upload(d_doff_, T_.delta_off, stream_);
     upload(d_groups_, T_.groups, stream_);
     upload(d_fclist_, T_.fclist, stream_);
     upload(d_diag_, T_.diag_fcs, stream_);
@@ -2404,6 +2467,7 @@ void Engine::build_h_t(HStats& st) {
     a.slots = slots_.as<uint64_t>();
     a.mask = hmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
+    a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
     a.fclist = d_fclist_.as<uint32_t>();
     a.elem = d_elem_.as<ElemFC>();
@@ -2674,6 +2738,64 @@ void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std:
     }
 }
 
+void Engine::get_h_rows(const int64_t* rows, int64_t nr, std::vector<int64_t>& rowptr, std::vector<int32_t>& col,
+                        std::vector<double>& val, std::vector<double>& diag) {
+    HIPCHK(hipSetDevice(device_));
+    PoolScope ps(&pool_);
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    if (matfree_) throw std::runtime_error("the Hamiltonian is matrix-free: no stored entries to return");
+    std::vector<int32_t> perm, inv;
+    if (perm_on_) {
+        perm.resize(n_);
+        HIPCHK(hipMemcpyAsync(perm.data(), perm_.p, n_ * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        inv.resize(n_);
+        for (int64_t r = 0; r < n_; r++) inv[perm[r]] = (int32_t)r;
+    }
+    rowptr.assign(1, 0);
+    col.clear();
+    val.clear();
+    diag.assign(nr, 0.0);
+    std::vector<std::pair<int32_t, double>> row;
+    for (int64_t k = 0; k < nr; k++) {
+        const int64_t b = rows[k];
+        if (b < 0 || b >= n_) throw std::runtime_error("get_h_rows: row out of range");
+        const int64_t r = perm_on_ ? inv[b] : b;
+        int64_t se[2];
+        HIPCHK(hipMemcpyAsync(se, rowptr_.as<int64_t>() + r, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipMemcpyAsync(&diag[k], D_.as<double>() + r, sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        const int64_t cnt = se[1] - se[0];
+        std::vector<int32_t> c(cnt);
+        std::vector<double> v(cnt);
+        if (cnt) {
+            HIPCHK(hipMemcpyAsync(c.data(), col_.as<int32_t>() + se[0], cnt * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  stream_));
+            if (form_ == kFP64) {
+                HIPCHK(hipMemcpyAsync(v.data(), val_.as<double>() + se[0], cnt * sizeof(double), hipMemcpyDeviceToHost,
+                                      stream_));
+                HIPCHK(hipStreamSynchronize(stream_));
+            } else {
+                std::vector<float> h(cnt), l(cnt);
+                HIPCHK(hipMemcpyAsync(h.data(), hi_.as<float>() + se[0], cnt * sizeof(float), hipMemcpyDeviceToHost,
+                                      stream_));
+                HIPCHK(hipMemcpyAsync(l.data(), lo_.as<float>() + se[0], cnt * sizeof(float), hipMemcpyDeviceToHost,
+                                      stream_));
+                HIPCHK(hipStreamSynchronize(stream_));
+                for (int64_t t = 0; t < cnt; t++) v[t] = (double)h[t] + (double)l[t];
+            }
+        }
+        row.clear();
+        for (int64_t t = 0; t < cnt; t++) row.emplace_back(perm_on_ ? perm[c[t]] : c[t], v[t]);
+        std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
+        for (auto& e : row) {
+            col.push_back(e.first);
+            val.push_back(e.second);
+        }
+        rowptr.push_back((int64_t)col.size());
+    }
+}
+
 // Structural nonzeros of the reference's triplet list (Ham.cpp:237-326):
 // every pair whose quanta differ by an even L1 distance <= fcmax or an odd
 // one <= fcmax-1 gets a triplet, zero or not; after H + H^T that is n plus
@@ -2683,7 +2805,10 @@ int64_t Engine::structural_nnz() {
     PoolScope ps(&pool_);
     if (T_.ball.empty() || n_ == 0) return -1;
     HIPCHK(hipSetDevice(device_));
-    if (!d_ball_.p) upload(d_ball_, T_.ball, stream_);
+    if (!d_ball_.p) {
+        upload(d_ball_, T_.ball, stream_);
+        upload(d_boff_, T_.ball_off, stream_);
+    }
     HBuildArgs a{};
     a.n = n_;
     a.M = P_.M;
@@ -2692,6 +2817,7 @@ int64_t Engine::structural_nnz() {
     a.slots = slots_.as<uint64_t>();
     a.mask = hmask_;
     a.deltas = d_ball_.as<DeltaEnt>();
+    a.poff = d_boff_.as<uint32_t>();
     rowcnt_.ensure((n_ + 1) * sizeof(int64_t));
     a.rowcnt = rowcnt_.as<int64_t>();
     switch (KW_) {
@@ -2739,6 +2865,7 @@ void Engine::spmm_free_t(const double* X, int ldx, double* Y, int ldy, int b) {
     a.slots = slots_.as<uint64_t>();
     a.mask = hmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
+    a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
     a.fclist = d_fclist_.as<uint32_t>();
     a.elem = d_elem_.as<ElemFC>();
@@ -2978,7 +3105,7 @@ void Engine::gemm_profile_read(double out[3]) {
     out[2] = (double)prof_.gemm_calls;
 }
 
-void Engine::phase_stats(double out[14]) const {
+void Engine::phase_stats(double out[15]) const {
     out[0] = (double)last_h_.n;
     out[1] = (double)last_h_.nnz_offdiag;
     out[2] = (double)P_deltas_;
@@ -2993,6 +3120,7 @@ void Engine::phase_stats(double out[14]) const {
     out[11] = (double)last_pt2_nopt_;
     out[12] = (double)last_h_.probed_rows;
     out[13] = (double)last_h_.carried;
+    out[14] = (double)last_pt2_.chunks;
 }
 
 void Engine::spmm_profile_read(double out[3]) {
@@ -3915,8 +4043,10 @@ void Engine::set_vectors(const double* C, int ncol) {
 }
 
 // ---------------------------------------------------------------- heat bath
-template <int NW>
-void Engine::hb_select_t(double eps, HbStats& st) {
+// Candidate counts of every basis state for threshold eps (from cmax_):
+// candcnt_/candoff_ hold the counts and their exclusive prefix; returns the
+// total (HB.cpp:24-26, the loop over the |f|-sorted constants).
+int64_t Engine::hb_count(double eps) {
     const int64_t n = n_;
     const int nhb = (int)T_.hb_fc.size();
     candcnt_.ensure((n + 1) * sizeof(int64_t));
@@ -3933,15 +4063,30 @@ void Engine::hb_select_t(double eps, HbStats& st) {
     int64_t total = 0;
     HIPCHK(hipMemcpyAsync(&total, candoff_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    st.candidates = total;
+    return total;
+}
+
+// The distinct new states among the counted candidates whose key hash tag
+// (the hash's high 32 bits) lies in [tlo, thi), in first-discovery order:
+// newkeys_, nnew_, and the hash (nslots_, pool_keys_, pool_pos_) that finds
+// them.  The dedup pool starts at an estimate and grows fourfold while it
+// overflows; false (nothing kept) when it would need more than max_pool
+// entries.
+template <int NW>
+bool Engine::hb_gen_t(int64_t total, uint64_t tlo, uint64_t thi, uint64_t max_pool, HbStats& st) {
+    const int64_t n = n_;
     nnew_ = 0;
-    if (total == 0) return;
-    uint64_t ucap = (uint64_t)std::min<int64_t>(total, std::max<int64_t>(1 << 20, 32 * n));
+    const double frac = (double)(thi - tlo) / 4294967296.0;
+    uint64_t ucap = (uint64_t)std::min<double>((double)total,
+                                               std::max(1048576.0, frac * (double)std::min<int64_t>(total, 32 * n)));
     counters_.ensure(4 * sizeof(uint32_t));
-    for (int attempt = 0; attempt < 8; attempt++) {
+    for (int attempt = 0; attempt < 12; attempt++) {
         uint64_t cap = next_pow2(std::max<uint64_t>(2048, 2 * ucap));
-        if (cap > (1ull << 32)) throw std::runtime_error("heat-bath candidate set too large");
-        const uint32_t pool_cap = (uint32_t)std::min<uint64_t>(cap / 2, 0xffffffffull - 1);
+        if (cap > (1ull << 32)) return false;
+        // the pool holds at most max_pool states; an overflow at that size
+        // ends the attempt (the caller splits the range)
+        const bool capped = std::min<uint64_t>(cap / 2, 0xffffffffull - 1) >= max_pool;
+        const uint32_t pool_cap = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(cap / 2, 0xffffffffull - 1), max_pool);
         nslots_.ensure(cap * sizeof(uint64_t));
         pool_keys_.ensure((size_t)pool_cap * KW_ * 16);
         pool_rank_.ensure((size_t)pool_cap * sizeof(uint64_t));
@@ -3965,6 +4110,8 @@ void Engine::hb_select_t(double eps, HbStats& st) {
         a.pool_rank = pool_rank_.as<uint64_t>();
         a.counters = counters_.as<uint32_t>();
         a.pool_cap = pool_cap;
+        a.tlo = tlo;
+        a.thi = thi;
         k_hb_gen<NW><<<std::min<unsigned>(grid_for(n * 64, 256), 16384), 256, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
         uint32_t cnt[4];
@@ -3972,6 +4119,7 @@ void Engine::hb_select_t(double eps, HbStats& st) {
         HIPCHK(hipStreamSynchronize(stream_));
         if (cnt[1] & 1u) throw std::runtime_error("Error: a heat-bath state exceeds 255 quanta in one mode.");
         if (cnt[1] & 2u) {
+            if (capped) return false;
             ucap *= 4;
             continue;
         }
@@ -4006,10 +4154,20 @@ void Engine::hb_select_t(double eps, HbStats& st) {
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(stream_));
         nnew_ = u;
-        st.unique = u;
-        return;
+        st.unique += u;
+        return true;
     }
-    throw std::runtime_error("heat-bath selection: candidate pool kept overflowing");
+    return false;
+}
+
+template <int NW>
+void Engine::hb_select_t(double eps, HbStats& st) {
+    const int64_t total = hb_count(eps);
+    st.candidates = total;
+    nnew_ = 0;
+    if (total == 0) return;
+    if (!hb_gen_t<NW>(total, 0, uint64_t(1) << 32, uint64_t(1) << 32, st))
+        throw std::runtime_error("heat-bath selection: the new states do not fit one hash of 2^32 slots");
 }
 
 HbStats Engine::hb_select(const double* cmax_host, int nopt, double eps) {
@@ -4101,8 +4259,15 @@ void Engine::pool_stats(int64_t out[5]) const {
 // ---------------------------------------------------------------- PT2
 // Push form; returns false (nothing launched) when the pair list would not
 // fit in a quarter of the free device memory.
+// One chunk of the push form: every variational state i applies every
+// pattern it can use; a result a in the chunk's external states (hash tag in
+// [tlo, thi)) gives the pair (a, i).  The pairs, sorted by a (stably, so i
+// ascends within a), feed one wave per external state (k_pt2_num).  Returns
+// false, with nothing summed, when the chunk has more than max_pairs pairs
+// or their buffers cannot be allocated.
 template <int NW>
-bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st) {
+bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pairs, bool carry, uint64_t tlo,
+                        uint64_t thi, Pt2Stats& st) {
     const int64_t n = n_, ns2 = nnew_;
     Pt2PushArgs a{};
     a.n = n;
@@ -4114,35 +4279,38 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st) {
     a.apool = pool_keys_.as<uint4>();
     a.apos = pool_pos_.as<uint32_t>();
     a.deltas = d_deltas_.as<DeltaEnt>();
+    a.poff = d_doff_.as<uint32_t>();
+    a.tlo = tlo;
+    a.thi = thi;
     rowcnt_.ensure((n + 1) * sizeof(int64_t));
-    candoff_.ensure((n + 1) * sizeof(int64_t));
+    pairoff_.ensure((n + 1) * sizeof(int64_t));
     a.cnt = rowcnt_.as<int64_t>();
-    if (a.P == 0) return false;
     k_pt2_push<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
     k_fill<int64_t><<<1, 1, 0, stream_>>>(a.cnt + n, 1, 0);
     size_t tmp = 0;
-    HIPCHK(rocprim::exclusive_scan(nullptr, tmp, a.cnt, candoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+    HIPCHK(rocprim::exclusive_scan(nullptr, tmp, a.cnt, pairoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
                                    rocprim::plus<int64_t>(), stream_));
     scan_tmp_.ensure(tmp);
-    HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, a.cnt, candoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+    HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, a.cnt, pairoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
                                    rocprim::plus<int64_t>(), stream_));
     int64_t npairs = 0;
-    HIPCHK(hipMemcpyAsync(&npairs, candoff_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipMemcpyAsync(&npairs, pairoff_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    st.pairs = npairs;
-    size_t freeb = 0, totb = 0;
-    HIPCHK(hipMemGetInfo(&freeb, &totb));
     if (std::getenv("VHCI_DEBUG"))
-        std::fprintf(stderr, "[pt2] push form: %lld external states, %lld pairs, %.1f GB free\n", (long long)ns2,
-                     (long long)npairs, freeb / 1e9);
-    if ((double)npairs * 32.0 > 0.25 * (double)freeb) return false;
+        std::fprintf(stderr, "[pt2] chunk [%#llx, %#llx): %lld external states, %lld pairs\n", (unsigned long long)tlo,
+                     (unsigned long long)thi, (long long)ns2, (long long)npairs);
+    if (npairs > max_pairs) return false;
     DevBuf pk, pv, sk, sv, seg;
-    pk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
-    pv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
-    sk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
-    sv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
-    seg.ensure((ns2 + 1) * sizeof(int64_t));
-    a.off = candoff_.as<int64_t>();
+    try {
+        pk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
+        pv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
+        sk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
+        sv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
+        seg.ensure((ns2 + 1) * sizeof(int64_t));
+    } catch (const std::runtime_error&) {
+        return false;  // the chunk is split instead
+    }
+    a.off = pairoff_.as<int64_t>();
     a.pkey = pk.as<uint32_t>();
     a.pval = pv.as<uint64_t>();
     if (npairs > 0) {
@@ -4181,6 +4349,7 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st) {
     b.ldc = mixed ? nopt : xcols_;
     b.E = Ed;
     b.part = pt2_part_.as<double>();
+    b.carry = carry ? 1 : 0;
     if (mixed && nopt <= 128)
         k_pt2_num<NW, true, 2><<<nwaves, 64, 0, stream_>>>(b);
     else if (mixed)
@@ -4191,38 +4360,18 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st) {
         k_pt2_num<NW, false, 4><<<nwaves, 64, 0, stream_>>>(b);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));
+    st.pairs += npairs;
     return true;
 }
 
+// One chunk of the pull form (VHCI_PT2_PULL=1, or a table without
+// patterns): one wave per external state probes its patterns against the
+// variational hash.
 template <int NW>
-void Engine::pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::vector<double>& dE) {
-    const int64_t ns2 = nnew_;
-    dE.assign(nopt, 0.0);
-    st.ns2 = ns2;
-    if (ns2 == 0) return;
-    auto t0 = clk::now();
-    DevBuf Ed, out;
-    Ed.ensure(nopt * sizeof(double));
-    out.ensure(nopt * sizeof(double));
-    HIPCHK(hipMemcpyAsync(Ed.p, E.data(), nopt * sizeof(double), hipMemcpyHostToDevice, stream_));
+void Engine::pt2_pull_t(const double* Ed, int nopt, int nwaves, bool carry) {
     const bool mixed = form_ == kMixed;
-    if (mixed) {
-        C32_.ensure((size_t)n_ * nopt * sizeof(float));
-        k_to_f32<<<grid_for(n_ * nopt, 256), 256, 0, stream_>>>(X_.as<double>(), xcols_, C32_.as<float>(), nopt, n_,
-                                                               nopt);
-    }
-    int nwaves = (int)std::min<int64_t>(ns2, 256 * 32);
-    pt2_part_.ensure((size_t)nwaves * nopt * 2 * sizeof(double));
-    if (!std::getenv("VHCI_PT2_PULL") && pt2_push_t<NW>(Ed.as<double>(), nopt, nwaves, st)) {
-        k_pt2_reduce<<<grid_for(nopt, 64), 64, 0, stream_>>>(pt2_part_.as<double>(), nwaves, nopt, out.as<double>());
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(dE.data(), out.p, nopt * sizeof(double), hipMemcpyDeviceToHost, stream_));
-        HIPCHK(hipStreamSynchronize(stream_));
-        st.sum_ms = ms_since(t0);
-        return;
-    }
     Pt2Args a{};
-    a.ns2 = ns2;
+    a.ns2 = nnew_;
     a.M = P_.M;
     a.P = P_deltas_;
     a.nq = 256;
@@ -4232,6 +4381,7 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::ve
     a.vslots = slots_.as<uint64_t>();
     a.vmask = hmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
+    a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
     a.fclist = d_fclist_.as<uint32_t>();
     a.elem = d_elem_.as<ElemFC>();
@@ -4242,18 +4392,112 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::ve
     a.C = X_.as<double>();
     a.C32 = C32_.as<float>();
     a.ldc = mixed ? nopt : xcols_;
-    a.E = Ed.as<double>();
+    a.E = Ed;
     a.part = pt2_part_.as<double>();
+    a.carry = carry ? 1 : 0;
     if (mixed)
         k_pt2<NW, true><<<nwaves, 64, 0, stream_>>>(a);
     else
         k_pt2<NW, false><<<nwaves, 64, 0, stream_>>>(a);
     HIPCHK(hipGetLastError());
-    k_pt2_reduce<<<grid_for(nopt, 64), 64, 0, stream_>>>(pt2_part_.as<double>(), nwaves, nopt, out.as<double>());
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dE.data(), out.p, nopt * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HIPCHK(hipStreamSynchronize(stream_));
-    st.sum_ms = ms_since(t0);
+}
+
+// DoPT2 over the external space in chunks.  The space is split by the hash
+// tag of its states (high 32 bits of the key hash): a chunk is a tag range,
+// and its states, their dedup hash and their (a, i) pairs are all that is
+// held at once.  The ranges start as one; a range whose states need more
+// than the state budget, or whose pairs exceed the pair budget, is halved
+// (the lower half first), so the chunks are the same on every run with the
+// same budgets.  Every chunk continues the same per-wave compensated sums,
+// which are reduced once at the end.
+template <int NW>
+void Engine::pt2_t(const std::vector<double>& E, int nopt, double eps2, Pt2Stats& st, std::vector<double>& dE) {
+    dE.assign(nopt, 0.0);
+    auto t0 = clk::now();
+    HbStats hs;
+    const int64_t total = hb_count(eps2);
+    hs.candidates = total;
+    st.candidates = total;
+    nnew_ = 0;
+    int64_t max_pairs = kPt2ChunkPairs, max_states = kPt2ChunkStates;
+    if (const char* e = std::getenv("VHCI_PT2_CHUNK_PAIRS")) max_pairs = std::max<int64_t>(1, std::atoll(e));
+    if (const char* e = std::getenv("VHCI_PT2_CHUNK_STATES")) max_states = std::max<int64_t>(1, std::atoll(e));
+    const bool pull = std::getenv("VHCI_PT2_PULL") != nullptr || P_deltas_ == 0;
+    DevBuf Ed, out;
+    Ed.ensure(nopt * sizeof(double));
+    out.ensure(nopt * sizeof(double));
+    HIPCHK(hipMemcpyAsync(Ed.p, E.data(), nopt * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (form_ == kMixed) {
+        C32_.ensure((size_t)n_ * nopt * sizeof(float));
+        k_to_f32<<<grid_for(n_ * nopt, 256), 256, 0, stream_>>>(X_.as<double>(), xcols_, C32_.as<float>(), nopt, n_,
+                                                               nopt);
+    }
+    double sel_ms = ms_since(t0), sum_ms = 0;
+    int nwaves = 0;
+    bool summed = false;
+    std::vector<std::pair<uint64_t, uint64_t>> work{{0, uint64_t(1) << 32}};
+    auto split = [&](uint64_t lo, uint64_t hi) {
+        if (hi - lo < 2) throw std::runtime_error("PT2: one hash tag holds more external states or pairs than a chunk");
+        const uint64_t mid = lo + (hi - lo) / 2;
+        work.push_back({mid, hi});
+        work.push_back({lo, mid});
+    };
+    while (total > 0 && !work.empty()) {
+        const auto [lo, hi] = work.back();
+        work.pop_back();
+        auto t1 = clk::now();
+        const int64_t u0 = hs.unique;
+        if (!hb_gen_t<NW>(total, lo, hi, (uint64_t)max_states, hs)) {
+            sel_ms += ms_since(t1);
+            split(lo, hi);
+            continue;
+        }
+        sel_ms += ms_since(t1);
+        const int64_t ns2 = nnew_;
+        if (ns2 == 0) continue;
+        if (nwaves == 0) {
+            // one chunk: as many waves as states, up to 8192 (the grid of
+            // the unchunked sum); several: 8192
+            nwaves = work.empty() ? (int)std::min<int64_t>(ns2, 256 * 32) : 256 * 32;
+            pt2_part_.ensure((size_t)nwaves * nopt * 2 * sizeof(double));
+        }
+        auto t2 = clk::now();
+        if (pull) {
+            pt2_pull_t<NW>(Ed.as<double>(), nopt, nwaves, summed);
+        } else if (!pt2_push_t<NW>(Ed.as<double>(), nopt, nwaves, max_pairs, summed, lo, hi, st)) {
+            sum_ms += ms_since(t2);
+            hs.unique = u0;
+            split(lo, hi);
+            continue;
+        }
+        sum_ms += ms_since(t2);
+        summed = true;
+        st.ns2 += ns2;
+        st.chunks++;
+    }
+    if (summed) {
+        auto t3 = clk::now();
+        k_pt2_reduce<<<grid_for(nopt, 64), 64, 0, stream_>>>(pt2_part_.as<double>(), nwaves, nopt, out.as<double>());
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(dE.data(), out.p, nopt * sizeof(double), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        sum_ms += ms_since(t3);
+    }
+    hs.ms = sel_ms;
+    last_hb_ = hs;
+    st.select_ms = sel_ms;
+    st.sum_ms = sum_ms;
+}
+
+// the eigensolver's work blocks (the Krylov basis, its products, the
+// Rayleigh-Ritz buffers) go back to the device before PT2; the next
+// eigensolve allocates them again
+void Engine::release_solver_work() {
+    for (DevBuf* b : {&V_, &W_, &HX_, &Xw_, &T_b_, &G_, &S_, &part_, &X32_, &stage_in_, &stage_in32_, &stage_out_,
+                      &otmp_, &Gw_, &tsm_part_, &rr_part_})
+        b->release();
+    pool_.trim();
 }
 
 Pt2Stats Engine::pt2(const std::vector<double>& E, int nopt, double eps2, std::vector<double>& dE) {
@@ -4261,24 +4505,27 @@ Pt2Stats Engine::pt2(const std::vector<double>& E, int nopt, double eps2, std::v
     HIPCHK(hipSetDevice(device_));
     if (nopt < 1 || nopt > xcols_ || (int)E.size() < nopt) throw std::runtime_error("PT2: bad number of states");
     if (nopt > 256) throw std::runtime_error("PT2: at most 256 states");
-    auto t0 = clk::now();
-    HbStats hs = hb_select(nullptr, nopt, eps2);
+    const int64_t n = n_;
+    if (xrows_ != n) throw std::runtime_error("no resident eigenvectors for this basis");
+    release_solver_work();
+    cmax_.ensure(n * sizeof(double));
+    k_cmax<<<grid_for(n, 256), 256, 0, stream_>>>(n, X_.as<double>(), xcols_, nopt, cmax_.as<double>());
+    HIPCHK(hipGetLastError());
     Pt2Stats st;
-    st.candidates = hs.candidates;
-    st.select_ms = ms_since(t0);
     switch (KW_) {
-        case 1: pt2_t<4>(E, nopt, st, dE); break;
-        case 2: pt2_t<8>(E, nopt, st, dE); break;
-        case 3: pt2_t<12>(E, nopt, st, dE); break;
-        default: pt2_t<16>(E, nopt, st, dE); break;
+        case 1: pt2_t<4>(E, nopt, eps2, st, dE); break;
+        case 2: pt2_t<8>(E, nopt, eps2, st, dE); break;
+        case 3: pt2_t<12>(E, nopt, eps2, st, dE); break;
+        default: pt2_t<16>(E, nopt, eps2, st, dE); break;
     }
     last_pt2_ = st;
     last_pt2_n_ = n_;
     last_pt2_nopt_ = nopt;
     if (std::getenv("VHCI_DEBUG"))
-        std::fprintf(stderr, "[pt2] n %lld candidates %lld external %lld pairs %lld select %.1f ms sum %.1f ms\n",
-                     (long long)n_, (long long)st.candidates, (long long)st.ns2, (long long)st.pairs, st.select_ms,
-                     st.sum_ms);
+        std::fprintf(stderr,
+                     "[pt2] n %lld candidates %lld external %lld pairs %lld chunks %d select %.1f ms sum %.1f ms\n",
+                     (long long)n_, (long long)st.candidates, (long long)st.ns2, (long long)st.pairs, st.chunks,
+                     st.select_ms, st.sum_ms);
     return st;
 }
 

@@ -119,6 +119,7 @@ struct Pt2Stats {
     int64_t ns2 = 0;
     int64_t pairs = 0;  // (external, variational) pairs of the push form
     int64_t candidates = 0;
+    int chunks = 0;     // hash-tag ranges the external space was summed in
     double select_ms = 0, sum_ms = 0;
 };
 
@@ -152,6 +153,11 @@ class Engine {
     // copy of the device CSR (off-diagonal part, stored order) and diagonal
     void get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std::vector<double>& val,
                std::vector<double>& diag);
+
+    // the same for the basis rows `rows` only (CSR over the nr rows,
+    // columns in basis order ascending; diag[nr])
+    void get_h_rows(const int64_t* rows, int64_t nr, std::vector<int64_t>& rowptr, std::vector<int32_t>& col,
+                    std::vector<double>& val, std::vector<double>& diag);
 
     // SparseDiagonalize (Ham.cpp:347-370): nev lowest eigenpairs.  Resident
     // Ritz vectors seed the next call (warm start) after the basis grows.
@@ -194,6 +200,11 @@ class Engine {
     static constexpr int64_t kReorderMin = 16384;
     // entries per call of the row sort of the H build (rocPRIM counts in 32 bits)
     static constexpr int64_t kSortSlice = int64_t(1) << 30;
+    // PT2 chunk budgets (VHCI_PT2_CHUNK_PAIRS / VHCI_PT2_CHUNK_STATES
+    // override them): (a, i) pairs (24 bytes each with the sort's copies)
+    // and external states (the dedup pool) one chunk may hold
+    static constexpr int64_t kPt2ChunkPairs = int64_t(1) << 30;
+    static constexpr int64_t kPt2ChunkStates = int64_t(1) << 28;
     // per-launch timing of the b-column products that the solver issues:
     // b > 0 starts (and clears) collection, b = 0 stops it.  The read gives
     // total ms, total algorithmic bytes and the launch count since the start.
@@ -206,7 +217,7 @@ class Engine {
     // heat-bath selection (the PT2 one after pt2()): candidates, unique, ms;
     // last PT2: external states, (external, variational) pairs, sum ms; key
     // bytes; PT2 states
-    void phase_stats(double out[14]) const;
+    void phase_stats(double out[15]) const;
     // algorithmic bytes of one product of b vectors: stored entries (value +
     // column), row pointers, the diagonal, X read once and Y written once
     double spmm_bytes(int b, int mode) const;
@@ -234,14 +245,21 @@ class Engine {
     void random_cm(double* V, int64_t ld, int k, uint64_t seed);
     void davidson(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st);
     void krylov_schur(int nev, double tol, int maxit, std::vector<double>& E, EigStats& st);
+    int64_t hb_count(double eps);
+    template <int NW>
+    bool hb_gen_t(int64_t total, uint64_t tlo, uint64_t thi, uint64_t max_pool, HbStats& st);
     template <int NW>
     void hb_select_t(double eps, HbStats& st);
     template <int NW>
     void build_h_t(HStats& st);
     template <int NW>
-    void pt2_t(const std::vector<double>& E, int nopt, Pt2Stats& st, std::vector<double>& dE);
+    void pt2_t(const std::vector<double>& E, int nopt, double eps2, Pt2Stats& st, std::vector<double>& dE);
     template <int NW>
-    bool pt2_push_t(const double* Ed, int nopt, int nwaves, Pt2Stats& st);
+    bool pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pairs, bool carry, uint64_t tlo,
+                    uint64_t thi, Pt2Stats& st);
+    template <int NW>
+    void pt2_pull_t(const double* Ed, int nopt, int nwaves, bool carry);
+    void release_solver_work();
 
     Problem P_;
     Tables T_;
@@ -258,6 +276,7 @@ class Engine {
     // tables
     DevBuf d_ladder_, d_elem_, d_deltas_, d_groups_, d_fclist_, d_diag_, d_freq_;
     DevBuf d_hbfc_, d_hbvaroff_, d_hbvars_, d_ball_;
+    DevBuf d_doff_, d_boff_;  // block offsets of the pattern table and the ball
     int P_deltas_ = 0;
     // Hamiltonian
     bool h_valid_ = false;
@@ -293,6 +312,7 @@ class Engine {
     DevBuf pool_pos_;     // pool entry -> position in first-discovery order
     // PT2 work
     DevBuf s2keys_, pt2_part_, C32_;
+    DevBuf pairoff_;  // PT2 push: prefix of the pair counts per variational state
     double last_spmm_ms_ = 0;
     struct SpmmProf {
         int b = 0;

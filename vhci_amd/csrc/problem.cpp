@@ -243,6 +243,27 @@ void build_ladder(int qmax, std::vector<double>& L) {
             }
 }
 
+void order_patterns(std::vector<DeltaEnt>& v, int M, std::vector<uint32_t>& off) {
+    // block of a pattern: 0 when no delta is negative, else 1 + the lowest
+    // mode with a negative delta
+    auto block = [](const DeltaEnt& e) {
+        const int nz = (int)(e.meta >> 28);
+        int lo = -1;
+        for (int t = 0; t < nz; t++)
+            if (e.delta[t] < 0 && (lo < 0 || e.mode[t] < lo)) lo = e.mode[t];
+        return lo + 1;
+    };
+    std::vector<std::vector<DeltaEnt>> b((size_t)M + 1);
+    for (const DeltaEnt& e : v) b[block(e)].push_back(e);
+    off.assign((size_t)M + 2, 0);
+    v.clear();
+    for (int k = 0; k <= M; k++) {
+        off[k] = (uint32_t)v.size();
+        v.insert(v.end(), b[k].begin(), b[k].end());
+    }
+    off[(size_t)M + 1] = (uint32_t)v.size();
+}
+
 void build_tables(const Problem& P, Tables& T) {
     T = Tables();
     // element FCs: Quartic, Sextic, Cubic, Quintic
@@ -360,6 +381,8 @@ void build_tables(const Problem& P, Tables& T) {
             rec(0, R);
         }
     }
+    order_patterns(T.deltas, P.M, T.delta_off);
+    order_patterns(T.ball, P.M, T.ball_off);
     // heat-bath variants in sorted-FC order (HB.cpp:27-164): delta_t runs
     // -k..k in steps of 2 with the last mode fastest; all-zero is skipped.
     T.hb_varoff.push_back(0);

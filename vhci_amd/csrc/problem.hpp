@@ -105,7 +105,16 @@ struct Tables {
     // exceed kMaxBall entries
     std::vector<DeltaEnt> ball;
     int max_delta = 0;                 // largest |delta| of any pattern
+    // deltas and ball are ordered in blocks: block 0 holds the patterns that
+    // lower no mode, block 1 + m those whose lowest lowered mode is m (a
+    // row can use them only when it has quanta in mode m).  Block b is
+    // [off[b], off[b + 1]); M + 2 offsets each.
+    std::vector<uint32_t> delta_off, ball_off;
 };
+
+// Stable reorder of patterns into the blocks described above; off receives
+// the M + 2 block offsets.
+void order_patterns(std::vector<DeltaEnt>& v, int M, std::vector<uint32_t>& off);
 
 void build_tables(const Problem& P, Tables& T);
 constexpr size_t kMaxBall = 500000;

@@ -118,6 +118,21 @@ def _iijk_pool(M):
     return out
 
 
+def _semidiag(M, order):
+    """Constants of one order over at most two distinct modes, i^a j^b with
+    a >= b: i^order, then i^a j^b for i != j (a > b) or i < j (a == b)."""
+    out = []
+    for a in range(order, (order - 1) // 2, -1):
+        b = order - a
+        if b == 0:
+            out += [(i,) * order for i in range(M)]
+        elif a > b:
+            out += [(i,) * a + (j,) * b for i in range(M) for j in range(M) if i != j]
+        else:
+            out += [(i,) * a + (j,) * b for i in range(M) for j in range(i + 1, M)]
+    return out
+
+
 def write_input(path_or_none, comment, eps1, nstate, pt2, eps2, ntot, freqs, caps, fcs):
     lines = ["Comment: " + comment,
              "HCI_Eps: %s" % repr(float(eps1)),
@@ -139,12 +154,15 @@ def write_input(path_or_none, comment, eps1, nstate, pt2, eps2, ntot, freqs, cap
 
 
 def field(M, flo, fhi, seed, cubic=(1.0, 300.0), quartic=(0.1, 60.0), n_extra_iijk=0, cutoff=0.0,
-          cubic_pool=None, quartic_pool=None, positive_iiii=False, freq_scale=0.0):
+          cubic_pool=None, quartic_pool=None, positive_iiii=False, freq_scale=0.0, higher=()):
     """Frequencies uniform in [flo, fhi]; one log-uniform magnitude with a
     random sign per constant of the pools.  freq_scale = p > 0 scales the
     upper end of each constant's range by (w / fhi)^p, w the geometric mean
     frequency of its modes (a range that would end below its lower end ends
-    just above it)."""
+    just above it).  Cubic and quartic constants below `cutoff` are dropped.
+    higher: (pool, range) pairs of further constants (quintic, sextic) drawn
+    after the quartics with no cutoff; their all-diagonal even constants are
+    positive."""
     import math
     rng = Rng(seed)
     freqs = [rng.uniform(flo, fhi) for _ in range(M)]
@@ -167,6 +185,12 @@ def field(M, flo, fhi, seed, cubic=(1.0, 300.0), quartic=(0.1, 60.0), n_extra_ii
         if positive_iiii and t[0] == t[1] == t[2] == t[3]:
             v = abs(v)
         if abs(v) >= cutoff:
+            fcs.append((t, v))
+    for pool, rr in higher:
+        for t in pool:
+            v = draw(t, rr)
+            if len(t) % 2 == 0 and len(set(t)) == 1:
+                v = abs(v)
             fcs.append((t, v))
     if n_extra_iijk:
         pool = _iijk_pool(M)
@@ -194,6 +218,12 @@ def config_input(c, path=None, **override):
         freqs, fcs = field(M, 150.0, 3200.0, seed, cubic=(0.5, 60.0), quartic=(0.25, 12.0), cutoff=0.5,
                            quartic_pool=_semidiag_quartic(M) + _iijk_pool(M), positive_iiii=True, freq_scale=1.0)
         args = dict(eps1=1.0, nstate=128, pt2=True, eps2=0.2, ntot=2, caps=[6] * M)
+    elif c == 4:
+        M = 60
+        freqs, fcs = field(M, 100.0, 3300.0, seed, cubic=(0.5, 60.0), quartic=(0.25, 12.0), cutoff=0.5,
+                           quartic_pool=_semidiag_quartic(M) + _iijk_pool(M)[::2], positive_iiii=True,
+                           freq_scale=1.0, higher=((_semidiag(M, 5), (0.01, 5.0)), (_semidiag(M, 6), (0.01, 5.0))))
+        args = dict(eps1=0.5, nstate=200, pt2=True, eps2=0.05, ntot=2, caps=[6] * M)
     else:
         raise ValueError("unknown workload %r" % c)
     args.update(override)
