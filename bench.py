@@ -1,40 +1,42 @@
 """Benchmark of the heat-bath VCI engine on one MI355X.
 
-Workload: the synthetic 15-mode cubic+quartic field (680 cubic, 600 quartic
-force constants; Ntot 2, caps 6; eps1 1.0, Nstate 100, PT2 with eps2 0.01),
-generated with a fixed seed.  One step = one whole solve: heat-bath
-iterations (select -> build H -> diagonalize) until the basis grows by at
-most 1 %, then Epstein-Nesbet PT2.  Inputs are generated on the host before
-the timed region; each step creates its device context (tables and initial
-basis upload) inside the timed region.
+Headline workload (the largest single-GPU configuration): the synthetic
+48-mode cubic+quartic field (19,600 cubic and 42,212 quartic constants with
+frequency-scaled log-uniform magnitudes; Ntot 2, caps 6; eps1 1.0, Nstate
+128, PT2 with eps2 0.2), generated with a fixed seed (vhci_amd/synth.py).
+Its basis grows 1225 -> 1,003,305 -> 1,548,210 -> 1,598,434 states with
+about 1.9e9 stored entries, and PT2 runs over 3.2 M external states.  One
+step = one whole solve: heat-bath iterations (select -> build H ->
+diagonalize) until the basis grows by at most 1 %, then Epstein-Nesbet PT2.
+Inputs are generated on the host before the timed region; each step
+creates its device context (tables and initial basis upload) inside it.
 
 Both precision forms run: FP64 and mixed (FP32 H/SpMM with FP64
 refinement, FP64 PT2 denominators and Kahan sums).  The headline value is
 the mixed form's solves/s when every energy is within 0.01 cm^-1 of FP64,
 the selected basis is identical and the mixed form is faster, else the FP64
-figure.
+figure.  The 15-mode workload (config 1) runs once per form beside it.
 
-The serving entry runs a stream of small requests in the style of the
-serving workload (seeded fields of 8-14 modes, eps1 1 or 2, Nstate 20-200,
-eps2 0, 0.01 or 0.1): each request alone first (the solo FP64 rate), then
-Poisson arrivals at 1.2x that rate through a four-slot server, once per
-precision form, with p50/p99 latency from submission to result and a check
-that every result equals its solo run; then the capacity: all requests at
-once through four- and eight-slot FP64 servers.
+The serving entry runs a stream of requests drawn from the serving
+workload's distribution (8-36 modes, eps1 0.5, 1 or 2, Nstate 20-200, eps2
+0, 0.01 or 0.1; --serve requests): each request alone first (the solo FP64
+rate), then Poisson arrivals at 1.2x that rate through a four-slot server,
+once per precision form, with p50/p99 latency from submission to result and
+a check that every result equals its solo run; then the capacity: all
+requests at once through a four-slot FP64 server.
 
 The reference-runs entry solves the inputs of the reference's own logged
 runs (ClO2 and acetonitrile examples, the ethylene oxide run) and reports
-each whole-program time beside the wall time its log prints; the
-large-field entry runs the 48-mode naphthalene field at the large
-workload's parameters (a few million states).
+each whole-program time beside the wall time its log prints.
 
-The roofline entry is for the FP64 eigensolver's block SpMM (k_spmm_lane),
-every launch of the timed solves bracketed by HIP events on the context's
-stream; its HBM bytes per launch come from two rocprofv3 counter passes
-(FETCH_SIZE, WRITE_SIZE) scaled by the same counters' reading of a 1 GiB
-copy (--no-traffic skips them).  The CPU baseline is the oracle's pair-evaluation rate on
-a bounded sample, extrapolated to the whole solve with the run's exact pair
-counts.
+The roofline entry is for the dominant kernel of the headline solve, the
+FP64 eigensolver's block SpMM (k_spmm_lane), every launch of the timed
+solves bracketed by HIP events on the context's stream; its HBM bytes per
+launch come from two rocprofv3 counter passes (FETCH_SIZE, WRITE_SIZE) over
+one solve, scaled by the same counters' reading of a 1 GiB copy
+(--no-traffic skips them).  The CPU baseline is the oracle's
+pair-evaluation rate on a bounded sample of the headline run's states,
+extrapolated to the whole solve with the run's exact pair counts.
 """
 import argparse
 import json
@@ -129,7 +131,7 @@ def spmm_kernel_name(bs, mode, lanes=64):
     return "k_spmm_lane<%d, %d, %d>" % (bs, mode, lanes)
 
 
-def traffic_pass(counters, kernel, timeout_s=170):
+def traffic_pass(counters, kernel, workload, timeout_s=420):
     """One rocprofv3 counter pass (a single FP64 solve of this workload plus
     the 1 GiB calibration copy) in a child process; returns, per counter, the
     mean value per dispatch of `kernel` and of the copy kernel k_copy16, and
@@ -141,10 +143,12 @@ def traffic_pass(counters, kernel, timeout_s=170):
     import tempfile
 
     d = tempfile.mkdtemp(prefix="vhci_pmc_")
-    cmd = ["rocprofv3", "--pmc"] + counters + ["--output-format", "csv", "-d", d, "-o", "pmc", "--",
+    cmd = ["rocprofv3", "--pmc"] + counters + ["--kernel-include-regex", "k_spmm_lane|k_copy16",
+                                               "--output-format", "csv", "-d", d, "-o", "pmc", "--",
                                                sys.executable, "-u", os.path.abspath(__file__), "--steps", "1",
                                                "--warmup", "0", "--forms", "fp64", "--no-cpu-baseline",
-                                               "--serve", "0", "--no-reference-runs", "--no-traffic"]
+                                               "--serve", "0", "--no-reference-runs", "--no-traffic",
+                                               "--no-secondary", "--workload", workload]
     print("[bench] counter pass: %s" % " ".join(cmd), file=sys.stderr, flush=True)
     log = os.path.join(d, "pass.log")
     with open(log, "w") as lf:
@@ -180,7 +184,7 @@ def traffic_pass(counters, kernel, timeout_s=170):
     return out, nk
 
 
-def measure_traffic(kernel):
+def measure_traffic(kernel, workload):
     """HBM bytes per launch of `kernel`.  traffic: a FETCH_SIZE pass and a
     WRITE_SIZE pass (KiB per dispatch), each scaled by what the same counter
     reads for the 1 GiB copy (1 GiB fetched, 1 GiB written).  FETCH_SIZE
@@ -189,10 +193,10 @@ def measure_traffic(kernel):
     copy-scaled figure overstates a kernel whose reads are 64-byte gathers;
     a third pass counts the read requests by size (32, 64, 128 bytes) and
     gives the read bytes directly."""
-    f, nk = traffic_pass(["FETCH_SIZE"], kernel)
-    w, _ = traffic_pass(["WRITE_SIZE"], kernel)
+    f, nk = traffic_pass(["FETCH_SIZE"], kernel, workload)
+    w, _ = traffic_pass(["WRITE_SIZE"], kernel, workload)
     sizes = ["TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum"]
-    r, _ = traffic_pass(sizes, kernel)
+    r, _ = traffic_pass(sizes, kernel, workload)
     gib_kib = float(1 << 20)
     fk, fc = f["FETCH_SIZE"]
     wk, wc = w["WRITE_SIZE"]
@@ -226,10 +230,10 @@ def cpu_baseline(ctx, r, problem_text, threads):
         os.unlink(path)
     q = ctx.basis()
     ns2 = r["ns2"]
-    nA = min(ns2, 40000)
+    nA = min(ns2, max(1000, int(4e9 / max(1, q.shape[0]))))
     qA = ctx.new_states(ns2)[:nA] if nA else np.zeros((0, q.shape[1]), dtype=np.int32)
     n = q.shape[0]
-    rows = min(n, max(1, int(2.4e10 / n)))
+    rows = min(n, max(1, int(3e9 / n)))
     nopt = min(o.neig, n)
     rate_h, rate_p, raw = o.sample_rates(q, rows, qA, nopt)
     sizes = ctx.iteration_sizes()
@@ -250,14 +254,15 @@ def cpu_baseline(ctx, r, problem_text, threads):
     }
 
 
-SERVE_RANGES = dict(m_range=(8, 14), eps1_set=(1.0, 2.0), nstate_range=(20, 200), eps2_set=(0.0, 0.01, 0.1))
+SERVE_RANGES = dict(m_range=(8, 36), eps1_set=(0.5, 1.0, 2.0), nstate_range=(20, 200), eps2_set=(0.0, 0.01, 0.1))
 
 
-def serving(nreq, slots, load, seed=7):
-    """Serving stream in the style of workload c3 with narrowed ranges
-    (SERVE_RANGES): each request alone on a one-slot server sets the solo
-    FP64 rate; then Poisson arrivals at `load` x that rate go through a
-    `slots`-slot server, once per form.  Latency = submission to result."""
+def serving(nreq, slots, load, seed=7, wide=False):
+    """Serving stream of the serving workload's distribution (SERVE_RANGES;
+    requests 0..nreq-1 of its seeded sequence): each request alone on a
+    one-slot server sets the solo FP64 rate; then Poisson arrivals at `load`
+    x that rate go through a `slots`-slot server, once per form.  Latency =
+    submission to result."""
     import threading
 
     texts = [synth.request_input(r, **SERVE_RANGES) for r in range(nreq)]
@@ -312,7 +317,7 @@ def serving(nreq, slots, load, seed=7):
     # capacity: every request submitted at once (FP64), served rate =
     # requests / makespan; the arrival stream above is paced below it
     out["saturated_fp64"] = {}
-    for ns in sorted({slots, 2 * slots}):
+    for ns in sorted({slots} | ({2 * slots} if wide else set())):
         srv = vhci_amd.Server(slots=ns)
         ids = [srv.submit(t) for t in texts]
         res = []
@@ -402,23 +407,67 @@ def large_field_run():
     return out
 
 
+WORKLOADS = {
+    "c2": ("synthetic 48-mode cubic+quartic field, 19,600 cubic + 42,212 quartic FCs (log-uniform |f| with "
+           "frequency-scaled ranges, |f| >= 0.5), Ntot 2, caps 6, eps1 1.0, Nstate 128, PT2 eps2 0.2 "
+           "(seed 0x5648434902)", dict(modes=48, nstate=128, eps1=1.0, eps2=0.2)),
+    "c1": ("synthetic 15-mode cubic+quartic field, 680 cubic + 600 quartic FCs, Ntot 2, caps 6, eps1 1.0, "
+           "Nstate 100, PT2 eps2 0.01 (seed 0x5648434901)", dict(modes=15, nstate=100, eps1=1.0, eps2=0.01)),
+}
+
+
+def form_summary(lat, ctx, r, sp):
+    return {"solves_per_s": 1.0 / float(np.mean(lat)), "p50_s": percentile(lat, 50), "p99_s": percentile(lat, 99),
+            "nvar": r["nvar"], "ns2": r["ns2"], "hb_iterations": r["hb_iterations"], "timings_ms": r["timings"],
+            "basis_sizes": ctx.iteration_sizes(), "E0": float(r["E"][0]),
+            "solver_spmm": {"launches": sp["launches"], "ms": sp["ms"],
+                            "GBs": sp["bytes"] / sp["ms"] / 1e6 if sp["ms"] > 0 else None}}
+
+
+def compare_forms(r64, r32):
+    err = float(np.max(np.abs(r64["E"] - r32["E"])))
+    err_var = float(np.max(np.abs(r64["E_var"] - r32["E_var"])))
+    same = r64["checksum"] == r32["checksum"] and r64["nvar"] == r32["nvar"] and r64["ns2"] == r32["ns2"]
+    return {"max_abs_err_cm": err, "max_abs_err_var_cm": err_var, "basis_identical": same}
+
+
+def secondary_run(name):
+    """One solve per form of another workload (no warmup: the kernels are
+    loaded by then), for continuity with earlier measurements."""
+    text = synth.config_input(int(name[1:]))
+    problem = vhci_amd.Problem(text=text)
+    out = {"workload": WORKLOADS[name][0]}
+    res = {}
+    for fname, f in (("fp64", vhci_amd.FP64), ("mixed", vhci_amd.MIXED)):
+        lat, (ctx, r), sp = run_form(problem, f, 1, 0, 8)
+        out[fname] = form_summary(lat, ctx, r, sp)
+        res[fname] = r
+        ctx.close()
+    out["mixed_vs_fp64"] = compare_forms(res["fp64"], res["mixed"])
+    out["mixed_vs_fp64"]["speedup"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--mixed-steps", type=int, default=1, help="timed solves of the mixed form (after the FP64 ones)")
+    ap.add_argument("--workload", default="c2", choices=sorted(WORKLOADS), help="headline workload")
     ap.add_argument("--forms", default="both", choices=["both", "fp64", "mixed"])
-    ap.add_argument("--spmm-reps", type=int, default=20)
+    ap.add_argument("--spmm-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--serve", type=int, default=32, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve", type=int, default=24, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving stream (one JSON line)")
     ap.add_argument("--no-reference-runs", action="store_true",
                     help="skip the reference's logged runs (acetonitrile, ethylene oxide, ClO2)")
+    ap.add_argument("--no-secondary", action="store_true", help="skip the 15-mode workload's solves")
+    ap.add_argument("--large-field", action="store_true",
+                    help="also run the 48-mode naphthalene field at the large workload's parameters")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=1.2, help="offered load / solo FP64 rate")
-    ap.add_argument("--traffic", action="store_true",
-                    help="measure the SpMM kernel's HBM traffic (the default; rocprofv3 counter passes in child "
-                         "processes, run before this process uses the GPU)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the counter passes (traffic is then null)")
     ap.add_argument("--large-field-only", action="store_true",
                     help="only the 48-mode naphthalene run at the large workload's parameters (one JSON line)")
@@ -433,21 +482,28 @@ def main():
     if args.large_field_only:
         print(json.dumps(large_field_run()), flush=True)
         return
-    text = synth.config_input(1)
+    if args.serve_only:
+        print(json.dumps(serving(args.serve, args.serve_slots, args.serve_load, wide=True)), flush=True)
+        return
+    wname = args.workload
+    cnum = int(wname[1:])
+    text = synth.config_input(cnum)
     problem = vhci_amd.Problem(text=text)
     bs = int(os.environ.get("VHCI_KS_BLOCK", "8"))
-    kname = spmm_kernel_name(bs, 0)
+    # rows of at least 128 entries on average take a whole wave (64 lanes):
+    # the case of both workloads' final matrices
+    kname = spmm_kernel_name(bs, 0, 64)
     traffic = None
     if not args.no_traffic and not args.profile_spmm:
         # the counter passes run before this process touches the GPU; a pass
         # that fails (no rocprofv3, a refused counter) leaves traffic null
         try:
-            traffic = measure_traffic(kname)
+            traffic = measure_traffic(kname, wname)
         except Exception as e:  # noqa: BLE001
             print("[bench] traffic not measured: %s" % e, file=sys.stderr, flush=True)
             traffic = None
     if args.profile_spmm:
-        ctx = vhci_amd.Context(vhci_amd.Problem(text=synth.config_input(1, pt2=False)), vhci_amd.FP64)
+        ctx = vhci_amd.Context(vhci_amd.Problem(text=synth.config_input(cnum, pt2=False)), vhci_amd.FP64)
         ctx.run()
         ms, by, nl = ctx.time_spmm(bs, args.spmm_reps)
         cms = ctx.copy_probe(1 << 30)
@@ -460,31 +516,24 @@ def main():
         forms = {args.forms: forms[args.forms]}
     res = {}
     for name, f in forms.items():
-        res[name] = run_form(problem, f, args.steps, args.warmup, bs)
+        if name == "mixed" and "fp64" in forms:
+            res[name] = run_form(problem, f, args.mixed_steps, 0, bs)
+        else:
+            res[name] = run_form(problem, f, args.steps, args.warmup, bs)
 
+    desc, params = WORKLOADS[wname]
     out = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "data": "synthetic",
-           "config": {"workload": "synthetic 15-mode cubic+quartic field, 680 cubic + 600 quartic FCs, "
-                                  "Ntot 2, caps 6, eps1 1.0, Nstate 100, PT2 eps2 0.01 (seed 0x5648434901)",
-                      "modes": 15, "nstate": 100, "eps1": 1.0, "eps2": 0.01}}
-    forms_out = {}
-    for name, (lat, (ctx, r), sp) in res.items():
-        forms_out[name] = {"solves_per_s": 1.0 / float(np.mean(lat)), "p50_s": percentile(lat, 50),
-                           "p99_s": percentile(lat, 99), "nvar": r["nvar"], "ns2": r["ns2"],
-                           "hb_iterations": r["hb_iterations"], "timings_ms": r["timings"],
-                           "basis_sizes": ctx.iteration_sizes(), "E0": float(r["E"][0]),
-                           "solver_spmm": {"launches": sp["launches"], "ms": sp["ms"],
-                                           "GBs": sp["bytes"] / sp["ms"] / 1e6 if sp["ms"] > 0 else None}}
+           "config": dict({"workload": desc}, **params)}
+    forms_out = {name: form_summary(lat, ctx, r, sp) for name, (lat, (ctx, r), sp) in res.items()}
     out["forms"] = forms_out
     head = "mixed" if "mixed" in res else "fp64"
     if "mixed" in res and "fp64" in res:
-        e64, e32 = res["fp64"][1][1], res["mixed"][1][1]
-        err = float(np.max(np.abs(e64["E"] - e32["E"])))
-        err_var = float(np.max(np.abs(e64["E_var"] - e32["E_var"])))
-        same = e64["checksum"] == e32["checksum"] and e64["nvar"] == e32["nvar"] and e64["ns2"] == e32["ns2"]
-        out["mixed_vs_fp64"] = {"max_abs_err_cm": err, "max_abs_err_var_cm": err_var, "basis_identical": same,
-                                "speedup": forms_out["mixed"]["solves_per_s"] / forms_out["fp64"]["solves_per_s"]}
-        if not (err <= 0.01 and same) or forms_out["mixed"]["solves_per_s"] < forms_out["fp64"]["solves_per_s"]:
+        cmp_ = compare_forms(res["fp64"][1][1], res["mixed"][1][1])
+        cmp_["speedup"] = forms_out["mixed"]["solves_per_s"] / forms_out["fp64"]["solves_per_s"]
+        out["mixed_vs_fp64"] = cmp_
+        if (not (cmp_["max_abs_err_cm"] <= 0.01 and cmp_["basis_identical"])
+                or forms_out["mixed"]["solves_per_s"] < forms_out["fp64"]["solves_per_s"]):
             head = "fp64"
     lat, (ctx, r), _ = res[head]
     out["value"] = forms_out[head]["solves_per_s"]
@@ -527,14 +576,19 @@ def main():
     cms = ctx64.copy_probe(1 << 30)
     out["hbm_copy"] = {"kernel": "k_copy16", "bytes_moved": 2 * (1 << 30), "ms": cms,
                        "GBs": 2.0 * (1 << 30) / cms / 1e6}
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,
+                                           args.cpu_threads)
+    for name, (_, (c, _r), _s) in res.items():
+        c.close()
+    if not args.no_secondary and wname != "c1":
+        out["secondary_c1"] = secondary_run("c1")
     if args.serve > 0:
         out["serving"] = serving(args.serve, args.serve_slots, args.serve_load)
     if not args.no_reference_runs:
         out["reference_runs"] = reference_runs()
+    if args.large_field:
         out["large_field"] = large_field_run()
-    if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,
-                                           args.cpu_threads)
     print(json.dumps(out), flush=True)
 
 

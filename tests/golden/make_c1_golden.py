@@ -1,11 +1,15 @@
-"""Oracle reference of the 15-mode heat-bath workload (variational stage).
+"""Oracle reference of the 15-mode heat-bath workload (c1).
 
-Runs the CPU oracle (about 20 minutes on 5-8 cores) on the synthetic
-15-mode field with PT2 switched off, and stores the basis size after every
-heat-bath iteration, the 100 variational energies (%.17g) and the checksum
-of the final basis (the same checksum vhci_run_ctx reports), in
-c1_oracle.json.  With an existing oracle run directory, pass the paths of
-its JSON summary and basis file to skip the run.
+Runs the CPU oracle on the synthetic 15-mode field (about 20 minutes on
+5-8 cores for the variational stage; with PT2 on, the gather over the 5.4 M
+external states adds about 40 minutes) and stores, in c1_oracle.json, the
+basis size after every heat-bath iteration, the 100 variational energies
+(%.17g), the checksum of the final basis (the same checksum vhci_run_ctx
+reports), and, from the run with PT2 on, the external-space size and the
+100 PT2-corrected energies.  With existing oracle run directories, pass the
+paths of the JSON summary and basis file of the PT2-off run and then the
+JSON summary of the PT2-on run to skip the runs; `--pt2-only JSON` adds the
+PT2 results of an existing PT2-on run to the stored file.
 """
 import json
 import os
@@ -17,6 +21,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+OUT = os.path.join(HERE, "c1_oracle.json")
 
 
 def checksum(q):
@@ -26,24 +31,43 @@ def checksum(q):
         return int(np.sum(v * pw, dtype=np.uint64))
 
 
-def main():
+def oracle_run(pt2):
     from vhci_amd import synth
-    if len(sys.argv) == 3:
-        js, basis = sys.argv[1], sys.argv[2]
+    work = os.path.join(HERE, "_c1_run")
+    os.makedirs(work, exist_ok=True)
+    tag = "c1_pt2" if pt2 else "c1_var"
+    inp = os.path.join(work, tag + ".inp")
+    synth.config_input(1, inp, pt2=pt2)
+    js, basis = os.path.join(work, tag + ".json"), os.path.join(work, tag + ".basis")
+    subprocess.run([os.path.join(ROOT, "oracle", "bin", "vhci_cpu"), "-n", "8", "-i", inp, "-o",
+                    os.path.join(work, tag + ".freqs"), "--json", js, "--basis", basis, "--quiet"], check=True)
+    return js, basis
+
+
+def pt2_fields(js):
+    d = json.load(open(js))
+    return {"pt2_eps2": 0.01, "pt2_space": d["pt2_space"], "E_pt2": d["E"], "oracle_pt2_seconds": d["seconds"]}
+
+
+def main():
+    if len(sys.argv) == 3 and sys.argv[1] == "--pt2-only":
+        out = json.load(open(OUT))
+        out.update(pt2_fields(sys.argv[2]))
+        out["workload"] = "synthetic c1 (seed 0x5648434901); sizes, E_var and checksum from the PT2-off run"
+        json.dump(out, open(OUT, "w"), indent=1)
+        return
+    if len(sys.argv) == 4:
+        js, basis, js2 = sys.argv[1], sys.argv[2], sys.argv[3]
     else:
-        work = os.path.join(HERE, "_c1_run")
-        os.makedirs(work, exist_ok=True)
-        inp = os.path.join(work, "c1_var.inp")
-        synth.config_input(1, inp, pt2=False)
-        js, basis = os.path.join(work, "c1_var.json"), os.path.join(work, "c1_var.basis")
-        subprocess.run([os.path.join(ROOT, "oracle", "bin", "vhci_cpu"), "-n", "8", "-i", inp, "-o",
-                        os.path.join(work, "c1_var.freqs"), "--json", js, "--basis", basis, "--quiet"], check=True)
+        js, basis = oracle_run(False)
+        js2, _ = oracle_run(True)
     d = json.load(open(js))
     q = np.loadtxt(basis, dtype=np.int32)
-    out = {"workload": "synthetic c1 (seed 0x5648434901), PT2 off",
+    out = {"workload": "synthetic c1 (seed 0x5648434901); sizes, E_var and checksum from the PT2-off run",
            "sizes": [d["N0"]] + [it["N"] for it in d["iters"]],
            "E_var": d["E_var"], "checksum": str(checksum(q)), "oracle_seconds": d["seconds"]}
-    with open(os.path.join(HERE, "c1_oracle.json"), "w") as f:
+    out.update(pt2_fields(js2))
+    with open(OUT, "w") as f:
         json.dump(out, f, indent=1)
 
 

@@ -61,6 +61,36 @@ def test_mix_matches_alone(alone):
         assert np.max(np.abs(a["E_var"] - b["E_var"])) <= TOL_CM
 
 
+def test_large_request_in_mix(alone):
+    """A request of the serving workload's own distribution with 34 modes
+    (request 8: eps1 2, Nstate 60, eps2 0.1) alone, then in a mix with the
+    small ones on four slots: every result the same as alone."""
+    big = synth.request_input(8)
+    assert synth.request_params(8)[0] >= 30
+    srv = vhci_amd.Server(slots=1)
+    rid = srv.submit(big)
+    assert srv.wait(rid, 300)
+    big_alone = srv.result(rid)
+    srv.close()
+    _say("[serving] large request alone: %d states, PT2 space %d, %.2f s" %
+         (big_alone["nvar"], big_alone["ns2"], big_alone["t_end"] - big_alone["t_start"]))
+    srv = vhci_amd.Server(slots=4)
+    ids = [srv.submit(big)] + [srv.submit(t) for t in _texts()]
+    res = []
+    for rid in ids:
+        assert srv.wait(rid, 300)
+        res.append(srv.result(rid))
+    st = srv.stats()
+    srv.close()
+    _say("[serving] large + small mix stats", st)
+    assert st["max_running"] > 1
+    for a, b in zip([big_alone] + alone, res):
+        assert a["state"] == b["state"] == "done"
+        assert a["nvar"] == b["nvar"] and a["ns2"] == b["ns2"] and a["checksum"] == b["checksum"]
+        assert np.max(np.abs(a["E"] - b["E"])) <= TOL_CM
+        assert np.max(np.abs(a["E_var"] - b["E_var"])) <= TOL_CM
+
+
 def test_requests_overlap_in_time(alone):
     """With four slots, later requests start before earlier ones end."""
     srv = vhci_amd.Server(slots=4)

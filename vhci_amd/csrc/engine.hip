@@ -80,16 +80,24 @@ __device__ __forceinline__ uint64_t fmix64(uint64_t x) {
     return x;
 }
 
+// Key hash: fmix64 of the linear sum  sum_m q_m * key_coef(m)  (mod 2^64).
+// The sum is linear in the quanta, so the hash of a key moved by a
+// connection pattern is fmix64(sum + the pattern's shift): the kernels that
+// probe many patterns per state add a precomputed shift instead of
+// rehashing the whole key.
+template <int NW>
+__device__ __forceinline__ uint64_t key_lin(const uint32_t (&w)[NW]) {
+    uint64_t h = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) h += (uint64_t)((w[i] >> (8 * b)) & 0xffu) * key_coef(4 * i + b);
+    return h;
+}
+
 template <int NW>
 __device__ __forceinline__ uint64_t key_hash(const uint32_t (&w)[NW]) {
-    uint64_t h = 0x243F6A8885A308D3ULL;
-#pragma unroll
-    for (int i = 0; i < NW; i += 2) {
-        uint64_t v = (uint64_t)w[i] | ((uint64_t)w[i + 1] << 32);
-        h = (h ^ v) * 0x9E3779B97F4A7C15ULL;
-        h ^= h >> 29;
-    }
-    return fmix64(h);
+    return fmix64(key_lin<NW>(w));
 }
 
 template <int NW>
@@ -161,6 +169,31 @@ __device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots,
                                              const uint4* __restrict__ keys, const uint32_t (&w)[NW]) {
     return hash_find_h<NW>(slots, mask, keys, w, key_hash<NW>(w));
 }
+
+// Bloom filter in front of a hash (one bit array, two bits per key, about
+// four bits per key: about 1 MB for a million keys, small enough to stay in
+// a die's L2 while the hash itself does not).  Most probes of the pattern
+// walks find nothing; the filter answers them without touching the hash.
+__device__ __forceinline__ uint64_t bloom_second(uint64_t H) {
+    return ((H ^ (H >> 29)) * 0xBF58476D1CE4E5B9ULL) >> 32;
+}
+__device__ __forceinline__ void bloom_set(uint32_t* f, uint64_t fm, uint64_t H) {
+    const uint64_t b1 = H & fm, b2 = bloom_second(H) & fm;
+    atomicOr(&f[b1 >> 5], 1u << (b1 & 31));
+    atomicOr(&f[b2 >> 5], 1u << (b2 & 31));
+}
+__device__ __forceinline__ bool bloom_maybe(const uint32_t* __restrict__ f, uint64_t fm, uint64_t H) {
+    const uint64_t b1 = H & fm, b2 = bloom_second(H) & fm;
+    return ((f[b1 >> 5] >> (b1 & 31)) & 1u) && ((f[b2 >> 5] >> (b2 & 31)) & 1u);
+}
+
+// Probe for the key w moved by pattern e (nz modes; the caller has checked
+// that every moved quantum stays in 0..255), with the moved key's hash H
+// given: the moved key is built only when a slot's tag matches.
+template <int NW>
+__device__ __forceinline__ int64_t probe_shift(const uint64_t* __restrict__ slots, uint64_t mask,
+                                               const uint4* __restrict__ keys, const uint32_t (&w)[NW],
+                                               const uint4 e, int nz, uint64_t H);
 
 // ------------------------------------------------------------------------
 // table records in registers.  DeltaEnt and HbVar (16 bytes) are read as one
@@ -261,12 +294,14 @@ __global__ void k_to_f32(const double* __restrict__ src, int lds, float* __restr
 }
 
 template <int NW>
-__global__ void k_hash_insert(const uint4* __restrict__ keys, int64_t i0, int64_t i1, uint64_t* slots, uint64_t mask) {
+__global__ void k_hash_insert(const uint4* __restrict__ keys, int64_t i0, int64_t i1, uint64_t* slots, uint64_t mask,
+                              uint32_t* bloom, uint64_t bmask) {
     int64_t i = i0 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= i1) return;
     uint32_t w[NW];
     key_load<NW>(keys, i, w);
     const uint64_t h = key_hash<NW>(w);
+    if (bloom) bloom_set(bloom, bmask, h);
     uint64_t pos = h & mask;
     const uint64_t val = (h & 0xffffffff00000000ULL) | (uint64_t)(uint32_t)i;
     while (true) {
@@ -312,8 +347,10 @@ struct HBuildArgs {
     const uint4* keys;
     const uint64_t* slots;
     uint64_t mask;
+    const uint32_t* bloom;  // filter of the hash's keys (bits bmask + 1)
+    uint64_t bmask;
     const DeltaEnt* deltas;
-    const uint32_t* poff;  // M + 2 block offsets of deltas
+    const uint32_t* poff;     // M + 2 block offsets of deltas
     const GroupEnt* groups;
     const uint32_t* fclist;
     const ElemFC* elem;
@@ -387,6 +424,48 @@ __device__ __forceinline__ bool apply_pattern(const uint4 e, int nz, const uint8
     return ok;
 }
 
+// true when pattern e (nz modes) keeps every quantum of the row in 0..255
+__device__ __forceinline__ bool pattern_ok(const uint4 e, int nz, const uint8_t* __restrict__ sq) {
+    bool ok = true;
+#pragma unroll
+    for (int s = 0; s < kMaxD; s++) {
+        if (s < nz) {
+            const int nqv = (int)sq[rec_u8(e, s)] + rec_i8(e, 6 + s);
+            ok = ok && nqv >= 0 && nqv <= 255;
+        }
+    }
+    return ok;
+}
+
+template <int NW>
+__device__ __forceinline__ int64_t probe_shift(const uint64_t* __restrict__ slots, uint64_t mask,
+                                               const uint4* __restrict__ keys, const uint32_t (&w)[NW],
+                                               const uint4 e, int nz, uint64_t H) {
+    uint64_t pos = H & mask;
+    const uint32_t tag = (uint32_t)(H >> 32);
+    // most probes end at an empty slot before any tag matches: the moved key
+    // is built only after the first match
+    uint64_t s;
+    while (true) {
+        s = slots[pos];
+        if (s == kEmpty) return -1;
+        if ((uint32_t)(s >> 32) == tag) break;
+        pos = (pos + 1) & mask;
+    }
+    uint32_t t[NW];
+#pragma unroll
+    for (int i = 0; i < NW; i++) t[i] = w[i];
+#pragma unroll
+    for (int k = 0; k < kMaxD; k++)
+        if (k < nz) key_add<NW>(t, (int)rec_u8(e, k), rec_i8(e, 6 + k));
+    while (true) {
+        if ((uint32_t)(s >> 32) == tag && key_eq<NW>(keys, (uint32_t)s, t)) return (int64_t)(uint32_t)s;
+        pos = (pos + 1) & mask;
+        s = slots[pos];
+        if (s == kEmpty) return -1;
+    }
+}
+
 // Walk of the pattern blocks a row can use (the table is ordered so that
 // block 0 lowers no mode and block 1 + m has m as its lowest lowered mode):
 // block 0, then the block of every mode with quanta.  b = -1 starts the
@@ -425,9 +504,55 @@ __device__ __forceinline__ double diag_fc_term(const ElemFC* __restrict__ elem, 
     return x;
 }
 
+// the key-sum coefficients of the 64 modes, for a block's LDS table
+__device__ __forceinline__ void load_coefs(uint64_t* cf) {
+    for (int m = threadIdx.x; m < 64; m += blockDim.x) cf[m] = key_coef(m);
+}
+
+// linear key sum of a key held in registers, from the coefficient table in
+// LDS (bytes beyond the modes are zero)
+template <int NW>
+__device__ __forceinline__ uint64_t key_lin_cf(const uint32_t (&w)[NW], const uint64_t* __restrict__ cf) {
+    uint64_t h = 0;
+#pragma unroll
+    for (int i = 0; i < NW; i++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) h += (uint64_t)((w[i] >> (8 * b)) & 0xffu) * cf[4 * i + b];
+    return h;
+}
+
+// linear key sum of a row whose quanta are in LDS (sq, M modes), from the
+// coefficient table in LDS (a loop over the modes: unrolled compile-time
+// coefficients would hold every one in registers)
+__device__ __forceinline__ uint64_t row_lin(const uint8_t* __restrict__ sq, int M, const uint64_t* __restrict__ cf) {
+    uint64_t h = 0;
+    for (int m = 0; m < M; m++) h += (uint64_t)sq[m] * cf[m];
+    return h;
+}
+
+// change of the linear key sum when pattern e (nz modes at bytes 0..5,
+// deltas at bytes 6..11) is applied; cf is the coefficient table in LDS
+__device__ __forceinline__ uint64_t pattern_shift(const uint4 e, int nz, const uint64_t* __restrict__ cf) {
+    uint64_t h = 0;
+#pragma unroll
+    for (int s = 0; s < kMaxD; s++)
+        if (s < nz) h += (uint64_t)(int64_t)rec_i8(e, 6 + s) * cf[rec_u8(e, s)];
+    return h;
+}
+
+// the same for a heat-bath variant (d modes at bytes 1..6, deltas at 7..12)
+__device__ __forceinline__ uint64_t variant_shift(const uint4 v, int d, const uint64_t* __restrict__ cf) {
+    uint64_t h = 0;
+#pragma unroll
+    for (int t = 0; t < kMaxD; t++)
+        if (t < d) h += (uint64_t)(int64_t)rec_i8(v, 7 + t) * cf[rec_u8(v, 1 + t)];
+    return h;
+}
+
 template <int NW, bool FILL>
 __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     __shared__ uint8_t sq[64];
+    __shared__ uint64_t cf[64];
     // fill pass: hits wait here (in pattern order) until 64 of them can be
     // evaluated at once, one per lane, instead of a few lanes per chunk
     __shared__ int32_t qj[2 * kWave];
@@ -444,7 +569,9 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     uint32_t w[NW];
     key_load<NW>(a.keys, src, w);
     if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+    load_coefs(cf);
     __syncthreads();
+    const uint64_t h0 = row_lin(sq, a.M, cf);
     int64_t base = FILL ? a.rowptr[row] : 0;
     int64_t cnt = 0;
     int nq = 0;
@@ -489,12 +616,13 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         int64_t j = -1;
         if (p < pb1) {
             const uint4 e = rec_load(a.deltas, p);
-            uint32_t t[NW];
-#pragma unroll
-            for (int i = 0; i < NW; i++) t[i] = w[i];
-            if (apply_pattern<NW>(e, delta_nz(e), sq, t)) {
-                j = hash_find<NW>(a.slots, a.mask, a.keys, t);
-                hit = j >= 0;
+            const int nz = delta_nz(e);
+            if (pattern_ok(e, nz, sq)) {
+                const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
+                if (bloom_maybe(a.bloom, a.bmask, H)) {
+                    j = probe_shift<NW>(a.slots, a.mask, a.keys, w, e, nz, H);
+                    hit = j >= 0;
+                }
             }
         }
         const uint64_t bal = __ballot(hit);
@@ -544,6 +672,7 @@ template <int NW, int B>
 __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
     const HBuildArgs& a = f.h;
     __shared__ uint8_t sq[64];
+    __shared__ uint64_t cf[64];
     __shared__ int32_t qj[2 * kWave];
     __shared__ int32_t qp[2 * kWave];
     const int lane = threadIdx.x;
@@ -552,6 +681,7 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
     uint32_t w[NW];
     key_load<NW>(a.keys, src, w);
     if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+    load_coefs(cf);
     __syncthreads();
     double acc[B];
 #pragma unroll
@@ -571,6 +701,7 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
                 if (c < f.bcols) acc[c] = acc[c] + v * xr[c];
         }
     };
+    const uint64_t h0 = row_lin(sq, a.M, cf);
     int blk = -1, pb0 = 0, pb1 = 0;
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
@@ -579,12 +710,13 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
         int64_t j = -1;
         if (p < pb1) {
             const uint4 e = rec_load(a.deltas, p);
-            uint32_t t[NW];
-#pragma unroll
-            for (int i = 0; i < NW; i++) t[i] = w[i];
-            if (apply_pattern<NW>(e, delta_nz(e), sq, t)) {
-                j = hash_find<NW>(a.slots, a.mask, a.keys, t);
-                hit = j >= 0;
+            const int nz = delta_nz(e);
+            if (pattern_ok(e, nz, sq)) {
+                const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
+                if (bloom_maybe(a.bloom, a.bmask, H)) {
+                    j = probe_shift<NW>(a.slots, a.mask, a.keys, w, e, nz, H);
+                    hit = j >= 0;
+                }
             }
         }
         const uint64_t bal = __ballot(hit);
@@ -1538,6 +1670,8 @@ struct HbGenArgs {
     const uint4* keys;
     const uint64_t* bslots;
     uint64_t bmask;
+    const uint32_t* bbloom;  // filter of the basis hash's keys
+    uint64_t bbmask;
     const HbVar* vars;
     uint64_t* nslots;
     uint64_t nmask;
@@ -1606,6 +1740,9 @@ __device__ void new_insert(const HbGenArgs& a, const uint32_t (&w)[NW], uint64_t
 // per basis state (grid-stride over states): the lanes walk its candidates.
 template <int NW>
 __global__ __launch_bounds__(256) void k_hb_gen(HbGenArgs a) {
+    __shared__ uint64_t cf[64];
+    load_coefs(cf);
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
     const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
@@ -1614,6 +1751,7 @@ __global__ __launch_bounds__(256) void k_hb_gen(HbGenArgs a) {
         if (cnt == 0) continue;
         uint32_t w0[NW];
         key_load<NW>(a.keys, n, w0);
+        const uint64_t h0 = key_lin_cf<NW>(w0, cf);
         for (int64_t r = lane; r < cnt; r += 64) {
             const uint4 v = rec_load(a.vars, r);
             const int d = (int)rec_u8(v, 0);
@@ -1635,13 +1773,13 @@ __global__ __launch_bounds__(256) void k_hb_gen(HbGenArgs a) {
                 atomicOr(&a.counters[1], 1u);
                 continue;
             }
+            const uint64_t h = fmix64(h0 + variant_shift(v, d, cf));
+            const uint64_t tag = h >> 32;
+            if (tag < a.tlo || tag >= a.thi) continue;
 #pragma unroll
             for (int t = 0; t < kMaxD; t++)
                 if (t < d) key_add<NW>(w, (int)rec_u8(v, 1 + t), rec_i8(v, 7 + t));
-            const uint64_t h = key_hash<NW>(w);
-            const uint64_t tag = h >> 32;
-            if (tag < a.tlo || tag >= a.thi) continue;
-            if (hash_find_h<NW>(a.bslots, a.bmask, a.keys, w, h) >= 0) continue;
+            if (bloom_maybe(a.bbloom, a.bbmask, h) && hash_find_h<NW>(a.bslots, a.bmask, a.keys, w, h) >= 0) continue;
             new_insert<NW>(a, w, h, (uint64_t)(g0 + r));
         }
     }
@@ -1684,8 +1822,10 @@ struct Pt2Args {
     const uint4* vkeys;
     const uint64_t* vslots;
     uint64_t vmask;
+    const uint32_t* vbloom;  // filter of the variational hash's keys
+    uint64_t vbmask;
     const DeltaEnt* deltas;
-    const uint32_t* poff;  // M + 2 block offsets of deltas
+    const uint32_t* poff;    // M + 2 block offsets of deltas
     const GroupEnt* groups;
     const uint32_t* fclist;
     const ElemFC* elem;
@@ -1709,7 +1849,9 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
     __shared__ int32_t hidx[kHitBuf];
     __shared__ double hval[kHitBuf];
     __shared__ double sdiag[64];
+    __shared__ uint64_t cf[64];
     const int lane = threadIdx.x;
+    load_coefs(cf);
     constexpr int R = 4;  // up to 256 columns
     double ks[R], kc[R];
 #pragma unroll
@@ -1731,6 +1873,7 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
             num[r] = 0.0;
             num32[r] = 0.0f;
         }
+        const uint64_t h0 = row_lin(sq, a.M, cf);
         int nh = 0;
         auto flush = [&]() {
             __syncthreads();
@@ -1762,12 +1905,12 @@ __global__ __launch_bounds__(64) void k_pt2(Pt2Args a) {
             if (p < pb1) {
                 e = rec_load(a.deltas, p);
                 nz = delta_nz(e);
-                uint32_t t[NW];
-#pragma unroll
-                for (int i = 0; i < NW; i++) t[i] = w[i];
-                if (apply_pattern<NW>(e, nz, sq, t)) {
-                    j = hash_find<NW>(a.vslots, a.vmask, a.vkeys, t);
-                    hit = j >= 0;
+                if (pattern_ok(e, nz, sq)) {
+                    const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
+                    if (bloom_maybe(a.vbloom, a.vbmask, H)) {
+                        j = probe_shift<NW>(a.vslots, a.vmask, a.vkeys, w, e, nz, H);
+                        hit = j >= 0;
+                    }
                 }
             }
             const uint64_t bal = __ballot(hit);
@@ -1904,11 +2047,14 @@ struct Pt2PushArgs {
     const uint4* vkeys;
     const uint64_t* aslots;
     uint64_t amask;
+    const uint32_t* abloom;  // filter of the chunk's external states
+    uint64_t abmask;
     const uint4* apool;
     const uint32_t* apos;
     const DeltaEnt* deltas;
-    const uint32_t* poff;  // M + 2 block offsets of deltas
-    uint64_t tlo, thi;     // external states whose hash tag lies in [tlo, thi)
+    const uint32_t* poff;    // M + 2 block offsets of deltas
+    uint64_t tlo, thi;       // external states whose hash tag lies in [tlo, thi)
+    int64_t rstride;       // count pass over every rstride-th state only (a sample)
     int64_t* cnt;
     const int64_t* off;
     uint32_t* pkey;
@@ -1918,12 +2064,15 @@ struct Pt2PushArgs {
 template <int NW, bool FILL>
 __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
     __shared__ uint8_t sq[64];
+    __shared__ uint64_t cf[64];
     const int lane = threadIdx.x;
-    const int64_t row = blockIdx.x;
+    const int64_t row = (int64_t)blockIdx.x * a.rstride;
     uint32_t w[NW];
     key_load<NW>(a.vkeys, row, w);
     if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+    load_coefs(cf);
     __syncthreads();
+    const uint64_t h0 = row_lin(sq, a.M, cf);
     int64_t base = FILL ? a.off[row] : 0;
     int64_t cnt = 0;
     int blk = -1, pb0 = 0, pb1 = 0;
@@ -1934,14 +2083,12 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
         int64_t idx = -1;
         if (p < pb1) {
             const uint4 e = rec_load(a.deltas, p);
-            uint32_t t[NW];
-#pragma unroll
-            for (int i = 0; i < NW; i++) t[i] = w[i];
-            if (apply_pattern<NW>(e, delta_nz(e), sq, t)) {
-                const uint64_t h = key_hash<NW>(t);
+            const int nz = delta_nz(e);
+            if (pattern_ok(e, nz, sq)) {
+                const uint64_t h = fmix64(h0 + pattern_shift(e, nz, cf));
                 const uint64_t tag = h >> 32;
-                if (tag >= a.tlo && tag < a.thi) {
-                    idx = hash_find_h<NW>(a.aslots, a.amask, a.apool, t, h);
+                if (tag >= a.tlo && tag < a.thi && bloom_maybe(a.abloom, a.abmask, h)) {
+                    idx = probe_shift<NW>(a.aslots, a.amask, a.apool, w, e, nz, h);
                     hit = idx >= 0;
                 }
             }
@@ -1952,13 +2099,13 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
         } else {
             if (hit) {
                 const int64_t pos = base + __popcll(bal & ((1ull << lane) - 1ull));
-                a.pkey[pos] = a.apos[idx];
+                a.pkey[pos] = a.apos ? a.apos[idx] : (uint32_t)idx;
                 a.pval[pos] = ((uint64_t)(uint32_t)row << 32) | (uint32_t)p;
             }
             base += __popcll(bal);
         }
     }
-    if (!FILL && lane == 0) a.cnt[row] = cnt;
+    if (!FILL && lane == 0) a.cnt[blockIdx.x] = cnt;
 }
 
 // canonical position of every pool entry (inverse of the sort by rank)
@@ -2411,12 +2558,19 @@ void Engine::rebuild_hash(int64_t n) {
     slots_.ensure(cap * sizeof(uint64_t));
     hmask_ = cap - 1;
     k_fill<uint64_t><<<grid_for(cap, 256), 256, 0, stream_>>>(slots_.as<uint64_t>(), (int64_t)cap, kEmpty);
+    // filter of 2 bits per slot (at least 4 per key while the hash holds
+    // at most half its slots)
+    const uint64_t fbits = 2 * cap;
+    bloom_.ensure(fbits / 8);
+    bmask_ = fbits - 1;
+    k_fill<uint32_t><<<grid_for((int64_t)(fbits / 32), 256), 256, 0, stream_>>>(bloom_.as<uint32_t>(),
+                                                                                (int64_t)(fbits / 32), 0u);
     const uint4* k = keys_.as<uint4>();
     switch (KW_) {
-        case 1: k_hash_insert<4><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
-        case 2: k_hash_insert<8><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
-        case 3: k_hash_insert<12><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
-        default: k_hash_insert<16><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_); break;
+        case 1: k_hash_insert<4><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
+        case 2: k_hash_insert<8><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
+        case 3: k_hash_insert<12><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
+        default: k_hash_insert<16><<<grid_for(n, 256), 256, 0, stream_>>>(k, 0, n, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
     }
     HIPCHK(hipGetLastError());
 }
@@ -2466,6 +2620,8 @@ void Engine::build_h_t(HStats& st) {
     a.keys = keys_.as<uint4>();
     a.slots = slots_.as<uint64_t>();
     a.mask = hmask_;
+    a.bloom = bloom_.as<uint32_t>();
+    a.bmask = bmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
     a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
@@ -2517,7 +2673,7 @@ void Engine::build_h_t(HStats& st) {
             extra_.as<unsigned long long>(), oldcnt_.as<int64_t>(), a.rowcnt);
         HIPCHK(hipGetLastError());
     } else if (a.P > 0) {
-        k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
+            k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     } else {
         k_fill<int64_t><<<grid_for(n, 256), 256, 0, stream_>>>(a.rowcnt, n, 0);
@@ -2574,11 +2730,11 @@ void Engine::build_h_t(HStats& st) {
             a.rowptr, a.col, a.val, a.hi, a.lo, form_);
         HIPCHK(hipGetLastError());
         if (nnz > 0) {
-            k_hbuild<NW, true><<<(unsigned)(n - n_old), 64, 0, stream_>>>(a);
+                k_hbuild<NW, true><<<(unsigned)(n - n_old), 64, 0, stream_>>>(a);
             HIPCHK(hipGetLastError());
         }
     } else if (a.P > 0 && nnz > 0) {
-        k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
+            k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     }
     // columns ascending within each row.  rocPRIM's segmented sort counts
@@ -2816,6 +2972,8 @@ int64_t Engine::structural_nnz() {
     a.keys = keys_.as<uint4>();
     a.slots = slots_.as<uint64_t>();
     a.mask = hmask_;
+    a.bloom = bloom_.as<uint32_t>();
+    a.bmask = bmask_;
     a.deltas = d_ball_.as<DeltaEnt>();
     a.poff = d_boff_.as<uint32_t>();
     rowcnt_.ensure((n_ + 1) * sizeof(int64_t));
@@ -2864,6 +3022,8 @@ void Engine::spmm_free_t(const double* X, int ldx, double* Y, int ldy, int b) {
     a.keys = keys_.as<uint4>();
     a.slots = slots_.as<uint64_t>();
     a.mask = hmask_;
+    a.bloom = bloom_.as<uint32_t>();
+    a.bmask = bmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
     a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
@@ -4103,6 +4263,8 @@ bool Engine::hb_gen_t(int64_t total, uint64_t tlo, uint64_t thi, uint64_t max_po
         a.keys = keys_.as<uint4>();
         a.bslots = slots_.as<uint64_t>();
         a.bmask = hmask_;
+        a.bbloom = bloom_.as<uint32_t>();
+        a.bbmask = bmask_;
         a.vars = d_hbvars_.as<HbVar>();
         a.nslots = nslots_.as<uint64_t>();
         a.nmask = cap - 1;
@@ -4235,10 +4397,10 @@ void Engine::append_new_states() {
     } else {
         const uint4* k = keys_.as<uint4>();
         switch (KW_) {
-            case 1: k_hash_insert<4><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
-            case 2: k_hash_insert<8><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
-            case 3: k_hash_insert<12><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
-            default: k_hash_insert<16><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_); break;
+            case 1: k_hash_insert<4><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
+            case 2: k_hash_insert<8><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
+            case 3: k_hash_insert<12><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
+            default: k_hash_insert<16><<<grid_for(n1 - n0, 256), 256, 0, stream_>>>(k, n0, n1, slots_.as<uint64_t>(), hmask_, bloom_.as<uint32_t>(), bmask_); break;
         }
         HIPCHK(hipGetLastError());
     }
@@ -4274,10 +4436,25 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
     a.M = P_.M;
     a.P = P_deltas_;
     a.vkeys = keys_.as<uint4>();
-    a.aslots = nslots_.as<uint64_t>();
-    a.amask = nmask_;
-    a.apool = pool_keys_.as<uint4>();
-    a.apos = pool_pos_.as<uint32_t>();
+    // the chunk's states in a hash sized for them (load <= 1/2; the dedup
+    // hash of the selection is sized from an estimate and is often far
+    // larger), found at their canonical positions
+    const uint64_t cap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)ns2));
+    s2slots_.ensure(cap * sizeof(uint64_t));
+    k_fill<uint64_t><<<grid_for(cap, 256), 256, 0, stream_>>>(s2slots_.as<uint64_t>(), (int64_t)cap, kEmpty);
+    const uint64_t fbits = 2 * cap;
+    s2bloom_.ensure(fbits / 8);
+    k_fill<uint32_t><<<grid_for((int64_t)(fbits / 32), 256), 256, 0, stream_>>>(s2bloom_.as<uint32_t>(),
+                                                                                (int64_t)(fbits / 32), 0u);
+    k_hash_insert<NW><<<grid_for(ns2, 256), 256, 0, stream_>>>(newkeys_.as<uint4>(), 0, ns2, s2slots_.as<uint64_t>(),
+                                                               cap - 1, s2bloom_.as<uint32_t>(), fbits - 1);
+    HIPCHK(hipGetLastError());
+    a.aslots = s2slots_.as<uint64_t>();
+    a.amask = cap - 1;
+    a.abloom = s2bloom_.as<uint32_t>();
+    a.abmask = fbits - 1;
+    a.apool = newkeys_.as<uint4>();
+    a.apos = nullptr;
     a.deltas = d_deltas_.as<DeltaEnt>();
     a.poff = d_doff_.as<uint32_t>();
     a.tlo = tlo;
@@ -4285,6 +4462,24 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
     rowcnt_.ensure((n + 1) * sizeof(int64_t));
     pairoff_.ensure((n + 1) * sizeof(int64_t));
     a.cnt = rowcnt_.as<int64_t>();
+    // pairs estimated from every 64th state first: a chunk far over the
+    // budget is split without a full count
+    constexpr int64_t kSample = 64;
+    if (n >= 64 * kSample) {
+        const int64_t ns = (n + kSample - 1) / kSample;
+        a.rstride = kSample;
+        k_pt2_push<NW, false><<<(unsigned)ns, 64, 0, stream_>>>(a);
+        HIPCHK(hipGetLastError());
+        std::vector<int64_t> sc(ns);
+        HIPCHK(hipMemcpyAsync(sc.data(), a.cnt, ns * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        const double est = (double)std::accumulate(sc.begin(), sc.end(), int64_t(0)) * ((double)n / (double)ns);
+        if (est > 1.25 * (double)max_pairs) {
+            last_pairs_ = (int64_t)(1.25 * est);
+            return false;
+        }
+    }
+    a.rstride = 1;
     k_pt2_push<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
     k_fill<int64_t><<<1, 1, 0, stream_>>>(a.cnt + n, 1, 0);
     size_t tmp = 0;
@@ -4299,7 +4494,10 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
     if (std::getenv("VHCI_DEBUG"))
         std::fprintf(stderr, "[pt2] chunk [%#llx, %#llx): %lld external states, %lld pairs\n", (unsigned long long)tlo,
                      (unsigned long long)thi, (long long)ns2, (long long)npairs);
-    if (npairs > max_pairs) return false;
+    if (npairs > max_pairs) {
+        last_pairs_ = npairs;
+        return false;
+    }
     DevBuf pk, pv, sk, sv, seg;
     try {
         pk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
@@ -4380,6 +4578,8 @@ void Engine::pt2_pull_t(const double* Ed, int nopt, int nwaves, bool carry) {
     a.vkeys = keys_.as<uint4>();
     a.vslots = slots_.as<uint64_t>();
     a.vmask = hmask_;
+    a.vbloom = bloom_.as<uint32_t>();
+    a.vbmask = bmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
     a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
@@ -4437,11 +4637,13 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, double eps2, Pt2Stats
     int nwaves = 0;
     bool summed = false;
     std::vector<std::pair<uint64_t, uint64_t>> work{{0, uint64_t(1) << 32}};
-    auto split = [&](uint64_t lo, uint64_t hi) {
+    // a range is cut into `parts` equal ranges (a power of two), pushed so
+    // that the lowest comes next
+    auto split = [&](uint64_t lo, uint64_t hi, uint64_t parts) {
         if (hi - lo < 2) throw std::runtime_error("PT2: one hash tag holds more external states or pairs than a chunk");
-        const uint64_t mid = lo + (hi - lo) / 2;
-        work.push_back({mid, hi});
-        work.push_back({lo, mid});
+        parts = std::min<uint64_t>(std::max<uint64_t>(2, next_pow2(parts)), hi - lo);
+        const uint64_t w = (hi - lo) / parts;
+        for (uint64_t k = parts; k-- > 0;) work.push_back({lo + k * w, k + 1 == parts ? hi : lo + (k + 1) * w});
     };
     while (total > 0 && !work.empty()) {
         const auto [lo, hi] = work.back();
@@ -4450,7 +4652,7 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, double eps2, Pt2Stats
         const int64_t u0 = hs.unique;
         if (!hb_gen_t<NW>(total, lo, hi, (uint64_t)max_states, hs)) {
             sel_ms += ms_since(t1);
-            split(lo, hi);
+            split(lo, hi, 2);
             continue;
         }
         sel_ms += ms_since(t1);
@@ -4468,7 +4670,10 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, double eps2, Pt2Stats
         } else if (!pt2_push_t<NW>(Ed.as<double>(), nopt, nwaves, max_pairs, summed, lo, hi, st)) {
             sum_ms += ms_since(t2);
             hs.unique = u0;
-            split(lo, hi);
+            // as many parts as the counted pairs need (two when the buffers
+            // could not be allocated)
+            split(lo, hi, last_pairs_ > max_pairs ? (uint64_t)((last_pairs_ + max_pairs - 1) / max_pairs) : 2);
+            last_pairs_ = 0;
             continue;
         }
         sum_ms += ms_since(t2);

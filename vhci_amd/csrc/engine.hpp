@@ -273,6 +273,8 @@ class Engine {
     int64_t n_ = 0;
     DevBuf keys_, slots_;
     uint64_t hmask_ = 0;
+    DevBuf bloom_;        // filter of the basis hash's keys
+    uint64_t bmask_ = 0;  // its bit count - 1
     // tables
     DevBuf d_ladder_, d_elem_, d_deltas_, d_groups_, d_fclist_, d_diag_, d_freq_;
     DevBuf d_hbfc_, d_hbvaroff_, d_hbvars_, d_ball_;
@@ -313,6 +315,9 @@ class Engine {
     // PT2 work
     DevBuf s2keys_, pt2_part_, C32_;
     DevBuf pairoff_;  // PT2 push: prefix of the pair counts per variational state
+    DevBuf s2slots_;  // PT2 push: hash of one chunk's external states
+    DevBuf s2bloom_;  // and its filter
+    int64_t last_pairs_ = 0;  // pairs counted by a PT2 chunk that exceeded the budget
     double last_spmm_ms_ = 0;
     struct SpmmProf {
         int b = 0;

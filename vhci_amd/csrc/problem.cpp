@@ -418,6 +418,7 @@ void build_tables(const Problem& P, Tables& T) {
         }
         T.hb_varoff.push_back(T.hb_varoff.back() + cnt);
     }
+
 }
 
 void write_freqs(std::ostream& os, const Problem& P, const std::vector<double>& E,

@@ -112,6 +112,22 @@ struct Tables {
     std::vector<uint32_t> delta_off, ball_off;
 };
 
+// Coefficient of mode m in the linear key sum of the device hash
+// (sum_m q_m * key_coef(m) mod 2^64, then a 64-bit finalizer): pseudo-random
+// and odd (splitmix64 of m), so that distinct small-integer quanta vectors
+// give distinct sums except with probability about 2^-64.  The sum is linear,
+// so the hash of a key moved by a pattern is that of the sum plus
+// sum_t delta_t * key_coef(mode_t): no rehash of the whole key.
+#if defined(__HIP__)
+__host__ __device__
+#endif
+inline constexpr uint64_t key_coef(int m) {
+    uint64_t z = 0x56484349ull * 0x9E3779B97F4A7C15ull + (uint64_t)(m + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (z ^ (z >> 31)) | 1ull;
+}
+
 // Stable reorder of patterns into the blocks described above; off receives
 // the M + 2 block offsets.
 void order_patterns(std::vector<DeltaEnt>& v, int M, std::vector<uint32_t>& off);
