@@ -1,8 +1,9 @@
 """Full-size parity of the 15-mode heat-bath workload against the oracle's
-20-minute CPU run (tests/golden/c1_oracle.json): the basis size after every
-heat-bath iteration, the final basis (checksum over the quanta in order) and
-the 100 variational energies; the mixed form must select the identical
-basis and agree within 0.01 cm^-1."""
+CPU runs (tests/golden/c1_oracle.json: 20 minutes without PT2, 48 minutes
+with it): the basis size after every heat-bath iteration, the final basis
+(checksum over the quanta in order), the 100 variational energies, the PT2
+external space and the 100 PT2-corrected energies; the mixed form must
+select the identical basis and agree within 0.01 cm^-1."""
 import json
 import os
 
@@ -51,6 +52,24 @@ def test_c1_mixed_matches_fp64(ref, fp64_run):
     assert ctx.iteration_sizes() == ref["sizes"]
     assert r["checksum"] == fp64_run["checksum"]
     assert np.max(np.abs(r["E_var"] - fp64_run["E_var"])) < 1e-2
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_c1_pt2_matches_oracle(ref, form):
+    """The whole workload with PT2 (eps2 0.01): the external space of the
+    oracle's run (5,381,485 states) and its 100 corrected energies, within
+    1e-6 cm^-1 (FP64) or 0.01 cm^-1 (mixed), with the oracle's basis."""
+    p = vhci_amd.Problem(text=synth.config_input(1))
+    ctx = vhci_amd.Context(p, form)
+    r = ctx.run()
+    sizes = ctx.iteration_sizes()
+    ctx.close()
+    err = float(np.max(np.abs(r["E"] - np.array(ref["E_pt2"]))))
+    print("c1 PT2 form %d: external %d, max |E - oracle| = %.3e cm^-1" % (form, r["ns2"], err), flush=True)
+    assert sizes == ref["sizes"]
+    assert r["checksum"] == int(ref["checksum"])
+    assert r["ns2"] == ref["pt2_space"]
+    assert err < (1e-6 if form == vhci_amd.FP64 else 1e-2)
 
 
 def test_c1_freq_assignments_match_full_scan(fp64_run):
