@@ -179,6 +179,16 @@ int vhci_pt2(vhci_ctx* ctx, const double* E, int nopt, double eps2, double* dE, 
  * product (FP32 values and vectors); otherwise the product is FP64 (FP64
  * values, or hi + lo in the mixed form).  Runs the solver's own kernels. */
 int vhci_apply_h(vhci_ctx* ctx, int b, int fp32, const double* X, double* Y);
+/* vhci_apply_h without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results (bit for bit) as vhci_apply_h;
+ * the device staging of X and Y and their solver-order copies live in a
+ * fixed scratch array of 112 MiB that each GPU the library runs on holds
+ * until the process ends, so n * b may be at most 4,194,304 (VHCI_E_LIMIT
+ * beyond).  Calls on one GPU share its array and are serialised by a lock
+ * (two calls never overlap on it); the array keeps the last call's data. */
+int vhci_apply_h_noalloc(vhci_ctx* ctx, int b, int fp32, const double* X, double* Y);
 /* Time b-vector SpMMs with the resident H (reps products, HIP events on
  * the context stream).  out[0] = ms per product, out[1] = algorithmic
  * bytes per product, out[2] = kernel launches per product.  fp32 != 0

@@ -65,6 +65,7 @@ _SIGS = {
     "vhci_append_new_states": (ctypes.c_int, [_vp]),
     "vhci_pt2": (ctypes.c_int, [_vp, _dp, ctypes.c_int, ctypes.c_double, _dp, _i64p]),
     "vhci_apply_h": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
+    "vhci_apply_h_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _dp, _dp]),
     "vhci_time_spmm": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp]),
     "vhci_copy_probe": (ctypes.c_int, [_vp, ctypes.c_int64, _dp]),
     "vhci_spmm_profile": (ctypes.c_int, [_vp, ctypes.c_int]),
@@ -299,6 +300,16 @@ class Context:
         X = np.ascontiguousarray(X.reshape(self.n, -1))
         Y = np.zeros_like(X)
         self._check(lib().vhci_apply_h(self.h, X.shape[1], int(fp32), _ptr(X, _dp), _ptr(Y, _dp)))
+        return Y[:, 0] if vec else Y
+
+    def apply_h_noalloc(self, X, fp32=False):
+        """apply_h through the entry point that allocates no device memory
+        (vhci_apply_h_noalloc)"""
+        X = np.asarray(X, dtype=np.float64)
+        vec = X.ndim == 1
+        X = np.ascontiguousarray(X.reshape(self.n, -1))
+        Y = np.zeros_like(X)
+        self._check(lib().vhci_apply_h_noalloc(self.h, X.shape[1], int(fp32), _ptr(X, _dp), _ptr(Y, _dp)))
         return Y[:, 0] if vec else Y
 
     def eigsolve(self, nev, tol=1e-10, maxit=500, vectors=True):

@@ -504,6 +504,13 @@ int vhci_apply_h(vhci_ctx* ctx, int b, int fp32, const double* X, double* Y) {
     });
 }
 
+int vhci_apply_h_noalloc(vhci_ctx* ctx, int b, int fp32, const double* X, double* Y) {
+    return guard(ctx, [&] {
+        if (!X || !Y) throw std::runtime_error("vhci_apply_h_noalloc: null vector");
+        ctx->eng->apply_h_noalloc(X, Y, b, fp32 != 0);
+    });
+}
+
 int vhci_time_spmm(vhci_ctx* ctx, int b, int reps, int fp32, double* out) {
     return guard(ctx, [&] {
         double bytes = 0;

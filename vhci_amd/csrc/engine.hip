@@ -25,6 +25,10 @@
 #include "engine.hpp"
 
 namespace vhci {
+static double* noalloc_scratch(int device, std::mutex** lock);
+}
+
+namespace vhci {
 
 namespace {
 
@@ -2425,6 +2429,8 @@ Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), dev
     KW_ = (P_.M + 15) / 16;
     HIPCHK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     pool_.stream = stream_;
+    // set-up of the no-allocation product: the scratch address of this device
+    (void)noalloc_scratch(device, nullptr);
     BLASCHK(rocblas_create_handle(&blas_));
     BLASCHK(rocblas_set_stream(blas_, stream_));
     // no atomics-based algorithms: the same inputs give the same bits on every run
@@ -3150,6 +3156,97 @@ void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low,
     else if (b <= 32) go(std::integral_constant<int, 32>());
     else go(std::integral_constant<int, 64>());
     HIPCHK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------- H X
+// without allocation.  The staging of apply_h_noalloc lives in `scratch`, a
+// fixed array in the code object: each GPU the library runs on holds its own
+// copy (112 MiB) until the process ends.  Host code never names the array
+// in a kernel argument or copy; it uses the address hipGetSymbolAddress
+// gives for the current device (noalloc_scratch).
+constexpr int64_t kScratchElems = int64_t(4) << 20;  // n * b at most
+constexpr int64_t kScratchDoubles = 3 * kScratchElems + kScratchElems / 2;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, double*> g_scratch_ptr;                    // device -> address of its scratch
+std::map<int, std::unique_ptr<std::mutex>> g_scratch_use;  // device -> lock held by a call
+}  // namespace
+
+// the device's scratch address (looked up once per device, with that device
+// current) and the lock that serialises the calls using it
+static double* noalloc_scratch(int device, std::mutex** lock) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto it = g_scratch_ptr.find(device);
+    if (it == g_scratch_ptr.end()) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("address of the no-allocation scratch array: ") +
+                                     hipGetErrorString(e));
+        it = g_scratch_ptr.emplace(device, static_cast<double*>(p)).first;
+        g_scratch_use.emplace(device, std::make_unique<std::mutex>());
+    }
+    if (lock) *lock = g_scratch_use[device].get();
+    return it->second;
+}
+
+// rows of a row-major n x b block moved through perm: gather (dst row r =
+// src row perm[r]) or scatter (dst row perm[r] = src row r)
+__global__ void k_perm_rows(const double* __restrict__ src, double* __restrict__ dst, int64_t n, int b,
+                            const int32_t* __restrict__ perm, int gather) {
+    const int64_t total = n * b;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / b;
+        const int c = (int)(t - r * b);
+        const int64_t pr = perm[r];
+        if (gather)
+            dst[t] = src[pr * b + c];
+        else
+            dst[pr * b + c] = src[t];
+    }
+}
+
+void Engine::apply_h_noalloc(const double* X, double* Y, int b, bool fp32) {
+    HIPCHK(hipSetDevice(device_));
+    if (!h_valid_) throw std::runtime_error("Hamiltonian not built");
+    if (b < 1 || b > 64) throw std::runtime_error("apply_h: 1..64 vectors");
+    if (fp32 && form_ != kMixed) throw std::runtime_error("apply_h: the FP32 product exists in the mixed form only");
+    if (fp32 && matfree_) throw std::runtime_error("apply_h: matrix-free products are FP64 only");
+    const int64_t n = n_;
+    const int64_t cnt = n * b;
+    if (cnt > kScratchElems)
+        throw std::length_error("apply_h_noalloc: n * b = " + std::to_string(cnt) + " is larger than the scratch limit of " +
+                                std::to_string(kScratchElems) + " elements");
+    std::mutex* lock = nullptr;
+    double* S = noalloc_scratch(device_, &lock);
+    std::lock_guard<std::mutex> lk(*lock);
+    double* A = S;                 // X, then Y, in basis order
+    double* B = S + cnt;           // X in solver order
+    double* C = S + 2 * cnt;       // Y in solver order
+    float* X32 = reinterpret_cast<float*>(S + 3 * cnt);  // FP32 copy of the solver-order X
+    HIPCHK(hipMemcpyAsync(A, X, (size_t)cnt * sizeof(double), hipMemcpyHostToDevice, stream_));
+    const unsigned g = std::min<unsigned>(grid_for(cnt, 256), 8192);
+    const double* xs = A;
+    if (perm_on_) {
+        k_perm_rows<<<g, 256, 0, stream_>>>(A, B, n, b, perm_.as<int32_t>(), 1);
+        HIPCHK(hipGetLastError());
+        xs = B;
+    }
+    if (fp32) {
+        k_to_f32<<<grid_for(cnt, 256), 256, 0, stream_>>>(xs, b, X32, b, n, b);
+        HIPCHK(hipGetLastError());
+    }
+    spmm(xs, b, C, b, b, fp32, fp32 ? X32 : nullptr, b);
+    const double* ys = C;
+    if (perm_on_) {
+        k_perm_rows<<<g, 256, 0, stream_>>>(C, A, n, b, perm_.as<int32_t>(), 0);
+        HIPCHK(hipGetLastError());
+        ys = A;
+    }
+    HIPCHK(hipMemcpyAsync(Y, ys, (size_t)cnt * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHK(hipStreamSynchronize(stream_));
 }
 
 // plain device copy of `bytes` bytes (16-byte words), timed with HIP events:

@@ -190,6 +190,11 @@ class Engine {
     // Y = H X for b host vectors (row-major n x b, basis order) through the
     // solver's product kernels; fp32 selects the mixed form's FP32 product
     void apply_h(const double* X, double* Y, int b, bool fp32);
+    // the same product with every device buffer it uses (staging of X and
+    // Y, their solver-order copies) in a fixed per-device scratch array:
+    // no device allocation.  n * b is limited (kScratchElems); calls on one
+    // GPU are serialised by a lock, since they share that GPU's array
+    void apply_h_noalloc(const double* X, double* Y, int b, bool fp32);
     // time one SpMM of b vectors, averaged over reps (for the roofline figure)
     double time_spmm(int b, int reps, bool fp32, double* bytes_per_call);
     // a plain copy of `bytes` bytes on the context stream; returns ms
