@@ -184,32 +184,34 @@ def traffic_pass(counters, kernel, workload, timeout_s=420):
     return out, nk
 
 
-def measure_traffic(kernel, workload):
+def measure_traffic(kernel, workload, request_sizes=False):
     """HBM bytes per launch of `kernel`.  traffic: a FETCH_SIZE pass and a
     WRITE_SIZE pass (KiB per dispatch), each scaled by what the same counter
     reads for the 1 GiB copy (1 GiB fetched, 1 GiB written).  FETCH_SIZE
     counts a 128-byte read request as 64 bytes on this GPU (the copy reads
     half its size), while 32- and 64-byte requests count in full, so the
     copy-scaled figure overstates a kernel whose reads are 64-byte gathers;
-    a third pass counts the read requests by size (32, 64, 128 bytes) and
-    gives the read bytes directly."""
+    with request_sizes a third pass counts the read requests by size (32,
+    64, 128 bytes) and gives the read bytes directly."""
     f, nk = traffic_pass(["FETCH_SIZE"], kernel, workload)
     w, _ = traffic_pass(["WRITE_SIZE"], kernel, workload)
-    sizes = ["TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum"]
-    r, _ = traffic_pass(sizes, kernel, workload)
     gib_kib = float(1 << 20)
     fk, fc = f["FETCH_SIZE"]
     wk, wc = w["WRITE_SIZE"]
     ff, fw = gib_kib / fc, gib_kib / wc
     traffic = 1024.0 * (fk * ff + wk * fw)
-    read_bytes = 32.0 * r[sizes[0]][0] + 64.0 * r[sizes[1]][0] + 128.0 * r[sizes[2]][0]
-    copy_read = 32.0 * r[sizes[0]][1] + 64.0 * r[sizes[1]][1] + 128.0 * r[sizes[2]][1]
-    return traffic, {"fetch_kib": fk, "write_kib": wk, "dispatches": nk, "copy_fetch_kib": fc,
-                     "copy_write_kib": wc, "fetch_scale": ff, "write_scale": fw,
-                     "read_requests_32_64_128_all": [r[c][0] for c in sizes],
-                     "read_bytes_by_request_size": read_bytes,
-                     "copy_read_bytes_by_request_size": copy_read,
-                     "traffic_by_request_size": read_bytes + 1024.0 * wk * fw}
+    out = {"fetch_kib": fk, "write_kib": wk, "dispatches": nk, "copy_fetch_kib": fc, "copy_write_kib": wc,
+           "fetch_scale": ff, "write_scale": fw}
+    if request_sizes:
+        sizes = ["TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum", "TCC_EA0_RDREQ_sum"]
+        r, _ = traffic_pass(sizes, kernel, workload)
+        read_bytes = 32.0 * r[sizes[0]][0] + 64.0 * r[sizes[1]][0] + 128.0 * r[sizes[2]][0]
+        out.update({"read_requests_32_64_128_all": [r[c][0] for c in sizes],
+                    "read_bytes_by_request_size": read_bytes,
+                    "copy_read_bytes_by_request_size": 32.0 * r[sizes[0]][1] + 64.0 * r[sizes[1]][1] +
+                    128.0 * r[sizes[2]][1],
+                    "traffic_by_request_size": read_bytes + 1024.0 * wk * fw})
+    return traffic, out
 
 
 def cpu_baseline(ctx, r, problem_text, threads):
@@ -230,10 +232,12 @@ def cpu_baseline(ctx, r, problem_text, threads):
         os.unlink(path)
     q = ctx.basis()
     ns2 = r["ns2"]
-    nA = min(ns2, max(1000, int(4e9 / max(1, q.shape[0]))))
+    # bounded sample, about 20 s of the oracle on the host cores: 3e8 pairs
+    # of the all-pairs H loop and 1e9 of the PT2 gather
+    nA = min(ns2, max(200, int(1e9 / max(1, q.shape[0]))))
     qA = ctx.new_states(ns2)[:nA] if nA else np.zeros((0, q.shape[1]), dtype=np.int32)
     n = q.shape[0]
-    rows = min(n, max(1, int(3e9 / n)))
+    rows = min(n, max(1, int(3e8 / n)))
     nopt = min(o.neig, n)
     rate_h, rate_p, raw = o.sample_rates(q, rows, qA, nopt)
     sizes = ctx.iteration_sizes()
@@ -268,10 +272,14 @@ def serving(nreq, slots, load, seed=7, wide=False):
     texts = [synth.request_input(r, **SERVE_RANGES) for r in range(nreq)]
     srv = vhci_amd.Server(slots=1)
     alone = []
-    for t in texts:
+    for r, t in enumerate(texts):
         rid = srv.submit(t)
         srv.wait(rid)
-        alone.append(srv.result(rid))
+        a = srv.result(rid)
+        alone.append(a)
+        print("[bench] serving request %d alone (M %d, eps1 %g, Nstate %d, eps2 %g): %d states, PT2 space %d, "
+              "%.2f s" % ((r,) + synth.request_params(r, **SERVE_RANGES)[:4] +
+                          (a["nvar"], a["ns2"], a["t_end"] - a["t_start"])), file=sys.stderr, flush=True)
     srv.close()
     service = [a["t_end"] - a["t_start"] for a in alone]
     rate = load * nreq / sum(service)
@@ -298,9 +306,10 @@ def serving(nreq, slots, load, seed=7, wide=False):
         th.start()
         th.join()
         res = []
-        for rid in ids:
+        for k, rid in enumerate(ids):
             srv.wait(rid)
             res.append(srv.result(rid))
+            print("[bench] serving %s: request %d done" % (name, k), file=sys.stderr, flush=True)
         st = srv.stats()
         srv.close()
         lat = [r["t_end"] - r["t_submit"] for r in res]
@@ -321,9 +330,10 @@ def serving(nreq, slots, load, seed=7, wide=False):
         srv = vhci_amd.Server(slots=ns)
         ids = [srv.submit(t) for t in texts]
         res = []
-        for rid in ids:
+        for k, rid in enumerate(ids):
             srv.wait(rid)
             res.append(srv.result(rid))
+            print("[bench] serving fp64 saturated, %d slots: request %d done" % (ns, k), file=sys.stderr, flush=True)
         st = srv.stats()
         srv.close()
         lat = [r["t_end"] - r["t_submit"] for r in res]
@@ -459,7 +469,7 @@ def main():
     ap.add_argument("--spmm-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--serve", type=int, default=24, help="requests in the serving stream (0: skip)")
+    ap.add_argument("--serve", type=int, default=8, help="requests in the serving stream (0: skip)")
     ap.add_argument("--serve-only", action="store_true", help="only the serving stream (one JSON line)")
     ap.add_argument("--no-reference-runs", action="store_true",
                     help="skip the reference's logged runs (acetonitrile, ethylene oxide, ClO2)")
@@ -469,6 +479,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=1.2, help="offered load / solo FP64 rate")
     ap.add_argument("--no-traffic", action="store_true", help="skip the counter passes (traffic is then null)")
+    ap.add_argument("--request-sizes", action="store_true",
+                    help="a third counter pass: the SpMM's read requests by size (32, 64, 128 bytes)")
     ap.add_argument("--large-field-only", action="store_true",
                     help="only the 48-mode naphthalene run at the large workload's parameters (one JSON line)")
     ap.add_argument("--profile-spmm", action="store_true",
@@ -498,7 +510,7 @@ def main():
         # the counter passes run before this process touches the GPU; a pass
         # that fails (no rocprofv3, a refused counter) leaves traffic null
         try:
-            traffic = measure_traffic(kname, wname)
+            traffic = measure_traffic(kname, wname, args.request_sizes)
         except Exception as e:  # noqa: BLE001
             print("[bench] traffic not measured: %s" % e, file=sys.stderr, flush=True)
             traffic = None
@@ -564,7 +576,8 @@ def main():
             t, detail = traffic
             out["roofline"]["traffic"] = t
             detail["traffic_over_algorithmic"] = t / per
-            detail["traffic_by_request_size_over_algorithmic"] = detail["traffic_by_request_size"] / per
+            if "traffic_by_request_size" in detail:
+                detail["traffic_by_request_size_over_algorithmic"] = detail["traffic_by_request_size"] / per
             out["roofline"]["traffic_detail"] = detail
     # single-vector SpMV on the final H, and a plain 1 GiB device copy
     ms1, b1, _ = ctx64.time_spmm(1, args.spmm_reps)
