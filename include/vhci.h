@@ -212,13 +212,14 @@ int vhci_spmm_profile_read(vhci_ctx* ctx, double* out);
  * support; no counterpart in the reference. */
 int vhci_gemm_profile_read(vhci_ctx* ctx, double* out);
 /* Counters of the last phases (measurement support; no counterpart in the
- * reference), out[15]: last H build n, stored entries, connection
+ * reference), out[16]: last H build n, off-diagonal entries, connection
  * patterns, build ms; last heat-bath selection (after vhci_pt2: the PT2
  * one, all chunks) candidates, unique states, ms; last PT2 external states,
  * (external, variational) pairs, sum ms; key bytes; PT2 states; rows the
  * last H build probed (all, or only the new states' rows of an incremental
  * build); entries it carried over from the previous build; chunks the last
- * PT2 summed its external space in. */
+ * PT2 summed its external space in; the last build's storage (0 full CSR,
+ * 1 upper triangle, 2 matrix-free products). */
 int vhci_phase_stats(vhci_ctx* ctx, double* out);
 /* Device-memory pool of the context, out[5]: hipMalloc calls, hipFree
  * calls, blocks handed out again from the idle set, bytes held (in use +

@@ -1,7 +1,8 @@
 """GPU parity of the product kernels: H X through the solver's own kernels
 (vhci_apply_h) against a float64 product of the stored H copied to the host,
 for every block-width class the dispatcher distinguishes, in each precision
-form, in basis order and in the solver's row order, and in matrix-free mode.
+form, in basis order and in the solver's row order, in matrix-free mode and
+with only the upper triangle stored (half storage).
 
 The error of each output element is measured against sum_j |H_ij| |X_jc|,
 the scale its rounding error is proportional to:
@@ -113,4 +114,43 @@ def test_apply_h_rejects_bad_arguments():
         ctx.apply_h(np.zeros((ctx.n, 65)))
     with pytest.raises(RuntimeError, match="mixed form only"):
         ctx.apply_h(np.zeros((ctx.n, 2)), fp32=True)
+    ctx.close()
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_products_half_storage(grown_text, form, monkeypatch):
+    """Upper triangle only: the same matrix (get_h and get_h_rows mirror it,
+    bit for bit equal to the full CSR's), the same basis, products to the
+    bars above, and every product the same bit for bit when repeated (the
+    transposed terms are summed in integer fixed point)."""
+    _say("products, serving-mix request 5, half storage against full, form", form)
+    p = vhci_amd.Problem(text=grown_text)
+    monkeypatch.setenv("VHCI_H_STORAGE", "full")
+    ref = vhci_amd.Context(p, form)
+    r0 = ref.run()
+    assert ref.phase_stats()["h_storage"] == 0
+    H, diag = _stored(ref)
+    rows = np.random.default_rng(5).choice(ref.n, 300, replace=False)
+    rows_ref = ref.get_h_rows(rows)
+    ref.close()
+    monkeypatch.setenv("VHCI_H_STORAGE", "half")
+    ctx = vhci_amd.Context(p, form)
+    r1 = ctx.run()
+    ph = ctx.phase_stats()
+    assert ph["h_storage"] == 1 and ph["nnz"] == H.nnz
+    assert r1["checksum"] == r0["checksum"] and r1["nvar"] == r0["nvar"]
+    assert np.max(np.abs(r1["E"] - r0["E"])) < 1e-7
+    H2, diag2 = _stored(ctx)
+    assert (H2 != H).nnz == 0 and np.array_equal(diag2, diag)
+    for a, b in zip(ctx.get_h_rows(rows), rows_ref):
+        assert np.array_equal(a, b)
+    for b, e in _errors(ctx, H, diag, fp32=False, widths=(1, 3, 4, 8, 16, 24, 40)).items():
+        assert e < 1e-12, (b, e)
+    if form == vhci_amd.MIXED:
+        for b, e in _errors(ctx, H, diag, fp32=True, widths=(1, 8, 16)).items():
+            assert 1e-9 < e < 1e-4, (b, e)
+    X = np.random.default_rng(3).standard_normal((ctx.n, 8))
+    y1 = ctx.apply_h(X)
+    y2 = ctx.apply_h(X)
+    assert np.array_equal(y1, y2)
     ctx.close()

@@ -422,10 +422,10 @@ class Context:
         return float(out[0]), float(out[1]), int(out[2])
 
     def phase_stats(self):
-        out = np.zeros(15)
+        out = np.zeros(16)
         self._check(lib().vhci_phase_stats(self.h, _ptr(out, _dp)))
         keys = ["n", "nnz", "patterns", "build_ms", "hb_candidates", "hb_unique", "hb_ms", "ns2", "pairs",
-                "pt2_sum_ms", "key_bytes", "nstate", "probed_rows", "carried", "pt2_chunks"]
+                "pt2_sum_ms", "key_bytes", "nstate", "probed_rows", "carried", "pt2_chunks", "h_storage"]
         return dict(zip(keys, out.tolist()))
 
     def pool_stats(self):

@@ -9,6 +9,7 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_segmented_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_reduce.hpp>
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
@@ -362,6 +363,9 @@ struct HBuildArgs {
     const int32_t* perm;  // solver row -> basis state (null: identity)
     const int32_t* inv;   // basis state -> solver row (null: identity)
     int64_t* rowcnt;
+    int64_t* rowcnt_up;  // count pass: entries with a higher stored column (null: not counted)
+    int half;            // fill pass: store only entries with a higher stored column
+    double* rowabs;      // half fill pass: sum of |H_ij| over the whole row
     const int64_t* rowptr;
     int32_t* col;
     double* val;
@@ -577,10 +581,15 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     __syncthreads();
     const uint64_t h0 = row_lin(sq, a.M, cf);
     int64_t base = FILL ? a.rowptr[row] : 0;
-    int64_t cnt = 0;
+    int64_t cnt = 0, cnt_up = 0;
     int nq = 0;
-    // evaluate and store the first `count` (<= 64) queued hits
+    double absum = 0.0;  // half fill: this lane's share of the row's sum of |H_ij|
+    // evaluate and store the first `count` (<= 64) queued hits (half
+    // storage: evaluate all, store those above the diagonal)
     auto drain = [&](int count) {
+        const int64_t cj = lane < count ? (a.inv ? a.inv[qj[lane]] : qj[lane]) : -1;
+        const bool keep = lane < count && (!a.half || cj > row);
+        const uint64_t km = __ballot(keep);
         if (lane < count) {
             const int p = qp[lane];
             const int64_t j = qj[lane];
@@ -588,29 +597,32 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
             const GroupEnt ge = a.groups[delta_group(e)];
             // the element starts from the lower basis index, as AnharmPot(min, max)
             const double v = group_element(ge, a.fclist, a.elem, a.ladder, a.nq, sq, e, delta_nz(e), src < j);
-            const int64_t pos = base + lane;
-            a.col[pos] = a.inv ? a.inv[j] : (int32_t)j;
-            // an old state's row gets the transposed entry (same value: the
-            // element starts from the lower basis index either way)
-            const bool tr = j < a.n_old;
-            const int64_t pt =
-                tr ? a.rowptr[a.inv ? a.inv[j] : j] + a.oldcnt[j] + (int64_t)atomicAdd(&a.cursor[j], 1ull) : 0;
-            if (tr) a.col[pt] = (int32_t)row;
-            if (a.form == kFP64) {
-                a.val[pos] = v;
-                if (tr) a.val[pt] = v;
-            } else {
-                const float h = (float)v;
-                const float l = (float)(v - (double)h);
-                a.hi[pos] = h;
-                a.lo[pos] = l;
-                if (tr) {
-                    a.hi[pt] = h;
-                    a.lo[pt] = l;
+            absum += fabs(v);
+            if (keep) {
+                const int64_t pos = base + __popcll(km & ((1ull << lane) - 1ull));
+                a.col[pos] = (int32_t)cj;
+                // an old state's row gets the transposed entry (same value:
+                // the element starts from the lower basis index either way)
+                const bool tr = j < a.n_old;
+                const int64_t pt =
+                    tr ? a.rowptr[a.inv ? a.inv[j] : j] + a.oldcnt[j] + (int64_t)atomicAdd(&a.cursor[j], 1ull) : 0;
+                if (tr) a.col[pt] = (int32_t)row;
+                if (a.form == kFP64) {
+                    a.val[pos] = v;
+                    if (tr) a.val[pt] = v;
+                } else {
+                    const float h = (float)v;
+                    const float l = (float)(v - (double)h);
+                    a.hi[pos] = h;
+                    a.lo[pos] = l;
+                    if (tr) {
+                        a.hi[pt] = h;
+                        a.lo[pt] = l;
+                    }
                 }
             }
         }
-        base += count;
+        base += __popcll(km);
     };
     int blk = -1, pb0 = 0, pb1 = 0;
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
@@ -632,6 +644,7 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         const uint64_t bal = __ballot(hit);
         if (!FILL) {
             cnt += __popcll(bal);
+            if (a.rowcnt_up) cnt_up += __popcll(__ballot(hit && (a.inv ? a.inv[j] : j) > row));
             if (hit && j < a.n_old) atomicAdd(&a.extra[j], 1ull);
         } else {
             if (hit) {
@@ -654,7 +667,17 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         }
     }
     if (FILL && nq > 0) drain(nq);
-    if (!FILL && lane == 0) a.rowcnt[row] = cnt;
+    if (!FILL && lane == 0) {
+        a.rowcnt[row] = cnt;
+        if (a.rowcnt_up) a.rowcnt_up[row] = cnt_up;
+    }
+    if (FILL && a.half) {
+        // fixed-order wave sum (the row's bound for the fixed-point
+        // accumulation of the transposed products)
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) absum += __shfl_xor(absum, off);
+        if (lane == 0) a.rowabs[row] = absum;
+    }
 }
 
 // Matrix-free product Y = (D + H_off) X for B-column blocks (bcols <= B
@@ -1113,6 +1136,165 @@ __global__ __launch_bounds__(256) void k_spmv(int64_t n, const int64_t* __restri
     if (live && sub == 0) y[row] = D[row] * x[row] + (double)acc;
 }
 
+// ------------------------------------------------------------------------
+// Half (upper-triangle) storage: Y = D X + U X + U^T X, U the strict upper
+// triangle in CSR.  The row's wave forms (D X + U X)_i in FP64 in a fixed
+// order and stores it.  The transposed products U_ij X_i go to row j of an
+// int64 fixed-point accumulator by integer atomic adds: integer addition is
+// associative, so the accumulated sums, and the product, are the same bit
+// for bit whatever order the adds land in.  Column c is scaled by
+// s_c = 2^e_c with hnorm * max_i |X_ic| * s_c < 2^62 (hnorm = max_i sum_j
+// |H_ij| over the whole row), which bounds every partial sum of every row
+// below 2^62; each term is rounded once, to a multiple of 1/s_c.
+// k_half_finish adds the accumulator to Y and clears it for the next
+// product.  The adds bound the product: about 1e11 atomic adds per second
+// (the same with one accumulator copy per XCD added at workgroup scope in
+// that XCD's L2), against the gathers' 1.5e11 entries per second of the
+// full-storage product.
+// ------------------------------------------------------------------------
+__device__ __forceinline__ double half_scale(double hnorm, uint64_t cmax_bits) {
+    // the margin covers FP32 rounding of values and X in the mixed form
+    const double bound = hnorm * __longlong_as_double((long long)cmax_bits) * 1.001;
+    if (!(bound > 0.0)) return 1.0;
+    int ex;
+    (void)frexp(bound, &ex);  // bound < 2^ex
+    return ldexp(1.0, max(-1000, min(1000, 62 - ex)));
+}
+
+// per-column max |X_rc| as the bit pattern of a non-negative double (such
+// patterns order as the values do), atomically maxed into cmax[c]
+__global__ __launch_bounds__(256) void k_colmax(int64_t n, int b, const double* __restrict__ X, int ldx,
+                                                unsigned long long* __restrict__ cmax) {
+    const int c = threadIdx.x % b;
+    const int rpb = 256 / b;  // rows per block step
+    if (threadIdx.x >= rpb * b) return;
+    double m = 0.0;
+    for (int64_t r = (int64_t)blockIdx.x * rpb + threadIdx.x / b; r < n; r += (int64_t)gridDim.x * rpb)
+        m = fmax(m, fabs(X[r * ldx + c]));
+    atomicMax(&cmax[c], (unsigned long long)__double_as_longlong(m));
+}
+
+template <int B, int MODE, int G = 64>
+__global__ __launch_bounds__(256) void k_spmm_half(int64_t n, const int64_t* __restrict__ rowptr,
+                                                   const int32_t* __restrict__ col, const double* __restrict__ val,
+                                                   const float* __restrict__ hi, const float* __restrict__ lo,
+                                                   const double* __restrict__ D, const float* __restrict__ X32,
+                                                   const double* __restrict__ X, double* __restrict__ Y, int ldy,
+                                                   double hnorm, const unsigned long long* __restrict__ cmax,
+                                                   unsigned long long* __restrict__ acc_t) {
+    using AT = typename SpAcc<MODE>::T;
+    using Cp = SpCoop<B, MODE>;
+    constexpr int W = Cp::W, P = Cp::P, E = G / P;
+    const int lane = threadIdx.x & (G - 1);
+    const int64_t row = (int64_t)blockIdx.x * (256 / G) + threadIdx.x / G;
+    if (row >= n) return;
+    const int q = lane % P, e = lane / P;
+    // this lane's columns q*W .. q*W+W-1: scales and the row's own X
+    double sc[W];
+    AT xi[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) sc[w] = half_scale(hnorm, cmax[q * W + w]);
+    sp_piece<B, MODE>(X, X32, row, q, xi);
+    AT acc[W];
+#pragma unroll
+    for (int w = 0; w < W; w++) acc[w] = 0;
+    const int64_t k0 = rowptr[row], k1 = rowptr[row + 1];
+    for (int64_t k = k0; k < k1; k += G) {
+        const bool in = k + lane < k1;
+        const int32_t jl = in ? __builtin_nontemporal_load(col + k + lane) : 0;
+        const AT vl = in ? sp_val<MODE>(val, hi, lo, k + lane) : AT(0);
+#pragma unroll
+        for (int u = 0; u < P; u++) {
+            const int32_t j = __shfl(jl, u * E + e, G);
+            const AT v = __shfl(vl, u * E + e, G);
+            if (k + u * E + e < k1) {
+                AT x[W];
+                sp_piece<B, MODE>(X, X32, j, q, x);
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    acc[w] = sp_fma(v, x[w], acc[w]);
+                    const unsigned long long t =
+                        (unsigned long long)__double2ll_rn((double)(v * xi[w]) * sc[w]);
+                    (void)__hip_atomic_fetch_add(&acc_t[(int64_t)j * B + q * W + w], t, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int off = P; off < G; off <<= 1)
+#pragma unroll
+        for (int w = 0; w < W; w++) acc[w] += __shfl_xor(acc[w], off, G);
+    if (lane < P) {
+        const double d = D[row];
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int cc = lane * W + w;
+            Y[row * ldy + cc] = d * X[row * B + cc] + (double)acc[w];
+        }
+    }
+}
+
+// Y[r, c] += acc_t[r, c] / s_c, and acc_t back to zero
+template <int B>
+__global__ void k_half_finish(int64_t n, unsigned long long* __restrict__ acc_t, double hnorm,
+                              const unsigned long long* __restrict__ cmax, double* __restrict__ Y, int ldy) {
+    const int64_t total = n * B;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / B;
+        const int c = (int)(t - r * B);
+        const double inv = 1.0 / half_scale(hnorm, cmax[c]);  // a power of two: exact
+        Y[r * ldy + c] += (double)(long long)acc_t[t] * inv;
+        acc_t[t] = 0ull;
+    }
+}
+
+// columns c0 .. c0+bc-1 of X (row-major, ldx) into a zero-padded n x B
+// block, and back
+__global__ void k_pack_cols(const double* __restrict__ X, int ldx, int64_t n, int bc, int B, double* __restrict__ out) {
+    const int64_t total = n * B;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / B;
+        const int c = (int)(t - r * B);
+        out[t] = c < bc ? X[r * ldx + c] : 0.0;
+    }
+}
+__global__ void k_unpack_cols(const double* __restrict__ in, int B, int64_t n, int bc, double* __restrict__ Y,
+                              int ldy) {
+    const int64_t total = n * bc;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / bc;
+        const int c = (int)(t - r * bc);
+        Y[r * ldy + c] = in[r * B + c];
+    }
+}
+
+// Entries of U in the columns of requested rows (mark[col] >= 0): the lower
+// half of those rows.  count pass (out_req null): their number in
+// counter[0]; fill pass: (request, source row, value) at counter[0]'s
+// successive positions.
+__global__ void k_lower_of(int64_t n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ val, const float* __restrict__ hi, const float* __restrict__ lo,
+                           const int32_t* __restrict__ mark, unsigned long long* __restrict__ counter,
+                           int32_t* __restrict__ out_req, int32_t* __restrict__ out_row, double* __restrict__ out_val,
+                           unsigned long long cap) {
+    const int64_t r = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (r >= n) return;
+    for (int64_t k = rowptr[r] + (threadIdx.x & 63); k < rowptr[r + 1]; k += 64) {
+        const int32_t q = mark[col[k]];
+        if (q < 0) continue;
+        if (!out_req) {
+            atomicAdd(&counter[0], 1ull);
+            continue;
+        }
+        const unsigned long long pos = atomicAdd(&counter[0], 1ull);
+        if (pos >= cap) continue;
+        out_req[pos] = q;
+        out_row[pos] = (int32_t)r;
+        out_val[pos] = val ? val[k] : (double)hi[k] + (double)lo[k];
+    }
+}
+
 // pack / unpack the mixed form's hi, lo pair into one 64-bit word (sort payload)
 __global__ void k_pack_hilo(int64_t nnz, const float* __restrict__ hi, const float* __restrict__ lo,
                             uint64_t* __restrict__ out) {
@@ -1226,14 +1408,16 @@ __global__ void k_symmetrize(double* G, int m, int ld) {
 }
 
 // dense symmetric matrix (column-major n x n) from the CSR
+// (half storage, sym: each stored entry also goes to its mirror position)
 __global__ void k_densify(int64_t n, const int64_t* __restrict__ rowptr, const int32_t* __restrict__ col,
                           const double* __restrict__ val, const float* __restrict__ hi, const float* __restrict__ lo,
-                          const double* __restrict__ D, double* __restrict__ A) {
+                          const double* __restrict__ D, double* __restrict__ A, int sym) {
     const int64_t i = blockIdx.x;
     if (i >= n) return;
     for (int64_t k = rowptr[i] + threadIdx.x; k < rowptr[i + 1]; k += blockDim.x) {
         const double v = val ? val[k] : (double)hi[k] + (double)lo[k];
         A[(int64_t)col[k] * n + i] = v;
+        if (sym) A[i * n + col[k]] = v;
     }
     if (threadIdx.x == 0) A[i * n + i] = D[i];
 }
@@ -2641,8 +2825,8 @@ void Engine::build_h_t(HStats& st) {
     // the transposes of the new rows' entries to them.
     const int64_t n_old = built_n_;
     const char* fb = std::getenv("VHCI_FULL_BUILD");
-    const bool incr = n_old > 0 && n_old < n && 2 * (n - n_old) < n && a.P > 0 && !matfree_ &&
-                      !(fb && std::string(fb) == "1");
+    bool incr = n_old > 0 && n_old < n && 2 * (n - n_old) < n && a.P > 0 && !matfree_ && !sym_ &&
+                !(fb && std::string(fb) == "1");
     built_n_ = 0;
     const bool old_perm_on = perm_on_;
     auto swap_buf = [](DevBuf& x, DevBuf& y) { x.swap(y); };
@@ -2661,6 +2845,9 @@ void Engine::build_h_t(HStats& st) {
     rowcnt_.ensure((n + 1) * sizeof(int64_t));
     rowptr_.ensure((n + 1) * sizeof(int64_t));
     a.rowcnt = rowcnt_.as<int64_t>();
+    // from-scratch count passes also count the upper triangle (half storage)
+    rowcnt_up_.ensure((n + 1) * sizeof(int64_t));
+    a.rowcnt_up = incr ? nullptr : rowcnt_up_.as<int64_t>();
     if (incr) {
         extra_.ensure(n_old * sizeof(uint64_t));
         oldcnt_.ensure(n_old * sizeof(int64_t));
@@ -2683,19 +2870,50 @@ void Engine::build_h_t(HStats& st) {
         HIPCHK(hipGetLastError());
     } else {
         k_fill<int64_t><<<grid_for(n, 256), 256, 0, stream_>>>(a.rowcnt, n, 0);
+        k_fill<int64_t><<<grid_for(n, 256), 256, 0, stream_>>>(a.rowcnt_up, n, 0);
     }
-    k_fill<int64_t><<<1, 1, 0, stream_>>>(a.rowcnt + n, 1, 0);
-    size_t tmp = 0;
-    HIPCHK(rocprim::exclusive_scan(nullptr, tmp, a.rowcnt, rowptr_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
-                                   rocprim::plus<int64_t>(), stream_));
-    scan_tmp_.ensure(tmp);
-    HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, a.rowcnt, rowptr_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
-                                   rocprim::plus<int64_t>(), stream_));
-    int64_t nnz = 0;
-    HIPCHK(hipMemcpyAsync(&nnz, rowptr_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
-    nnz_ = nnz;
-    matfree_ = use_matrix_free(nnz);
+    // row offsets (exclusive scan of n + 1 counts, the last one zero); returns the total
+    auto scan_counts = [&](int64_t* cnt) {
+        k_fill<int64_t><<<1, 1, 0, stream_>>>(cnt + n, 1, 0);
+        size_t tmp = 0;
+        HIPCHK(rocprim::exclusive_scan(nullptr, tmp, cnt, rowptr_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                       rocprim::plus<int64_t>(), stream_));
+        scan_tmp_.ensure(tmp);
+        HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, cnt, rowptr_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                       rocprim::plus<int64_t>(), stream_));
+        int64_t tot = 0;
+        HIPCHK(hipMemcpyAsync(&tot, rowptr_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        return tot;
+    };
+    int64_t nnz_up = a.rowcnt_up ? scan_counts(a.rowcnt_up) : -1;
+    int64_t nnz = scan_counts(a.rowcnt);
+    int storage = choose_storage(nnz, nnz_up >= 0 ? nnz_up : nnz / 2);
+    if (incr && storage != 0) {
+        // the grown matrix no longer fits in full: count again from scratch
+        // (half storage is built whole)
+        col_o_.release();
+        val_o_.release();
+        hi_o_.release();
+        lo_o_.release();
+        incr = false;
+        a.first = 0;
+        a.n_old = 0;
+        a.extra = nullptr;
+        a.oldcnt = nullptr;
+        a.cursor = nullptr;
+        a.rowcnt_up = rowcnt_up_.as<int64_t>();
+        k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
+        HIPCHK(hipGetLastError());
+        nnz_up = scan_counts(a.rowcnt_up);
+        nnz = scan_counts(a.rowcnt);
+        storage = choose_storage(nnz, nnz_up);
+    }
+    sym_ = storage == 1;
+    if (sym_) (void)scan_counts(a.rowcnt_up);
+    const int64_t nstore = sym_ ? nnz_up : nnz;
+    nnz_ = nstore;
+    matfree_ = storage == 2;
     if (matfree_) {
         // no stored entries: products regenerate them (k_hfree)
         col_.release();
@@ -2717,18 +2935,27 @@ void Engine::build_h_t(HStats& st) {
         st.probed_rows = incr ? n - n_old : n;
         return;
     }
-    col_.ensure((nnz + kCsrPad) * sizeof(int32_t));
+    col_.ensure((nstore + kCsrPad) * sizeof(int32_t));
     if (form_ == kFP64) {
-        val_.ensure((nnz + kCsrPad) * sizeof(double));
+        val_.ensure((nstore + kCsrPad) * sizeof(double));
     } else {
-        hi_.ensure((nnz + kCsrPad) * sizeof(float));
-        lo_.ensure((nnz + kCsrPad) * sizeof(float));
+        hi_.ensure((nstore + kCsrPad) * sizeof(float));
+        lo_.ensure((nstore + kCsrPad) * sizeof(float));
     }
     a.rowptr = rowptr_.as<int64_t>();
     a.col = col_.as<int32_t>();
     a.val = val_.as<double>();
     a.hi = hi_.as<float>();
     a.lo = lo_.as<float>();
+    a.half = sym_ ? 1 : 0;
+    if (sym_) {
+        rowabs_.ensure(n * sizeof(double));
+        a.rowabs = rowabs_.as<double>();
+    } else {
+        rowabs_.release();
+        hacc_.release();
+        hacc_zero_ = 0;
+    }
     if (incr) {
         k_copy_old_rows<<<grid_for(n_old, 4), 256, 0, stream_>>>(
             n_old, a.inv, old_perm_on ? inv_o_.as<int32_t>() : nullptr, old_perm_on ? perm_o_.as<int32_t>() : nullptr,
@@ -2743,10 +2970,26 @@ void Engine::build_h_t(HStats& st) {
             k_hbuild<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     }
+    if (sym_) {
+        // the accumulator's bound: the largest sum of |H_ij| over a row
+        hnorm_ = 0;
+        if (nnz > 0) {
+            DevBuf mx;
+            mx.ensure(sizeof(double));
+            size_t tmp = 0;
+            HIPCHK(rocprim::reduce(nullptr, tmp, rowabs_.as<double>(), mx.as<double>(), 0.0, (size_t)n,
+                                   rocprim::maximum<double>(), stream_));
+            scan_tmp_.ensure(tmp);
+            HIPCHK(rocprim::reduce(scan_tmp_.p, tmp, rowabs_.as<double>(), mx.as<double>(), 0.0, (size_t)n,
+                                   rocprim::maximum<double>(), stream_));
+            HIPCHK(hipMemcpyAsync(&hnorm_, mx.p, sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+        }
+    }
     // columns ascending within each row.  rocPRIM's segmented sort counts
     // entries in 32 bits, so rows are sorted in slices of at most kSortSlice
     // entries, each with its own row offsets relative to the slice start.
-    if (nnz > 0) {
+    if (nstore > 0) {
         unsigned endbit = 1;
         while (endbit < 31 && ((uint64_t)(n - 1) >> endbit) != 0) endbit++;
         std::vector<int64_t> rph(n + 1);
@@ -2754,7 +2997,7 @@ void Engine::build_h_t(HStats& st) {
         HIPCHK(hipStreamSynchronize(stream_));
         int64_t slice = kSortSlice;
         if (const char* e = std::getenv("VHCI_SORT_SLICE")) slice = std::max<int64_t>(1, std::atoll(e));
-        const int64_t slice_cap = std::min<int64_t>(nnz, slice);
+        const int64_t slice_cap = std::min<int64_t>(nstore, slice);
         DevBuf ctmp, vin, vout, rel;
         ctmp.ensure(slice_cap * sizeof(int32_t));
         if (form_ != kFP64) vin.ensure(slice_cap * sizeof(uint64_t));
@@ -2842,6 +3085,16 @@ HStats Engine::build_h() {
     h_valid_ = true;
     st.build_ms = ms_since(t0);
     last_h_ = st;
+    if (std::getenv("VHCI_DEBUG")) {
+        size_t freeb = 0, totb = 0;
+        HIPCHK(hipMemGetInfo(&freeb, &totb));
+        static const char* const kind[3] = {"full CSR", "upper triangle", "matrix-free products"};
+        std::fprintf(stderr,
+                     "[h-build] n %lld off-diagonal entries %lld, %s (%lld stored) %.1f ms; context holds %.1f GB, "
+                     "%.1f GB free\n",
+                     (long long)n_, (long long)st.nnz_offdiag, kind[h_storage()], (long long)(matfree_ ? 0 : nnz_),
+                     st.build_ms, pool_.held / 1e9, freeb / 1e9);
+    }
     return st;
 }
 
@@ -2870,6 +3123,37 @@ void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std:
         }
     }
     HIPCHK(hipStreamSynchronize(stream_));
+    if (sym_) {
+        // upper triangle -> both triangles (solver order, columns ascending:
+        // row c takes its lower entries in increasing source row, then its
+        // own upper ones)
+        std::vector<int64_t> cnt(n_, 0);
+        for (int64_t r = 0; r < n_; r++) {
+            cnt[r] += rowptr[r + 1] - rowptr[r];
+            for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) cnt[col[k]]++;
+        }
+        std::vector<int64_t> rp(n_ + 1, 0);
+        for (int64_t r = 0; r < n_; r++) rp[r + 1] = rp[r] + cnt[r];
+        std::vector<int32_t> c2(rp[n_]);
+        std::vector<double> v2(rp[n_]);
+        std::vector<int64_t> cur(rp.begin(), rp.end() - 1);
+        for (int64_t r = 0; r < n_; r++) {
+            for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+                const int64_t c = col[k];
+                c2[cur[c]] = (int32_t)r;
+                v2[cur[c]++] = val[k];
+            }
+            // row r's lower entries all come from rows < r: placed already
+            for (int64_t k = rowptr[r]; k < rowptr[r + 1]; k++) {
+                c2[cur[r]] = col[k];
+                v2[cur[r]++] = val[k];
+            }
+        }
+        rowptr.swap(rp);
+        col.swap(c2);
+        val.swap(v2);
+    }
+    const int64_t nnz_all = (int64_t)col.size();
     if (perm_on_) {
         // back to basis order: row perm[r] holds stored row r, columns mapped
         // through perm and sorted again
@@ -2879,8 +3163,8 @@ void Engine::get_h(std::vector<int64_t>& rowptr, std::vector<int32_t>& col, std:
         for (int64_t r = 0; r < n_; r++) inv[perm[r]] = (int32_t)r;
         std::vector<int64_t> rp(n_ + 1, 0);
         for (int64_t i = 0; i < n_; i++) rp[i + 1] = rp[i] + (rowptr[inv[i] + 1] - rowptr[inv[i]]);
-        std::vector<int32_t> c2(nnz_);
-        std::vector<double> v2(nnz_), d2(n_);
+        std::vector<int32_t> c2(nnz_all);
+        std::vector<double> v2(nnz_all), d2(n_);
         std::vector<std::pair<int32_t, double>> row;
         for (int64_t i = 0; i < n_; i++) {
             const int64_t r = inv[i];
@@ -2918,10 +3202,59 @@ void Engine::get_h_rows(const int64_t* rows, int64_t nr, std::vector<int64_t>& r
     col.clear();
     val.clear();
     diag.assign(nr, 0.0);
+    for (int64_t k = 0; k < nr; k++)
+        if (rows[k] < 0 || rows[k] >= n_) throw std::runtime_error("get_h_rows: row out of range");
+    // half storage: the lower half of the requested rows (entries of the
+    // stored triangle in their columns), as (solver row, value) lists
+    std::vector<std::vector<std::pair<int32_t, double>>> lower;
+    if (sym_ && nnz_ > 0) {
+        lower.resize(nr);
+        std::vector<int32_t> mk(n_, -1);
+        for (int64_t k = 0; k < nr; k++) {
+            const int64_t r = perm_on_ ? inv[rows[k]] : rows[k];
+            if (mk[r] < 0) mk[r] = (int32_t)k;
+        }
+        DevBuf mark, ctr;
+        mark.ensure(n_ * sizeof(int32_t));
+        ctr.ensure(sizeof(unsigned long long));
+        HIPCHK(hipMemcpyAsync(mark.p, mk.data(), n_ * sizeof(int32_t), hipMemcpyHostToDevice, stream_));
+        const bool f64 = form_ == kFP64;
+        auto scan = [&](int32_t* oq, int32_t* orow, double* ov, int64_t cap) {
+            k_fill<uint64_t><<<1, 64, 0, stream_>>>(ctr.as<uint64_t>(), 1, 0ull);
+            k_lower_of<<<grid_for(n_, 4), 256, 0, stream_>>>(
+                n_, rowptr_.as<int64_t>(), col_.as<int32_t>(), f64 ? val_.as<double>() : nullptr, hi_.as<float>(),
+                lo_.as<float>(), mark.as<int32_t>(), ctr.as<unsigned long long>(), oq, orow, ov,
+                (unsigned long long)cap);
+            HIPCHK(hipGetLastError());
+            unsigned long long c = 0;
+            HIPCHK(hipMemcpyAsync(&c, ctr.p, sizeof(c), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            return (int64_t)c;
+        };
+        const int64_t m = scan(nullptr, nullptr, nullptr, 0);
+        if (m > 0) {
+            DevBuf oq, orow, ov;
+            oq.ensure(m * sizeof(int32_t));
+            orow.ensure(m * sizeof(int32_t));
+            ov.ensure(m * sizeof(double));
+            if (scan(oq.as<int32_t>(), orow.as<int32_t>(), ov.as<double>(), m) != m)
+                throw std::runtime_error("get_h_rows: the lower-half scan changed between passes");
+            std::vector<int32_t> hq(m), hr(m);
+            std::vector<double> hv(m);
+            HIPCHK(hipMemcpyAsync(hq.data(), oq.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipMemcpyAsync(hr.data(), orow.p, m * sizeof(int32_t), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipMemcpyAsync(hv.data(), ov.p, m * sizeof(double), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            for (int64_t t = 0; t < m; t++) lower[hq[t]].emplace_back(hr[t], hv[t]);
+        }
+        for (int64_t k = 0; k < nr; k++) {
+            const int64_t r = perm_on_ ? inv[rows[k]] : rows[k];
+            if (mk[r] != k) lower[k] = lower[mk[r]];
+        }
+    }
     std::vector<std::pair<int32_t, double>> row;
     for (int64_t k = 0; k < nr; k++) {
         const int64_t b = rows[k];
-        if (b < 0 || b >= n_) throw std::runtime_error("get_h_rows: row out of range");
         const int64_t r = perm_on_ ? inv[b] : b;
         int64_t se[2];
         HIPCHK(hipMemcpyAsync(se, rowptr_.as<int64_t>() + r, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
@@ -2949,6 +3282,8 @@ void Engine::get_h_rows(const int64_t* rows, int64_t nr, std::vector<int64_t>& r
         }
         row.clear();
         for (int64_t t = 0; t < cnt; t++) row.emplace_back(perm_on_ ? perm[c[t]] : c[t], v[t]);
+        if (!lower.empty())
+            for (const auto& e : lower[k]) row.emplace_back(perm_on_ ? perm[e.first] : e.first, e.second);
         std::sort(row.begin(), row.end(), [](const auto& x, const auto& y) { return x.first < y.first; });
         for (auto& e : row) {
             col.push_back(e.first);
@@ -3002,19 +3337,54 @@ int64_t Engine::structural_nnz() {
 // ---------------------------------------------------------------- SpMM
 int Engine::spmm_launches(int) const { return 1; }
 
-// Stored CSR when it fits beside the eigensolver's blocks, else matrix-free
-// products.  VHCI_MATRIX_FREE=1 / 0 forces either.
-bool Engine::use_matrix_free(int64_t nnz) const {
-    if (const char* e = std::getenv("VHCI_MATRIX_FREE")) return std::string(e) == "1";
+// Full CSR when it fits beside the eigensolver's blocks, else the upper
+// triangle, else matrix-free products.  VHCI_H_STORAGE=full|half|free forces
+// one (VHCI_MATRIX_FREE=1 / 0 forces matrix-free / full).  The memory counted
+// as available: free device memory, the previous matrix (released first) and
+// the context pool's idle blocks (returned to the device when an
+// allocation would fail).
+int Engine::choose_storage(int64_t nnz, int64_t nnz_up) {
+    ks_mfac_ = 4.0;
+    if (const char* e = std::getenv("VHCI_H_STORAGE")) {
+        const std::string v(e);
+        if (v == "full") return 0;
+        if (v == "half") return 1;
+        if (v == "free") return 2;
+        throw std::runtime_error("VHCI_H_STORAGE must be full, half or free, not '" + v + "'");
+    }
+    if (const char* e = std::getenv("VHCI_MATRIX_FREE")) return std::string(e) == "1" ? 2 : 0;
     size_t free_b = 0, total_b = 0;
     HIPCHK(hipMemGetInfo(&free_b, &total_b));
-    const double sv = form_ == kFP64 ? 8.0 : 8.0;  // FP64 value, or FP32 hi + lo
-    const double csr = (double)nnz * (sv + 4.0) * 2.0;  // entries, plus the sort's copy of a slice
-    // Krylov-Schur blocks (V, W: 4 nev + bs columns each; X, HX: 1.6 nev each)
+    const double sv = 8.0;  // FP64 value, or FP32 hi + lo
+    // the column sort's slice buffers
+    const double sort = (double)kSortSlice * (4.0 + 8.0 + (form_ == kFP64 ? 0.0 : 8.0));
+    const double n = (double)n_;
+    // Krylov-Schur blocks: V, W (mfac nev + bs columns each), the kept Ritz
+    // vectors and their products (1.6 nev each) and the resident Ritz block
     const double nev = std::max(1, P_.neig);
-    const double eig = (double)n_ * 8.0 * (2.0 * (4.0 * nev + 8.0) + 2.0 * 1.6 * nev);
-    const double held = (double)(col_.cap + val_.cap + hi_.cap + lo_.cap);  // released first
-    return csr + eig > 0.9 * ((double)free_b + held);
+    auto eig = [&](double mfac) { return n * 8.0 * (2.0 * (mfac * nev + 8.0) + 2.0 * 1.6 * nev + nev); };
+    const double held = (double)(col_.cap + val_.cap + hi_.cap + lo_.cap + col_o_.cap + val_o_.cap + hi_o_.cap +
+                                 lo_o_.cap + hacc_.cap) +
+                        (double)pool_.idle_bytes;
+    const double avail = 0.9 * ((double)free_b + held);
+    // the sort's buffers are freed before the eigensolver starts
+    const double full = (double)nnz * (sv + 4.0);
+    // upper triangle, the row sums and the accumulator (n x 8)
+    const double half = (double)nnz_up * (sv + 4.0) + n * 8.0 + n * 8.0 * 8.0;
+    // full storage, with the Krylov-Schur basis down to 2.5 nev if need be
+    // (more restarts cost less than the half form's slower products), then
+    // half storage with the basis down to 2 nev
+    for (double mf = 4.0; mf >= 2.5; mf -= 0.25)
+        if (full + std::max(sort, eig(mf)) <= avail) {
+            ks_mfac_ = mf;
+            return 0;
+        }
+    for (double mf = 4.0; mf >= 2.0; mf -= 0.25)
+        if (half + std::max(sort, eig(mf)) <= avail) {
+            ks_mfac_ = mf;
+            return 1;
+        }
+    return 2;
 }
 
 template <int NW>
@@ -3053,6 +3423,73 @@ void Engine::spmm_free_t(const double* X, int ldx, double* Y, int ldy, int b) {
     }
 }
 
+// Half storage: B = 4, 8 or 16 dense columns go straight to k_spmm_half;
+// other widths and strided blocks through zero-padded dense chunks.
+void Engine::spmm_half(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32,
+                       int ldx32) {
+    const int64_t n = n_;
+    const int mode = form_ == kFP64 ? 0 : (low ? 1 : 2);
+    const bool dense = ldx == b && ((uintptr_t)X & 15) == 0 && (b == 4 || b == 8) &&
+                       (mode != 1 || !X32 || (ldx32 == b && ((uintptr_t)X32 & 15) == 0));
+    if (!dense) {
+        const int B = b <= 4 ? 4 : 8;
+        hpack_x_.ensure((size_t)n * B * sizeof(double));
+        hpack_y_.ensure((size_t)n * B * sizeof(double));
+        const unsigned g = std::min<unsigned>(grid_for(n * B, 256), 8192);
+        for (int c0 = 0; c0 < b; c0 += B) {
+            const int bc = std::min(B, b - c0);
+            k_pack_cols<<<g, 256, 0, stream_>>>(X + c0, ldx, n, bc, B, hpack_x_.as<double>());
+            HIPCHK(hipGetLastError());
+            spmm_half(hpack_x_.as<double>(), B, hpack_y_.as<double>(), B, B, low, nullptr, 0);
+            k_unpack_cols<<<g, 256, 0, stream_>>>(hpack_y_.as<double>(), B, n, bc, Y + c0, ldy);
+            HIPCHK(hipGetLastError());
+        }
+        return;
+    }
+    if (mode == 1 && !X32) {
+        X32_.ensure((size_t)n * b * sizeof(float));
+        k_to_f32<<<grid_for(n * b, 256), 256, 0, stream_>>>(X, ldx, X32_.as<float>(), b, n, b);
+        X32 = X32_.as<float>();
+    }
+    // the accumulator (n x 8) starts at zero and every product leaves it so
+    const int64_t acc_elems = n * 8;
+    if (hacc_zero_ < (size_t)acc_elems * sizeof(unsigned long long)) {
+        hacc_.ensure((size_t)acc_elems * sizeof(unsigned long long));
+        k_fill<uint64_t><<<std::min<unsigned>(grid_for(acc_elems, 256), 8192), 256, 0, stream_>>>(
+            hacc_.as<uint64_t>(), acc_elems, 0ull);
+        hacc_zero_ = (size_t)acc_elems * sizeof(unsigned long long);
+    }
+    hcmax_.ensure(16 * sizeof(unsigned long long));
+    k_fill<uint64_t><<<1, 64, 0, stream_>>>(hcmax_.as<uint64_t>(), b, 0ull);
+    k_colmax<<<std::min<unsigned>(grid_for(n * b, 256), 2048), 256, 0, stream_>>>(
+        n, b, X, ldx, hcmax_.as<unsigned long long>());
+    HIPCHK(hipGetLastError());
+    const int64_t* rp = rowptr_.as<int64_t>();
+    const int32_t* cl = col_.as<int32_t>();
+    const double* vl = val_.as<double>();
+    const float* h = hi_.as<float>();
+    const float* l = lo_.as<float>();
+    const double* D = D_.as<double>();
+    const unsigned long long* cm = hcmax_.as<unsigned long long>();
+    unsigned long long* acc = hacc_.as<unsigned long long>();
+    const unsigned grid = grid_for(n, 4);
+    const unsigned gf = std::min<unsigned>(grid_for(n * b, 256), 8192);
+    auto go = [&](auto tag) {
+        constexpr int B = decltype(tag)::value;
+        if (mode == 0)
+            k_spmm_half<B, 0><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy, hnorm_, cm, acc);
+        else if (mode == 1)
+            k_spmm_half<B, 1><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy, hnorm_, cm, acc);
+        else
+            k_spmm_half<B, 2><<<grid, 256, 0, stream_>>>(n, rp, cl, vl, h, l, D, X32, X, Y, ldy, hnorm_, cm, acc);
+        HIPCHK(hipGetLastError());
+        k_half_finish<B><<<gf, 256, 0, stream_>>>(n, acc, hnorm_, cm, Y, ldy);
+        HIPCHK(hipGetLastError());
+    };
+    if (b == 4) go(std::integral_constant<int, 4>());
+    else go(std::integral_constant<int, 8>());
+}
+
 void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32, int ldx32) {
     const int64_t n = n_;
     if (matfree_) {
@@ -3066,6 +3503,10 @@ void Engine::spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low,
             case 3: spmm_free_t<12>(X, ldx, Y, ldy, b); break;
             default: spmm_free_t<16>(X, ldx, Y, ldy, b); break;
         }
+        return;
+    }
+    if (sym_) {
+        spmm_half(X, ldx, Y, ldy, b, low, X32, ldx32);
         return;
     }
     const int64_t* rp = rowptr_.as<int64_t>();
@@ -3251,9 +3692,24 @@ void Engine::apply_h_noalloc(const double* X, double* Y, int b, bool fp32) {
 
 // plain device copy of `bytes` bytes (16-byte words), timed with HIP events:
 // the calibration kernel for memory-counter readings
-__global__ void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, int64_t n) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        b[i] = a[i];
+// four 16-byte loads in flight per thread (n a multiple of 4 * the grid's
+// thread count is not required: the tail is copied one word at a time)
+typedef unsigned cp_u4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy16(const cp_u4* __restrict__ a, cp_u4* __restrict__ b, int64_t n) {
+    const int64_t nt = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t i = t;
+    for (; i + 3 * nt < n; i += 4 * nt) {
+        const cp_u4 v0 = __builtin_nontemporal_load(a + i);
+        const cp_u4 v1 = __builtin_nontemporal_load(a + i + nt);
+        const cp_u4 v2 = __builtin_nontemporal_load(a + i + 2 * nt);
+        const cp_u4 v3 = __builtin_nontemporal_load(a + i + 3 * nt);
+        __builtin_nontemporal_store(v0, b + i);
+        __builtin_nontemporal_store(v1, b + i + nt);
+        __builtin_nontemporal_store(v2, b + i + 2 * nt);
+        __builtin_nontemporal_store(v3, b + i + 3 * nt);
+    }
+    for (; i < n; i += nt) b[i] = a[i];
 }
 
 double Engine::copy_probe(size_t bytes) {
@@ -3267,8 +3723,11 @@ double Engine::copy_probe(size_t bytes) {
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
+    const unsigned g = std::min<unsigned>(grid_for(nw, 256), 4096);
+    // the first copy touches both buffers; the second is timed
+    k_copy16<<<g, 256, 0, stream_>>>(a.as<cp_u4>(), b.as<cp_u4>(), nw);
     HIPCHK(hipEventRecord(e0, stream_));
-    k_copy16<<<std::min<unsigned>(grid_for(nw, 256), 8192), 256, 0, stream_>>>(a.as<uint4>(), b.as<uint4>(), nw);
+    k_copy16<<<g, 256, 0, stream_>>>(a.as<cp_u4>(), b.as<cp_u4>(), nw);
     HIPCHK(hipEventRecord(e1, stream_));
     HIPCHK(hipEventSynchronize(e1));
     float ms = 0;
@@ -3343,7 +3802,10 @@ double Engine::spmm_bytes(int b, int mode) const {
     const double sx = mode == 1 ? 4.0 : 8.0;  // gathered X element bytes
     // FP32 products also read the FP64 X row of the diagonal term
     const double xdiag = mode == 1 ? n * b * 8.0 : 0.0;
-    return (double)nnz_ * (sv + 4) + 8.0 * (n + 1) + 8.0 * n + n * b * sx + n * b * 8.0 + xdiag;
+    // half storage: the upper triangle's entries, with the vector traffic
+    // counted twice (the transposed products read and write rows again)
+    const double vec = (sym_ ? 2.0 : 1.0) * (n * b * sx + n * b * 8.0);
+    return (double)nnz_ * (sv + 4) + 8.0 * (n + 1) + 8.0 * n + vec + xdiag;
 }
 
 void Engine::spmm_profile(int b) {
@@ -3362,7 +3824,7 @@ void Engine::gemm_profile_read(double out[3]) {
     out[2] = (double)prof_.gemm_calls;
 }
 
-void Engine::phase_stats(double out[15]) const {
+void Engine::phase_stats(double out[16]) const {
     out[0] = (double)last_h_.n;
     out[1] = (double)last_h_.nnz_offdiag;
     out[2] = (double)P_deltas_;
@@ -3378,6 +3840,7 @@ void Engine::phase_stats(double out[15]) const {
     out[12] = (double)last_h_.probed_rows;
     out[13] = (double)last_h_.carried;
     out[14] = (double)last_pt2_.chunks;
+    out[15] = (double)h_storage();
 }
 
 void Engine::spmm_profile_read(double out[3]) {
@@ -3419,7 +3882,7 @@ void Engine::dense_solve(int nev, std::vector<double>& E, EigStats& st) {
         k_fill<double><<<grid_for(n * n, 256), 256, 0, stream_>>>(A.as<double>(), n * n, 0.0);
         k_densify<<<(unsigned)n, 64, 0, stream_>>>(n, rowptr_.as<int64_t>(), col_.as<int32_t>(),
                                                    form_ == kFP64 ? val_.as<double>() : nullptr, hi_.as<float>(),
-                                                   lo_.as<float>(), D_.as<double>(), A.as<double>());
+                                                   lo_.as<float>(), D_.as<double>(), A.as<double>(), sym_ ? 1 : 0);
         HIPCHK(hipGetLastError());
     }
     // The lower triangle is reduced from the first column, as Eigen's
@@ -3927,7 +4390,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     // 15-mode, 100-state workload 7 % faster than m = 3.5 nev, and the
     // 8-36-mode request mix slightly faster too)
     const auto up = [&](double x) { return ((int)std::ceil(x) + bs - 1) / bs * bs; };
-    double mfac = 4.0;
+    double mfac = ks_mfac_;  // below 4 when half storage needed a smaller basis to fit
     if (const char* e = std::getenv("VHCI_KS_MFAC")) mfac = std::max(2.0, std::atof(e));
     int m = std::max(up(mfac * nev), up((double)nev + 6 * bs));
     if (const char* e = std::getenv("VHCI_KS_DIM")) m = std::max(nev + 2 * bs, (std::atoi(e) / bs) * bs);

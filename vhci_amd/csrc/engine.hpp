@@ -167,6 +167,8 @@ class Engine {
     // resident eigenvectors, row-major n x ncol
     // products regenerate H (no stored entries): chosen when CSR would not fit
     bool matrix_free() const { return matfree_; }
+    // storage of the last build: 0 full CSR, 1 upper triangle, 2 matrix-free
+    int h_storage() const { return matfree_ ? 2 : (sym_ ? 1 : 0); }
     int ritz_cols() const { return xcols_; }
     void get_vectors(double* C, int ncol) const;
     // PrintFreqs assignment scan of the first ncol resident eigenvectors on
@@ -222,7 +224,7 @@ class Engine {
     // heat-bath selection (the PT2 one after pt2()): candidates, unique, ms;
     // last PT2: external states, (external, variational) pairs, sum ms; key
     // bytes; PT2 states
-    void phase_stats(double out[15]) const;
+    void phase_stats(double out[16]) const;
     // algorithmic bytes of one product of b vectors: stored entries (value +
     // column), row pointers, the diagonal, X read once and Y written once
     double spmm_bytes(int b, int mode) const;
@@ -295,7 +297,18 @@ class Engine {
     int64_t built_n_ = 0;
     DevBuf perm_o_, inv_o_, rowptr_o_, col_o_, val_o_, hi_o_, lo_o_, extra_, oldcnt_, cursor_;
     bool matfree_ = false;  // products regenerate H from the hash (no CSR)
-    bool use_matrix_free(int64_t nnz) const;
+    // half storage: only the strict upper triangle (solver order) is stored;
+    // products add the transposed terms through a fixed-point accumulator
+    bool sym_ = false;
+    double hnorm_ = 0;  // max over rows of sum_j |H_ij| (off-diagonal), the accumulator's bound
+    DevBuf rowcnt_up_, rowabs_, hacc_, hcmax_, hpack_x_, hpack_y_;
+    size_t hacc_zero_ = 0;  // bytes of hacc_ known to hold zeros
+    // 0 full CSR, 1 half storage, 2 matrix-free: the first that fits beside
+    // the eigensolver's blocks (VHCI_H_STORAGE=full|half|free forces one);
+    // half storage may also shrink the Krylov-Schur basis (ks_mfac_) to fit
+    int choose_storage(int64_t nnz, int64_t nnz_up);
+    double ks_mfac_ = 4.0;  // Krylov-Schur basis size / nev
+    void spmm_half(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32, int ldx32);
     template <int NW>
     void spmm_free_t(const double* X, int ldx, double* Y, int ldy, int b);
     DevBuf rowcnt_, rowptr_, col_, val_, hi_, lo_, D_;
