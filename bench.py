@@ -258,35 +258,61 @@ def cpu_baseline(ctx, r, problem_text, threads):
     }
 
 
-SERVE_RANGES = dict(m_range=(8, 36), eps1_set=(0.5, 1.0, 2.0), nstate_range=(20, 200), eps2_set=(0.0, 0.01, 0.1))
+# the serving workload's distribution; eps2 = 0 draws on fields of more than
+# 20 modes become eps2 = 0.01 (synth.request_params: PT2 over every connected
+# state of such a field holds 10^8 to 10^9 states)
+SERVE_RANGES = dict(m_range=(8, 36), eps1_set=(0.5, 1.0, 2.0), nstate_range=(20, 200), eps2_set=(0.0, 0.01, 0.1),
+                    eps2_zero_max_modes=20)
 
 
-def serving(nreq, slots, load, seed=7, wide=False):
+def serving(nreq, slots, load, seed=7, wide=False, alone_file=None, solo_only=False):
     """Serving stream of the serving workload's distribution (SERVE_RANGES;
     requests 0..nreq-1 of its seeded sequence): each request alone on a
     one-slot server sets the solo FP64 rate; then Poisson arrivals at `load`
     x that rate go through a `slots`-slot server, once per form.  Latency =
-    submission to result."""
+    submission to result.  alone_file: the solo results are read from it
+    when it exists, else written to it (a long stream split over two runs)."""
     import threading
 
     texts = [synth.request_input(r, **SERVE_RANGES) for r in range(nreq)]
-    srv = vhci_amd.Server(slots=1)
     alone = []
-    for r, t in enumerate(texts):
-        rid = srv.submit(t)
-        srv.wait(rid)
-        a = srv.result(rid)
-        alone.append(a)
-        print("[bench] serving request %d alone (M %d, eps1 %g, Nstate %d, eps2 %g): %d states, PT2 space %d, "
-              "%.2f s" % ((r,) + synth.request_params(r, **SERVE_RANGES)[:4] +
-                          (a["nvar"], a["ns2"], a["t_end"] - a["t_start"])), file=sys.stderr, flush=True)
-    srv.close()
+    if alone_file and os.path.exists(alone_file):
+        saved = json.load(open(alone_file))
+        if saved["requests"] < nreq:
+            raise RuntimeError("%s holds %d solo results, %d needed" % (alone_file, saved["requests"], nreq))
+        for a in saved["alone"][:nreq]:
+            a["E"] = np.array(a["E"])
+            a["checksum"] = int(a["checksum"])
+            alone.append(a)
+        print("[bench] serving: %d solo results read from %s" % (nreq, alone_file), file=sys.stderr, flush=True)
+    else:
+        srv = vhci_amd.Server(slots=1)
+        for r, t in enumerate(texts):
+            rid = srv.submit(t)
+            srv.wait(rid)
+            a = srv.result(rid)
+            alone.append(a)
+            print("[bench] serving request %d alone (M %d, eps1 %g, Nstate %d, eps2 %g): %d states, PT2 space %d, "
+                  "%.2f s" % ((r,) + synth.request_params(r, **SERVE_RANGES)[:4] +
+                              (a["nvar"], a["ns2"], a["t_end"] - a["t_start"])), file=sys.stderr, flush=True)
+            if alone_file:
+                # rewritten after every request, so a run cut short keeps what it finished
+                keep = ("nvar", "ns2", "checksum", "t_start", "t_end", "peak_bytes")
+                rows = [dict({k: x[k] for k in keep}, E=x["E"].tolist()) for x in alone]
+                with open(alone_file + ".tmp", "w") as f:
+                    json.dump({"requests": len(rows), "alone": rows}, f)
+                os.replace(alone_file + ".tmp", alone_file)
+        srv.close()
+    if solo_only:
+        return None
     service = [a["t_end"] - a["t_start"] for a in alone]
     rate = load * nreq / sum(service)
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate, size=nreq)
-    out = {"requests": nreq, "slots": slots, "ranges": {k: list(v) for k, v in SERVE_RANGES.items()},
+    out = {"requests": nreq, "slots": slots,
+           "ranges": {k: (list(v) if isinstance(v, tuple) else v) for k, v in SERVE_RANGES.items()},
            "solo_fp64_solves_per_s": nreq / sum(service), "offered_rate_per_s": rate,
+           "solo_p50_s": percentile(service, 50), "solo_p99_s": percentile(service, 99),
            "basis_sizes": [a["nvar"] for a in alone], "pt2_sizes": [a["ns2"] for a in alone]}
     for name, form in (("fp64", vhci_amd.FP64), ("mixed", vhci_amd.MIXED)):
         srv = vhci_amd.Server(slots=slots)
@@ -478,6 +504,10 @@ def main():
                     help="also run the 48-mode naphthalene field at the large workload's parameters")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-load", type=float, default=1.2, help="offered load / solo FP64 rate")
+    ap.add_argument("--serve-alone-file", default=None,
+                    help="solo results of the serving stream: read when the file exists, else written")
+    ap.add_argument("--serve-solo-only", action="store_true",
+                    help="with --serve-only: only the solo pass (writes --serve-alone-file)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the counter passes (traffic is then null)")
     ap.add_argument("--request-sizes", action="store_true",
                     help="a third counter pass: the SpMM's read requests by size (32, 64, 128 bytes)")
@@ -495,7 +525,14 @@ def main():
         print(json.dumps(large_field_run()), flush=True)
         return
     if args.serve_only:
-        print(json.dumps(serving(args.serve, args.serve_slots, args.serve_load, wide=True)), flush=True)
+        if args.serve_solo_only:
+            if not args.serve_alone_file or os.path.exists(args.serve_alone_file):
+                sys.exit("--serve-solo-only needs a --serve-alone-file that does not exist yet")
+            serving(args.serve, 1, args.serve_load, alone_file=args.serve_alone_file, solo_only=True)
+            print(json.dumps({"serving_solo": args.serve, "file": args.serve_alone_file}), flush=True)
+            return
+        print(json.dumps(serving(args.serve, args.serve_slots, args.serve_load, wide=True,
+                                 alone_file=args.serve_alone_file)), flush=True)
         return
     wname = args.workload
     cnum = int(wname[1:])

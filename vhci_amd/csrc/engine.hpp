@@ -208,9 +208,12 @@ class Engine {
     // entries per call of the row sort of the H build (rocPRIM counts in 32 bits)
     static constexpr int64_t kSortSlice = int64_t(1) << 30;
     // PT2 chunk budgets (VHCI_PT2_CHUNK_PAIRS / VHCI_PT2_CHUNK_STATES
-    // override them): (a, i) pairs (24 bytes each with the sort's copies)
-    // and external states (the dedup pool) one chunk may hold
-    static constexpr int64_t kPt2ChunkPairs = int64_t(1) << 30;
+    // override them): (a, i) pairs (24 bytes each with the sort's copies,
+    // 51.5 GB at the budget) and external states (the dedup pool) one chunk
+    // may hold.  Every chunk probes all connection patterns of every
+    // variational state twice (count, fill), so fewer chunks cost less; a
+    // chunk whose buffers cannot be allocated is split.
+    static constexpr int64_t kPt2ChunkPairs = int64_t(1) << 31;
     static constexpr int64_t kPt2ChunkStates = int64_t(1) << 28;
     // per-launch timing of the b-column products that the solver issues:
     // b > 0 starts (and clears) collection, b = 0 stops it.  The read gives

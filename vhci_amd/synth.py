@@ -233,14 +233,24 @@ def config_input(c, path=None, **override):
 
 
 def request_params(r, m_range=(8, 36), eps1_set=(0.5, 1.0, 2.0), nstate_range=(20, 200),
-                   eps2_set=(0.0, 0.01, 0.1)):
-    """Parameters of serving request r: (M, eps1, Nstate, eps2, field seed)."""
+                   eps2_set=(0.0, 0.01, 0.1), eps2_zero_max_modes=None):
+    """Parameters of serving request r: (M, eps1, Nstate, eps2, field seed).
+
+    eps2 = 0 makes PT2 run over every connected state.  On fields of more
+    than about 20 modes that space holds 10^8 to 10^9 states (1.4e8 for
+    request 3, 25 modes, 43 s on one MI355X; request 11, 35 modes, ran for
+    over 3 minutes), so a stream of seconds-long requests would be set by a
+    handful of them.  With eps2_zero_max_modes = m, an eps2 = 0 draw on a
+    field of more than m modes becomes eps2 = 0.01 (the same random draws
+    otherwise, so every other request is unchanged)."""
     seed = BASE_SEED + 1000 + r
     rng = Rng(seed)
     M = m_range[0] + rng.randint(m_range[1] - m_range[0] + 1)
     eps1 = eps1_set[rng.randint(len(eps1_set))]
     nstate = nstate_range[0] + rng.randint(nstate_range[1] - nstate_range[0] + 1)
     eps2 = eps2_set[rng.randint(len(eps2_set))]
+    if eps2 == 0.0 and eps2_zero_max_modes is not None and M > eps2_zero_max_modes:
+        eps2 = 0.01
     return M, eps1, nstate, eps2, (seed + (1 << 40)) & MASK
 
 
