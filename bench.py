@@ -265,7 +265,8 @@ SERVE_RANGES = dict(m_range=(8, 36), eps1_set=(0.5, 1.0, 2.0), nstate_range=(20,
                     eps2_zero_max_modes=20)
 
 
-def serving(nreq, slots, load, seed=7, wide=False, alone_file=None, solo_only=False):
+def serving(nreq, slots, load, seed=7, wide=False, alone_file=None, solo_only=False, forms=("fp64", "mixed"),
+            saturated=True):
     """Serving stream of the serving workload's distribution (SERVE_RANGES;
     requests 0..nreq-1 of its seeded sequence): each request alone on a
     one-slot server sets the solo FP64 rate; then Poisson arrivals at `load`
@@ -315,6 +316,8 @@ def serving(nreq, slots, load, seed=7, wide=False, alone_file=None, solo_only=Fa
            "solo_p50_s": percentile(service, 50), "solo_p99_s": percentile(service, 99),
            "basis_sizes": [a["nvar"] for a in alone], "pt2_sizes": [a["ns2"] for a in alone]}
     for name, form in (("fp64", vhci_amd.FP64), ("mixed", vhci_amd.MIXED)):
+        if name not in forms:
+            continue
         srv = vhci_amd.Server(slots=slots)
         ids = [None] * nreq
 
@@ -348,11 +351,12 @@ def serving(nreq, slots, load, seed=7, wide=False, alone_file=None, solo_only=Fa
         print("[bench] serving %s: %.2f solves/s p50 %.3f s p99 %.3f s" % (name, nreq / span, out[name]["p50_latency_s"],
                                                                        out[name]["p99_latency_s"]),
               file=sys.stderr, flush=True)
-    out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
+    if "mixed" in out and "fp64" in out:
+        out["mixed_over_fp64"] = out["mixed"]["solves_per_s"] / out["fp64"]["solves_per_s"]
     # capacity: every request submitted at once (FP64), served rate =
     # requests / makespan; the arrival stream above is paced below it
     out["saturated_fp64"] = {}
-    for ns in sorted({slots} | ({2 * slots} if wide else set())):
+    for ns in (sorted({slots} | ({2 * slots} if wide else set())) if saturated else []):
         srv = vhci_amd.Server(slots=ns)
         ids = [srv.submit(t) for t in texts]
         res = []
@@ -508,6 +512,8 @@ def main():
                     help="solo results of the serving stream: read when the file exists, else written")
     ap.add_argument("--serve-solo-only", action="store_true",
                     help="with --serve-only: only the solo pass (writes --serve-alone-file)")
+    ap.add_argument("--serve-forms", default="fp64,mixed", help="forms of the arrival-stream passes")
+    ap.add_argument("--serve-no-saturated", action="store_true", help="skip the capacity passes")
     ap.add_argument("--no-traffic", action="store_true", help="skip the counter passes (traffic is then null)")
     ap.add_argument("--request-sizes", action="store_true",
                     help="a third counter pass: the SpMM's read requests by size (32, 64, 128 bytes)")
@@ -532,7 +538,8 @@ def main():
             print(json.dumps({"serving_solo": args.serve, "file": args.serve_alone_file}), flush=True)
             return
         print(json.dumps(serving(args.serve, args.serve_slots, args.serve_load, wide=True,
-                                 alone_file=args.serve_alone_file)), flush=True)
+                                 alone_file=args.serve_alone_file, forms=args.serve_forms.split(","),
+                                 saturated=not args.serve_no_saturated)), flush=True)
         return
     wname = args.workload
     cnum = int(wname[1:])

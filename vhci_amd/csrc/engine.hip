@@ -1767,6 +1767,127 @@ __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double 
     }
 }
 
+// ------------------------------------------------------------------------
+// Ritz rotation X = A S on the matrix cores (v_mfma_f64_16x16x4f64): A is
+// n x p (column-major, lda; the Krylov basis or its product block), S is
+// p x q (column-major, lds; the kept eigenvectors of the projected matrix),
+// X is n x q (column-major, ldx).  A workgroup of four waves takes 64 rows
+// and CT column tiles of 16 (all q columns for q <= 16 CT), so A is read from
+// HBM once; the wave's 16 rows x 16 CT columns accumulate in 4 CT registers
+// per lane.  S is staged through LDS in chunks of 32 of its rows.  MFMA
+// operands: lane l holds A[i][k] and S[k][j] with i = j = l % 16, k = l / 16,
+// and gets X[l / 16 + 4 r][l % 16] in register r.  Each product is summed
+// in a fixed order (k ascending in steps of 4 inside the MFMA), so the
+// rotation is reproducible bit for bit.
+// ------------------------------------------------------------------------
+typedef double rot_d4 __attribute__((ext_vector_type(4)));
+constexpr int kRotKC = 16;  // rows of S per LDS chunk
+
+// S (p x q, column-major, lds) as launch tiles of TW = 16 CT columns, each
+// pk x TW with zeros beyond p rows and q columns (pk = p rounded up to
+// kRotKC): every LDS chunk is then one contiguous block.  Within a row k of
+// a tile, column 16 t + j sits at position CT j + t, so that the lane of
+// column j reads its CT operands of the row as CT / 2 16-byte words.
+__global__ void k_rot_stage(const double* __restrict__ S, int lds, int p, int q, int pk, int CT, int ntile,
+                            double* __restrict__ Sr) {
+    const int TW = 16 * CT;
+    const int64_t total = (int64_t)ntile * pk * TW;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t tile = t / ((int64_t)pk * TW);
+        const int64_t rem = t - tile * pk * TW;
+        const int k = (int)(rem / TW), pos = (int)(rem - (int64_t)k * TW);
+        const int j = pos / CT, tt = pos - j * CT;
+        const int c = (int)tile * TW + 16 * tt + j;
+        Sr[t] = k < p && c < q ? S[k + (int64_t)c * lds] : 0.0;
+    }
+}
+
+template <int CT>
+__global__ __launch_bounds__(256) void k_rot_mfma(int64_t n, int p, int pk, int q, const double* __restrict__ A,
+                                                  int64_t lda, const double* __restrict__ Sr, double* __restrict__ X,
+                                                  int64_t ldx) {
+    constexpr int TW = 16 * CT;
+    constexpr int SPT = kRotKC * TW / 256;  // S elements each thread stages per chunk
+    __shared__ __align__(16) double Ss[kRotKC * TW];
+    const int l = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int c0 = blockIdx.y * TW;  // first column of this launch tile
+    const double* St = Sr + (int64_t)blockIdx.y * pk * TW;
+    const int64_t rbase = (int64_t)blockIdx.x * 64 + w * 16;
+    const int64_t arow = rbase + (l & 15);
+    const bool rin = arow < n;
+    const int kl = l >> 4;
+    rot_d4 acc[CT];
+#pragma unroll
+    for (int t = 0; t < CT; t++) acc[t] = rot_d4{0.0, 0.0, 0.0, 0.0};
+    // chunk 0 in registers; each step stages the next one while the
+    // current one is multiplied
+    double a_cur[kRotKC / 4], a_nxt[kRotKC / 4], s_nxt[SPT];
+#pragma unroll
+    for (int s = 0; s < kRotKC / 4; s++) {
+        const int gk = 4 * s + kl;
+        a_cur[s] = rin && gk < p ? A[arow + (int64_t)gk * lda] : 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < SPT; i++) s_nxt[i] = St[threadIdx.x + 256 * i];
+    for (int k0 = 0; k0 < pk; k0 += kRotKC) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < SPT; i++) Ss[threadIdx.x + 256 * i] = s_nxt[i];
+        __syncthreads();
+        const int k1 = k0 + kRotKC;
+        if (k1 < pk) {
+#pragma unroll
+            for (int s = 0; s < kRotKC / 4; s++) {
+                const int gk = k1 + 4 * s + kl;
+                a_nxt[s] = rin && gk < p ? A[arow + (int64_t)gk * lda] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < SPT; i++) s_nxt[i] = St[(int64_t)k1 * TW + threadIdx.x + 256 * i];
+        }
+#pragma unroll
+        for (int s = 0; s < kRotKC / 4; s++) {
+            const double2* srow = reinterpret_cast<const double2*>(Ss + (4 * s + kl) * TW + (l & 15) * CT);
+#pragma unroll
+            for (int t = 0; t < CT; t += 2) {
+                const double2 b = srow[t / 2];
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_cur[s], b.x, acc[t], 0, 0, 0);
+                acc[t + 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_cur[s], b.y, acc[t + 1], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < kRotKC / 4; s++) a_cur[s] = a_nxt[s];
+    }
+#pragma unroll
+    for (int t = 0; t < CT; t++) {
+        const int col = c0 + 16 * t + (l & 15);
+        if (col >= q) continue;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int64_t row = rbase + kl + 4 * r;
+            if (row < n) X[row + (int64_t)col * ldx] = acc[t][r];
+        }
+    }
+}
+
+// X = A S (see k_rot_mfma); work holds the staged S
+static void rot_mfma(hipStream_t s, int64_t n, int p, int q, const double* A, int64_t lda, const double* S, int lds,
+                     double* X, int64_t ldx, DevBuf& work) {
+    const unsigned gx = grid_for(n, 64);
+    const int pk = (p + kRotKC - 1) / kRotKC * kRotKC;
+    const int TW = q <= 128 ? 128 : 256;
+    const int ntile = (q + TW - 1) / TW;
+    work.ensure((size_t)ntile * pk * TW * sizeof(double));
+    k_rot_stage<<<grid_for((int64_t)ntile * pk * TW, 256), 256, 0, s>>>(S, lds, p, q, pk, TW / 16, ntile,
+                                                                        work.as<double>());
+    HIPCHK(hipGetLastError());
+    if (TW == 128)
+        k_rot_mfma<8><<<dim3(gx, ntile), 256, 0, s>>>(n, p, pk, q, A, lda, work.as<double>(), X, ldx);
+    else
+        k_rot_mfma<16><<<dim3(gx, ntile), 256, 0, s>>>(n, p, pk, q, A, lda, work.as<double>(), X, ldx);
+    HIPCHK(hipGetLastError());
+}
+
 // R (p x k, column-major, ld p) holds V^T T in rows 0..m0 and T^T T in rows
 // m0..m0+k.  *gate = 1 if any projection coefficient exceeds rel times the
 // smallest column norm of T, else 0 (one workgroup)
@@ -4546,11 +4667,20 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         // at random (the relation then misses what was dropped), with H X
         // from the product since W's kept columns are not carried in FP64.
         const bool relation = !low && !refilled;
-        BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, V, ld, Gw, m,
-                              &zero, X, ld));
-        if (low)
-            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, W, ld, Gw,
-                                  m, &zero, HX, ld));
+        // the rotation X = V S (and H X = W S for FP32 cycles) on the
+        // matrix cores; VHCI_ROT=blas takes rocBLAS dgemm instead
+        const char* rot_env = std::getenv("VHCI_ROT");
+        const bool rot_blas = rot_env && std::string(rot_env) == "blas";
+        if (rot_blas) {
+            BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, V, ld, Gw,
+                                  m, &zero, X, ld));
+            if (low)
+                BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, keep, m, &one, W, ld,
+                                      Gw, m, &zero, HX, ld));
+        } else {
+            rot_mfma(stream_, n, m, keep, V, ld, Gw, m, X, ld, rot_s_);
+            if (low) rot_mfma(stream_, n, m, keep, W, ld, Gw, m, HX, ld, rot_s_);
+        }
         if (gprof) {
             HIPCHK(hipEventRecord(prof_.gev[1], stream_));
             HIPCHK(hipEventSynchronize(prof_.gev[1]));
@@ -5260,7 +5390,7 @@ void Engine::pt2_t(const std::vector<double>& E, int nopt, double eps2, Pt2Stats
 // eigensolve allocates them again
 void Engine::release_solver_work() {
     for (DevBuf* b : {&V_, &W_, &HX_, &Xw_, &T_b_, &G_, &S_, &part_, &X32_, &stage_in_, &stage_in32_, &stage_out_,
-                      &otmp_, &Gw_, &tsm_part_, &rr_part_})
+                      &otmp_, &Gw_, &tsm_part_, &rr_part_, &rot_s_})
         b->release();
     pool_.trim();
 }

@@ -326,7 +326,7 @@ class Engine {
     int xcols_ = 0;
     // eigensolver work
     DevBuf V_, W_, HX_, Xw_, T_b_, G_, S_, theta_, work_, part_, X32_, info_;
-    DevBuf stage_in_, stage_in32_, stage_out_, gram_, otmp_, Gw_, tsm_part_, rr_part_;
+    DevBuf stage_in_, stage_in32_, stage_out_, gram_, otmp_, Gw_, tsm_part_, rr_part_, rot_s_;
     // heat-bath work
     DevBuf cmax_, candcnt_, candoff_, nslots_, pool_keys_, pool_rank_, counters_, col_rank_, col_idx_,
         srt_rank_, srt_idx_, newkeys_, scan_tmp_;
