@@ -330,6 +330,7 @@ def serving(nreq, slots, load, seed=7, wide=False, alone_file=None, solo_only=Fa
                 if dt > 0:
                     time.sleep(dt)
                 ids[i] = srv.submit(t, form=form)
+                print("[bench] serving %s: request %d submitted" % (name, i), file=sys.stderr, flush=True)
 
         th = threading.Thread(target=feed)
         th.start()

@@ -76,3 +76,30 @@ def test_c1_chunked_matches_one_chunk(monkeypatch):
     assert nsc == r1["ns2"]
     assert phc["pairs"] == ph1["pairs"]
     assert np.max(np.abs((E + dEc) - r1["E"])) < 1e-9
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_c1_one_pass_matches_counted(form, monkeypatch):
+    """PT2 pairs written in one pass (hits queued per wave, sorted by
+    (a, i)) give the counted form's sorted pairs, so the corrections are
+    the same bits; also with a pair budget that forces several chunks."""
+    p = vhci_amd.Problem(text=synth.config_input(1))
+    ctx = vhci_amd.Context(p, form)
+    r1 = ctx.run()
+    ph1 = ctx.phase_stats()
+    E = r1["E_var"]
+    nopt = len(E)
+    monkeypatch.setenv("VHCI_PT2_COUNTED", "1")
+    dEc, nsc = ctx.pt2(E, nopt, p.eps2)
+    phc = ctx.phase_stats()
+    monkeypatch.setenv("VHCI_PT2_CHUNK_PAIRS", str(1 << 26))
+    monkeypatch.delenv("VHCI_PT2_COUNTED")
+    dE4, ns4 = ctx.pt2(E, nopt, p.eps2)
+    monkeypatch.setenv("VHCI_PT2_COUNTED", "1")
+    dE4c, ns4c = ctx.pt2(E, nopt, p.eps2)
+    ctx.close()
+    print("c1 PT2 one pass vs counted: pairs", ph1["pairs"], phc["pairs"], flush=True)
+    assert nsc == r1["ns2"] and ns4 == ns4c == r1["ns2"]
+    assert phc["pairs"] == ph1["pairs"]
+    assert np.array_equal(E + dEc, r1["E"])
+    assert np.array_equal(dE4, dE4c)

@@ -2417,6 +2417,94 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
     if (!FILL && lane == 0) a.cnt[blockIdx.x] = cnt;
 }
 
+// PT2 pairs in one pass: each wave queues its row's hits in LDS and, when
+// the queue fills or the row ends, reserves space for them with one atomic
+// add on a global cursor.  Rows interleave in the output, so a pair is
+// written as the key (a << 32 | i) with its pattern as the value; sorting by
+// the key (unique: one pattern connects a given pair) restores the order
+// of the counted form, a-major and ascending i, on every run.  Pairs beyond
+// the capacity are counted but not written (the caller then counts exactly
+// and fills again).
+constexpr int kPushQ = 256;  // LDS queue of a wave, in pairs
+template <int NW>
+__global__ __launch_bounds__(64) void k_pt2_push1(Pt2PushArgs a, unsigned long long* __restrict__ cursor,
+                                                  uint64_t* __restrict__ okey, uint32_t* __restrict__ oval,
+                                                  unsigned long long cap) {
+    __shared__ uint8_t sq[64];
+    __shared__ uint64_t cf[64];
+    __shared__ uint64_t qk[kPushQ];
+    __shared__ uint32_t qv[kPushQ];
+    const int lane = threadIdx.x;
+    const int64_t row = blockIdx.x;
+    uint32_t w[NW];
+    key_load<NW>(a.vkeys, row, w);
+    if (lane < a.M) sq[lane] = (uint8_t)key_q<NW>(w, lane);
+    load_coefs(cf);
+    __syncthreads();
+    const uint64_t h0 = row_lin(sq, a.M, cf);
+    int nq = 0;  // queued pairs (wave-uniform)
+    auto flush = [&]() {
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(cursor, (unsigned long long)nq);
+        base = __shfl(base, 0);
+        for (int t = lane; t < nq; t += 64) {
+            const unsigned long long pos = base + (unsigned long long)t;
+            if (pos < cap) {
+                okey[pos] = qk[t];
+                oval[pos] = qv[t];
+            }
+        }
+        nq = 0;
+    };
+    int blk = -1, pb0 = 0, pb1 = 0;
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    for (int p0 = pb0; p0 < pb1; p0 += kWave) {
+        const int p = p0 + lane;
+        bool hit = false;
+        int64_t idx = -1;
+        if (p < pb1) {
+            const uint4 e = rec_load(a.deltas, p);
+            const int nz = delta_nz(e);
+            if (pattern_ok(e, nz, sq)) {
+                const uint64_t h = fmix64(h0 + pattern_shift(e, nz, cf));
+                const uint64_t tag = h >> 32;
+                if (tag >= a.tlo && tag < a.thi && bloom_maybe(a.abloom, a.abmask, h)) {
+                    idx = probe_shift<NW>(a.aslots, a.amask, a.apool, w, e, nz, h);
+                    hit = idx >= 0;
+                }
+            }
+        }
+        const uint64_t bal = __ballot(hit);
+        const int nh = __popcll(bal);
+        if (nh == 0) continue;
+        if (nq + nh > kPushQ) {
+            __syncthreads();
+            flush();
+            __syncthreads();
+        }
+        if (hit) {
+            const int t = nq + __popcll(bal & ((1ull << lane) - 1ull));
+            const uint32_t ai = a.apos ? a.apos[idx] : (uint32_t)idx;
+            qk[t] = ((uint64_t)ai << 32) | (uint32_t)row;
+            qv[t] = (uint32_t)p;
+        }
+        nq += nh;
+    }
+    __syncthreads();
+    if (nq > 0) flush();
+}
+
+// (a << 32 | i, pattern) pairs sorted by key into the counted form's
+// layout: key a, value (i << 32 | pattern)
+__global__ void k_pair_split(const uint64_t* __restrict__ skey, const uint32_t* __restrict__ sval, int64_t npairs,
+                             uint32_t* __restrict__ akey, uint64_t* __restrict__ ival) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < npairs; t += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = skey[t];
+        akey[t] = (uint32_t)(k >> 32);
+        ival[t] = (k << 32) | sval[t];
+    }
+}
+
 // canonical position of every pool entry (inverse of the sort by rank)
 __global__ void k_pool_pos(const uint32_t* __restrict__ sorted_idx, int64_t u, uint32_t* __restrict__ pos) {
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -5155,6 +5243,7 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
     // pairs estimated from every 64th state first: a chunk far over the
     // budget is split without a full count
     constexpr int64_t kSample = 64;
+    double est = -1.0;
     if (n >= 64 * kSample) {
         const int64_t ns = (n + kSample - 1) / kSample;
         a.rstride = kSample;
@@ -5163,57 +5252,111 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
         std::vector<int64_t> sc(ns);
         HIPCHK(hipMemcpyAsync(sc.data(), a.cnt, ns * sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
-        const double est = (double)std::accumulate(sc.begin(), sc.end(), int64_t(0)) * ((double)n / (double)ns);
+        est = (double)std::accumulate(sc.begin(), sc.end(), int64_t(0)) * ((double)n / (double)ns);
         if (est > 1.25 * (double)max_pairs) {
             last_pairs_ = (int64_t)(1.25 * est);
             return false;
         }
     }
-    a.rstride = 1;
-    k_pt2_push<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
-    k_fill<int64_t><<<1, 1, 0, stream_>>>(a.cnt + n, 1, 0);
-    size_t tmp = 0;
-    HIPCHK(rocprim::exclusive_scan(nullptr, tmp, a.cnt, pairoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
-                                   rocprim::plus<int64_t>(), stream_));
-    scan_tmp_.ensure(tmp);
-    HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, a.cnt, pairoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
-                                   rocprim::plus<int64_t>(), stream_));
-    int64_t npairs = 0;
-    HIPCHK(hipMemcpyAsync(&npairs, pairoff_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
-    HIPCHK(hipStreamSynchronize(stream_));
+    unsigned endbit = 1;
+    while (endbit < 32 && ((uint64_t)ns2 >> endbit) != 0) endbit++;
+    DevBuf pk, pv, sk, sv, seg, cur;
+    int64_t npairs = -1;
+    const uint32_t* skp = nullptr;  // sorted: external state of each pair
+    const uint64_t* svp = nullptr;  // sorted: (variational state << 32 | pattern)
+    // One pass (k_pt2_push1) into buffers sized from the estimate; the
+    // counted form (count, scan, fill) when there is no estimate, when the
+    // pairs overflow the buffers, or with VHCI_PT2_COUNTED=1.  Both give the
+    // same sorted pairs.
+    const char* ce = std::getenv("VHCI_PT2_COUNTED");
+    if (est >= 0.0 && !(ce && std::string(ce) == "1")) {
+        const int64_t cap = std::min<int64_t>(max_pairs, (int64_t)(1.25 * est) + (int64_t(1) << 20));
+        bool ok = true;
+        try {
+            pk.ensure((size_t)cap * sizeof(uint64_t));  // keys (a << 32 | i); after the sort, the values
+            pv.ensure((size_t)cap * sizeof(uint32_t));  // patterns; after the sort, the external states
+            sk.ensure((size_t)cap * sizeof(uint64_t));
+            sv.ensure((size_t)cap * sizeof(uint32_t));
+            seg.ensure((ns2 + 1) * sizeof(int64_t));
+            cur.ensure(sizeof(unsigned long long));
+        } catch (const std::runtime_error&) {
+            ok = false;
+        }
+        if (ok) {
+            k_fill<uint64_t><<<1, 64, 0, stream_>>>(cur.as<uint64_t>(), 1, 0ull);
+            k_pt2_push1<NW><<<(unsigned)n, 64, 0, stream_>>>(a, cur.as<unsigned long long>(), pk.as<uint64_t>(),
+                                                             pv.as<uint32_t>(), (unsigned long long)cap);
+            HIPCHK(hipGetLastError());
+            unsigned long long got = 0;
+            HIPCHK(hipMemcpyAsync(&got, cur.p, sizeof(got), hipMemcpyDeviceToHost, stream_));
+            HIPCHK(hipStreamSynchronize(stream_));
+            if ((int64_t)got <= cap) {
+                npairs = (int64_t)got;
+                if (npairs > 0) {
+                    size_t stmp = 0;
+                    HIPCHK(rocprim::radix_sort_pairs(nullptr, stmp, pk.as<uint64_t>(), sk.as<uint64_t>(),
+                                                     pv.as<uint32_t>(), sv.as<uint32_t>(), (size_t)npairs, 0,
+                                                     32 + endbit, stream_));
+                    scan_tmp_.ensure(stmp);
+                    HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, stmp, pk.as<uint64_t>(), sk.as<uint64_t>(),
+                                                     pv.as<uint32_t>(), sv.as<uint32_t>(), (size_t)npairs, 0,
+                                                     32 + endbit, stream_));
+                    k_pair_split<<<std::min<unsigned>(grid_for(npairs, 256), 65536), 256, 0, stream_>>>(
+                        sk.as<uint64_t>(), sv.as<uint32_t>(), npairs, pv.as<uint32_t>(), pk.as<uint64_t>());
+                    HIPCHK(hipGetLastError());
+                }
+                skp = pv.as<uint32_t>();
+                svp = pk.as<uint64_t>();
+            }
+        }
+    }
+    if (npairs < 0) {
+        a.rstride = 1;
+        k_pt2_push<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
+        k_fill<int64_t><<<1, 1, 0, stream_>>>(a.cnt + n, 1, 0);
+        size_t tmp = 0;
+        HIPCHK(rocprim::exclusive_scan(nullptr, tmp, a.cnt, pairoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                       rocprim::plus<int64_t>(), stream_));
+        scan_tmp_.ensure(tmp);
+        HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, a.cnt, pairoff_.as<int64_t>(), (int64_t)0, (size_t)(n + 1),
+                                       rocprim::plus<int64_t>(), stream_));
+        HIPCHK(hipMemcpyAsync(&npairs, pairoff_.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        if (npairs > max_pairs) {
+            last_pairs_ = npairs;
+            return false;
+        }
+        try {
+            pk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
+            pv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
+            sk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
+            sv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
+            seg.ensure((ns2 + 1) * sizeof(int64_t));
+        } catch (const std::runtime_error&) {
+            return false;  // the chunk is split instead
+        }
+        a.off = pairoff_.as<int64_t>();
+        a.pkey = pk.as<uint32_t>();
+        a.pval = pv.as<uint64_t>();
+        if (npairs > 0) {
+            k_pt2_push<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
+            HIPCHK(hipGetLastError());
+            size_t stmp = 0;
+            HIPCHK(rocprim::radix_sort_pairs(nullptr, stmp, pk.as<uint32_t>(), sk.as<uint32_t>(), pv.as<uint64_t>(),
+                                             sv.as<uint64_t>(), (size_t)npairs, 0, endbit, stream_));
+            scan_tmp_.ensure(stmp);
+            HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, stmp, pk.as<uint32_t>(), sk.as<uint32_t>(),
+                                             pv.as<uint64_t>(), sv.as<uint64_t>(), (size_t)npairs, 0, endbit,
+                                             stream_));
+        }
+        skp = sk.as<uint32_t>();
+        svp = sv.as<uint64_t>();
+    }
     if (std::getenv("VHCI_DEBUG"))
-        std::fprintf(stderr, "[pt2] chunk [%#llx, %#llx): %lld external states, %lld pairs\n", (unsigned long long)tlo,
-                     (unsigned long long)thi, (long long)ns2, (long long)npairs);
-    if (npairs > max_pairs) {
-        last_pairs_ = npairs;
-        return false;
-    }
-    DevBuf pk, pv, sk, sv, seg;
-    try {
-        pk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
-        pv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
-        sk.ensure(std::max<int64_t>(1, npairs) * sizeof(uint32_t));
-        sv.ensure(std::max<int64_t>(1, npairs) * sizeof(uint64_t));
-        seg.ensure((ns2 + 1) * sizeof(int64_t));
-    } catch (const std::runtime_error&) {
-        return false;  // the chunk is split instead
-    }
-    a.off = pairoff_.as<int64_t>();
-    a.pkey = pk.as<uint32_t>();
-    a.pval = pv.as<uint64_t>();
-    if (npairs > 0) {
-        k_pt2_push<NW, true><<<(unsigned)n, 64, 0, stream_>>>(a);
-        HIPCHK(hipGetLastError());
-        unsigned endbit = 1;
-        while (endbit < 32 && ((uint64_t)ns2 >> endbit) != 0) endbit++;
-        size_t stmp = 0;
-        HIPCHK(rocprim::radix_sort_pairs(nullptr, stmp, pk.as<uint32_t>(), sk.as<uint32_t>(), pv.as<uint64_t>(),
-                                         sv.as<uint64_t>(), (size_t)npairs, 0, endbit, stream_));
-        scan_tmp_.ensure(stmp);
-        HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, stmp, pk.as<uint32_t>(), sk.as<uint32_t>(), pv.as<uint64_t>(),
-                                         sv.as<uint64_t>(), (size_t)npairs, 0, endbit, stream_));
-    }
-    k_seg_start<<<grid_for(ns2 + 1, 256), 256, 0, stream_>>>(sk.as<uint32_t>(), npairs, ns2, seg.as<int64_t>());
+        std::fprintf(stderr, "[pt2] chunk [%#llx, %#llx): %lld external states, %lld pairs (%s)\n",
+                     (unsigned long long)tlo, (unsigned long long)thi, (long long)ns2, (long long)npairs,
+                     skp == sk.as<uint32_t>() ? "counted" : "one pass");
+    k_seg_start<<<grid_for(ns2 + 1, 256), 256, 0, stream_>>>(skp, npairs, ns2, seg.as<int64_t>());
     HIPCHK(hipGetLastError());
     const bool mixed = form_ == kMixed;
     Pt2NumArgs b{};
@@ -5223,7 +5366,7 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
     b.nopt = nopt;
     b.akeys = newkeys_.as<uint4>();
     b.seg = seg.as<int64_t>();
-    b.sval = sv.as<uint64_t>();
+    b.sval = svp;
     b.deltas = d_deltas_.as<DeltaEnt>();
     b.groups = d_groups_.as<GroupEnt>();
     b.fclist = d_fclist_.as<uint32_t>();

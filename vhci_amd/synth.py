@@ -26,6 +26,14 @@ Workloads (seed = 0x5648434900 + config):
       frequency, as in real force fields, and the basis converges after
       three iterations at 1,598,434 states (1225 -> 1,003,305 ->
       1,548,210 -> 1,598,434) with about 1.7e9 nonzeros.
+  c4  sextic stress case, 60 modes, frequencies in [100, 3300]: cubic and
+      quartic constants as for c2 (all 37,820 cubic f_ijk, and the
+      semi-diagonal plus every other f_iijk quartic with |f| >= 0.5 kept:
+      43,722), plus all 7,140 semi-diagonal quintic (i^5, i^4 j, i^3 j^2)
+      and 8,910 semi-diagonal sextic (i^6, i^5 j, i^4 j^2, i^3 j^3)
+      constants with |f| log-uniform in [0.01, 5] and f_i^6 > 0: 97,592
+      constants of orders 3-6.  Ntot 2, caps 6, eps1 = 0.5, Nstate = 200,
+      eps2 = 0.05.
   c3  serving stream: request r (seed 0x5648434900 + 1000 + r) draws
       M uniform in [8, 36], eps1 from {0.5, 1, 2}, Nstate uniform in
       [20, 200] and eps2 from {0, 0.01, 0.1}; the field has all cubic and
