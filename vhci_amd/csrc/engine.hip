@@ -625,13 +625,16 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         base += __popcll(km);
     };
     int blk = -1, pb0 = 0, pb1 = 0;
-    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
+    // each lane's next pattern record is loaded one step ahead
+    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
+        const uint4 e = en;
+        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
         bool hit = false;
         int64_t j = -1;
         if (p < pb1) {
-            const uint4 e = rec_load(a.deltas, p);
             const int nz = delta_nz(e);
             if (pattern_ok(e, nz, sq)) {
                 const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
@@ -665,6 +668,7 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
                 __syncthreads();
             }
         }
+    }
     }
     if (FILL && nq > 0) drain(nq);
     if (!FILL && lane == 0) {
@@ -730,13 +734,16 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
     };
     const uint64_t h0 = row_lin(sq, a.M, cf);
     int blk = -1, pb0 = 0, pb1 = 0;
-    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
+    // each lane's next pattern record is loaded one step ahead
+    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
+        const uint4 e = en;
+        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
         bool hit = false;
         int64_t j = -1;
         if (p < pb1) {
-            const uint4 e = rec_load(a.deltas, p);
             const int nz = delta_nz(e);
             if (pattern_ok(e, nz, sq)) {
                 const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
@@ -764,6 +771,7 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
             nq -= kWave;
             __syncthreads();
         }
+    }
     }
     if (nq > 0) drain(nq);
     const double s = lane_colsum<B>(acc, lane);
@@ -2385,13 +2393,16 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
     int64_t base = FILL ? a.off[row] : 0;
     int64_t cnt = 0;
     int blk = -1, pb0 = 0, pb1 = 0;
-    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
+    // each lane's next pattern record is loaded one step ahead
+    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
+        const uint4 e = en;
+        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
         bool hit = false;
         int64_t idx = -1;
         if (p < pb1) {
-            const uint4 e = rec_load(a.deltas, p);
             const int nz = delta_nz(e);
             if (pattern_ok(e, nz, sq)) {
                 const uint64_t h = fmix64(h0 + pattern_shift(e, nz, cf));
@@ -2413,6 +2424,7 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
             }
             base += __popcll(bal);
         }
+    }
     }
     if (!FILL && lane == 0) a.cnt[blockIdx.x] = cnt;
 }
@@ -2457,13 +2469,16 @@ __global__ __launch_bounds__(64) void k_pt2_push1(Pt2PushArgs a, unsigned long l
         nq = 0;
     };
     int blk = -1, pb0 = 0, pb1 = 0;
-    while (next_block(a.poff, a.M, sq, blk, pb0, pb1))
+    while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
+    // each lane's next pattern record is loaded one step ahead
+    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
+        const uint4 e = en;
+        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
         bool hit = false;
         int64_t idx = -1;
         if (p < pb1) {
-            const uint4 e = rec_load(a.deltas, p);
             const int nz = delta_nz(e);
             if (pattern_ok(e, nz, sq)) {
                 const uint64_t h = fmix64(h0 + pattern_shift(e, nz, cf));
@@ -2489,6 +2504,7 @@ __global__ __launch_bounds__(64) void k_pt2_push1(Pt2PushArgs a, unsigned long l
             qv[t] = (uint32_t)p;
         }
         nq += nh;
+    }
     }
     __syncthreads();
     if (nq > 0) flush();
