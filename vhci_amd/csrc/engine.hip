@@ -381,7 +381,39 @@ struct HBuildArgs {
     unsigned long long* extra;
     const int64_t* oldcnt;
     unsigned long long* cursor;
+    // hit words (null: none): the count pass stores the ballot of every
+    // 64-pattern step of a row's walk at hitw[woff[b] + step] (b the block
+    // index of the launch), and the fill pass then visits only those hits
+    uint64_t* hitw;
+    const int64_t* woff;
 };
+
+// steps of 64 patterns in the walk of each launched row (the count pass's
+// hit words per row); row mapping as k_hbuild's
+template <int NW>
+__global__ void k_walk_words(HBuildArgs a, int64_t nb, int64_t* __restrict__ words) {
+    const int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b > nb) return;
+    if (b == nb) {
+        words[nb] = 0;
+        return;
+    }
+    int64_t src;
+    if (a.n_old > 0) {
+        src = a.first + b;
+    } else {
+        src = a.perm ? a.perm[b] : b;
+    }
+    uint32_t w[NW];
+    key_load<NW>(a.keys, src, w);
+    int64_t cnt = 0;
+    for (int m = -1; m < a.M; m++) {
+        if (m >= 0 && key_q<NW>(w, m) == 0) continue;
+        const int blk = m + 1;
+        cnt += ((int64_t)a.poff[blk + 1] - (int64_t)a.poff[blk] + kWave - 1) / kWave;
+    }
+    words[b] = cnt;
+}
 
 // sum over the group's FCs of fc * prod_t L[k][q_start][q_end], in the
 // element order (= the reference's Quartic/Sextic or Cubic/Quintic order)
@@ -625,26 +657,44 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
         base += __popcll(km);
     };
     int blk = -1, pb0 = 0, pb1 = 0;
+    uint64_t* hw = a.hitw ? a.hitw + a.woff[blockIdx.x] : nullptr;
+    const bool known = FILL && hw;  // fill pass over recorded hits
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
     // each lane's next pattern record is loaded one step ahead
-    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
+    uint4 en = !known && pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
-        const uint4 e = en;
-        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
         bool hit = false;
         int64_t j = -1;
-        if (p < pb1) {
-            const int nz = delta_nz(e);
-            if (pattern_ok(e, nz, sq)) {
+        if (known) {
+            const uint64_t word = *hw++;
+            if (word == 0ull) continue;
+            if ((word >> lane) & 1ull) {
+                const uint4 e = rec_load(a.deltas, p);
+                const int nz = delta_nz(e);
                 const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
-                if (bloom_maybe(a.bloom, a.bmask, H)) {
-                    j = probe_shift<NW>(a.slots, a.mask, a.keys, w, e, nz, H);
-                    hit = j >= 0;
+                j = probe_shift<NW>(a.slots, a.mask, a.keys, w, e, nz, H);
+                hit = j >= 0;
+            }
+        } else {
+            const uint4 e = en;
+            if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
+            if (p < pb1) {
+                const int nz = delta_nz(e);
+                if (pattern_ok(e, nz, sq)) {
+                    const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
+                    if (bloom_maybe(a.bloom, a.bmask, H)) {
+                        j = probe_shift<NW>(a.slots, a.mask, a.keys, w, e, nz, H);
+                        hit = j >= 0;
+                    }
                 }
             }
         }
         const uint64_t bal = __ballot(hit);
+        if (!FILL && hw) {
+            if (lane == 0) *hw = bal;
+            hw++;
+        }
         if (!FILL) {
             cnt += __popcll(bal);
             if (a.rowcnt_up) cnt_up += __popcll(__ballot(hit && (a.inv ? a.inv[j] : j) > row));
@@ -3073,6 +3123,41 @@ void Engine::build_h_t(HStats& st) {
     // from-scratch count passes also count the upper triangle (half storage)
     rowcnt_up_.ensure((n + 1) * sizeof(int64_t));
     a.rowcnt_up = incr ? nullptr : rowcnt_up_.as<int64_t>();
+    // Hit words: the count pass records the ballot of every 64-pattern step
+    // of each row's walk, so the fill pass probes only the hits instead of
+    // walking every pattern again; made when the words take at most a
+    // quarter of the free device memory (VHCI_HIT_WORDS=0: never).
+    DevBuf wcnt, woff, hitw;
+    auto prepare_words = [&](int64_t nb) {
+        a.hitw = nullptr;
+        a.woff = nullptr;
+        hitw.release();
+        const char* he = std::getenv("VHCI_HIT_WORDS");
+        if (a.P <= 0 || nb <= 0 || (he && std::string(he) == "0")) return;
+        wcnt.ensure((nb + 1) * sizeof(int64_t));
+        woff.ensure((nb + 1) * sizeof(int64_t));
+        k_walk_words<NW><<<grid_for(nb + 1, 256), 256, 0, stream_>>>(a, nb, wcnt.as<int64_t>());
+        HIPCHK(hipGetLastError());
+        size_t tmp = 0;
+        HIPCHK(rocprim::exclusive_scan(nullptr, tmp, wcnt.as<int64_t>(), woff.as<int64_t>(), (int64_t)0,
+                                       (size_t)(nb + 1), rocprim::plus<int64_t>(), stream_));
+        scan_tmp_.ensure(tmp);
+        HIPCHK(rocprim::exclusive_scan(scan_tmp_.p, tmp, wcnt.as<int64_t>(), woff.as<int64_t>(), (int64_t)0,
+                                       (size_t)(nb + 1), rocprim::plus<int64_t>(), stream_));
+        int64_t words = 0;
+        HIPCHK(hipMemcpyAsync(&words, woff.as<int64_t>() + nb, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
+        HIPCHK(hipStreamSynchronize(stream_));
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        if ((double)words * 8.0 > 0.25 * ((double)free_b + (double)pool_.idle_bytes)) return;
+        try {
+            hitw.ensure((size_t)std::max<int64_t>(1, words) * sizeof(uint64_t));
+        } catch (const std::runtime_error&) {
+            return;
+        }
+        a.hitw = hitw.as<uint64_t>();
+        a.woff = woff.as<int64_t>();
+    };
     if (incr) {
         extra_.ensure(n_old * sizeof(uint64_t));
         oldcnt_.ensure(n_old * sizeof(int64_t));
@@ -3084,6 +3169,7 @@ void Engine::build_h_t(HStats& st) {
         a.extra = extra_.as<unsigned long long>();
         a.oldcnt = oldcnt_.as<int64_t>();
         a.cursor = cursor_.as<unsigned long long>();
+        prepare_words(n - n_old);
         k_hbuild<NW, false><<<(unsigned)(n - n_old), 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
         k_old_counts<<<grid_for(n_old, 256), 256, 0, stream_>>>(
@@ -3091,7 +3177,8 @@ void Engine::build_h_t(HStats& st) {
             extra_.as<unsigned long long>(), oldcnt_.as<int64_t>(), a.rowcnt);
         HIPCHK(hipGetLastError());
     } else if (a.P > 0) {
-            k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
+        prepare_words(n);
+        k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
     } else {
         k_fill<int64_t><<<grid_for(n, 256), 256, 0, stream_>>>(a.rowcnt, n, 0);
@@ -3113,7 +3200,8 @@ void Engine::build_h_t(HStats& st) {
     };
     int64_t nnz_up = a.rowcnt_up ? scan_counts(a.rowcnt_up) : -1;
     int64_t nnz = scan_counts(a.rowcnt);
-    int storage = choose_storage(nnz, nnz_up >= 0 ? nnz_up : nnz / 2);
+    bool keep_words = false;
+    int storage = choose_storage(nnz, nnz_up >= 0 ? nnz_up : nnz / 2, a.hitw ? hitw.cap : 0, &keep_words);
     if (incr && storage != 0) {
         // the grown matrix no longer fits in full: count again from scratch
         // (half storage is built whole)
@@ -3128,11 +3216,18 @@ void Engine::build_h_t(HStats& st) {
         a.oldcnt = nullptr;
         a.cursor = nullptr;
         a.rowcnt_up = rowcnt_up_.as<int64_t>();
+        prepare_words(n);
         k_hbuild<NW, false><<<(unsigned)n, 64, 0, stream_>>>(a);
         HIPCHK(hipGetLastError());
         nnz_up = scan_counts(a.rowcnt_up);
         nnz = scan_counts(a.rowcnt);
-        storage = choose_storage(nnz, nnz_up);
+        storage = choose_storage(nnz, nnz_up, a.hitw ? hitw.cap : 0, &keep_words);
+    }
+    if (!keep_words) {
+        // only the full form's fill reads them, and only where they fit
+        hitw.release();
+        a.hitw = nullptr;
+        a.woff = nullptr;
     }
     sym_ = storage == 1;
     if (sym_) (void)scan_counts(a.rowcnt_up);
@@ -3568,11 +3663,15 @@ int Engine::spmm_launches(int) const { return 1; }
 // as available: free device memory, the previous matrix (released first) and
 // the context pool's idle blocks (returned to the device when an
 // allocation would fail).
-int Engine::choose_storage(int64_t nnz, int64_t nnz_up) {
+int Engine::choose_storage(int64_t nnz, int64_t nnz_up, size_t words, bool* keep_words) {
     ks_mfac_ = 4.0;
+    if (keep_words) *keep_words = false;
     if (const char* e = std::getenv("VHCI_H_STORAGE")) {
         const std::string v(e);
-        if (v == "full") return 0;
+        if (v == "full") {
+            if (keep_words) *keep_words = true;
+            return 0;
+        }
         if (v == "half") return 1;
         if (v == "free") return 2;
         throw std::runtime_error("VHCI_H_STORAGE must be full, half or free, not '" + v + "'");
@@ -3590,7 +3689,7 @@ int Engine::choose_storage(int64_t nnz, int64_t nnz_up) {
     auto eig = [&](double mfac) { return n * 8.0 * (2.0 * (mfac * nev + 8.0) + 2.0 * 1.6 * nev + nev); };
     const double held = (double)(col_.cap + val_.cap + hi_.cap + lo_.cap + col_o_.cap + val_o_.cap + hi_o_.cap +
                                  lo_o_.cap + hacc_.cap) +
-                        (double)pool_.idle_bytes;
+                        (double)pool_.idle_bytes + (double)words;
     const double avail = 0.9 * ((double)free_b + held);
     // the sort's buffers are freed before the eigensolver starts
     const double full = (double)nnz * (sv + 4.0);
@@ -3602,6 +3701,7 @@ int Engine::choose_storage(int64_t nnz, int64_t nnz_up) {
     for (double mf = 4.0; mf >= 2.5; mf -= 0.25)
         if (full + std::max(sort, eig(mf)) <= avail) {
             ks_mfac_ = mf;
+            if (keep_words) *keep_words = full + sort + (double)words <= avail;
             return 0;
         }
     for (double mf = 4.0; mf >= 2.0; mf -= 0.25)

@@ -309,7 +309,9 @@ class Engine {
     // 0 full CSR, 1 half storage, 2 matrix-free: the first that fits beside
     // the eigensolver's blocks (VHCI_H_STORAGE=full|half|free forces one);
     // half storage may also shrink the Krylov-Schur basis (ks_mfac_) to fit
-    int choose_storage(int64_t nnz, int64_t nnz_up);
+    // words: bytes of the build's hit words, held now and counted as free;
+    // *keep_words: full storage, and its fill and sort fit beside them
+    int choose_storage(int64_t nnz, int64_t nnz_up, size_t words = 0, bool* keep_words = nullptr);
     double ks_mfac_ = 4.0;  // Krylov-Schur basis size / nev
     void spmm_half(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32, int ldx32);
     template <int NW>
