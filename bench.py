@@ -269,6 +269,72 @@ def cpu_baseline(ctx, r, problem_text, threads):
     }
 
 
+def cpu_fast_baseline(ctx, r, problem_text, threads, rows_h=4096, rows_p=2048):
+    """The oracle's hash-based host form (the generate-and-lookup algorithm
+    of the device engine on the host cores) timed on a bounded sample of this
+    run's states and extrapolated with the run's own counts: H rows probed
+    per build (new rows only when fewer than half are new, as the engine
+    does), the eigensolver's product vectors at each iteration's basis size,
+    and the PT2 pairs.  Heat-bath selection, Gram-Schmidt and Rayleigh-Ritz
+    are not priced, which favours the CPU."""
+    import tempfile
+
+    from oracle import pyoracle
+
+    pyoracle.set_threads(threads)
+    with tempfile.NamedTemporaryFile("w", suffix=".inp", delete=False) as f:
+        f.write(problem_text)
+        path = f.name
+    try:
+        o = pyoracle.OProblem(path)
+    finally:
+        os.unlink(path)
+    q = ctx.basis()
+    n = q.shape[0]
+    ns2 = int(r["ns2"])
+    qA = ctx.new_states(ns2) if ns2 else np.zeros((0, q.shape[1]), dtype=np.int32)
+    pairs_total = float(ctx.phase_stats()["pairs"]) if ns2 else 0.0
+    nopt = min(o.neig, n)
+    t0 = time.time()
+    s = o.fast_sample(q, qA, rows_h, rows_p if ns2 else 0, nopt)
+    sample_s = time.time() - t0
+    (t_index, nh, t_h, ent_h, t_prod, t_index_a, npr, t_gen, npairs, t_acc, _chk, seen, t_ea) = s.tolist()
+    sizes = ctx.iteration_sizes()
+    probed = float(sizes[0])
+    for a, b in zip(sizes[:-1], sizes[1:]):
+        probed += (b - a) if 2 * (b - a) < b else b
+    build = t_h / nh * probed + t_index * sum(sizes) / n
+    # product vectors split evenly over the sparse iterations, each priced at
+    # the sampled time per stored entry times the final H's entries per row
+    # (diagonal included) times that iteration's basis size
+    sparse = [k for k in sizes[1:]] or [sizes[0]]
+    vectors = float(r["timings"].get("spmm_vectors", 0.0))
+    nnz = float(ctx.phase_stats().get("nnz", 0.0))
+    per_row = (nnz + n) / n if nnz > 0 else ent_h / nh
+    eig = sum(vectors / 8.0 / len(sparse) * (t_prod / ent_h) * per_row * k for k in sparse)
+    pt2 = 0.0
+    if ns2 and npr > 0 and npairs > 0:
+        pt2 = t_index_a + t_gen / npr * n + t_acc / npairs * pairs_total + t_ea / max(1.0, seen) * ns2
+    t = build + eig + pt2
+    return {
+        "value": 1.0 / t,
+        "unit": "solves/s",
+        "cores": threads,
+        "kind": "port",
+        "form": "hash-based (oracle fast form)",
+        "sample": ("oracle fast form on this run's states: index of %d states (%.2f s); %d H rows (%.3e entries, "
+                   "%.2f s; one 8-column product over them %.4f s); external index of %d states (%.2f s); PT2 pairs "
+                   "of %d rows (%d pairs, generation %.2f s, accumulation %.2f s; E_a of the %d external states they "
+                   "reach %.2f s); extrapolated to %d probed rows, %.0f product vectors, %.3e PT2 pairs and %d "
+                   "external states: H builds %.0f s + products %.0f s + PT2 %.0f s = %.0f s per solve (heat-bath "
+                   "selection, Gram-Schmidt and Rayleigh-Ritz not included)"
+                   % (n, t_index, nh, ent_h, t_h, t_prod, ns2, t_index_a, npr, npairs, t_gen, t_acc, seen, t_ea,
+                      probed, vectors, pairs_total, ns2, build, eig, pt2, t)),
+        "phases_s": {"h_builds": build, "products": eig, "pt2": pt2},
+        "sample_seconds": sample_s,
+    }
+
+
 # the serving workload's distribution; eps2 = 0 draws on fields of more than
 # 20 modes become eps2 = 0.01 (synth.request_params: PT2 over every connected
 # state of such a field holds 10^8 to 10^9 states)
@@ -646,8 +712,10 @@ def main():
     out["hbm_copy"] = {"kernel": "k_copy16", "bytes_moved": 2 * (1 << 30), "ms": cms,
                        "GBs": 2.0 * (1 << 30) / cms / 1e6}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(ctx64, res["fp64"][1][1] if "fp64" in res else r, text,
-                                           args.cpu_threads)
+        rr = res["fp64"][1][1] if "fp64" in res else r
+        out["cpu_baseline"] = cpu_baseline(ctx64, rr, text, args.cpu_threads)
+        out["cpu_baseline_fast"] = cpu_fast_baseline(ctx64, rr, text, args.cpu_threads)
+        out["cpu_baseline_fast"]["gpu_over_cpu"] = out["value"] / out["cpu_baseline_fast"]["value"]
     for name, (_, (c, _r), _s) in res.items():
         c.close()
     if not args.no_secondary and wname != "c1":

@@ -57,6 +57,12 @@ def lib():
                                        ctypes.c_double, _dp, _i64p]),
             "orc_sample_rates": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, ctypes.c_int64, _i32p, ctypes.c_int64,
                                                 ctypes.c_int, _dp]),
+            "orc_fast_h_rows": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _i64p, ctypes.c_int64, _i64p, _i64p,
+                                               _i32p, _dp]),
+            "orc_fast_pt2": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _dp, ctypes.c_int, _dp, ctypes.c_int, _i32p,
+                                            ctypes.c_int64, _dp, _i64p]),
+            "orc_fast_sample": (ctypes.c_int, [_vp, _i32p, ctypes.c_int64, _i32p, ctypes.c_int64, ctypes.c_int64,
+                                               ctypes.c_int64, ctypes.c_int, _dp]),
             "orc_run": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int,
                                        ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]),
         }
@@ -136,6 +142,47 @@ class OProblem:
         lib().orc_h_rows(self.h, _p(q, _i32p), q.shape[0], _p(rows, _i64p), len(rows), ctypes.byref(nnz),
                          _p(rp, _i64p), _p(col, _i32p), _p(val, _dp))
         return rp, col, val
+
+    def fast_h_rows(self, q, rows):
+        """rows of H by the hash-based host form: (rowptr, col, val), no
+        stored zeros"""
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        rows = np.ascontiguousarray(rows, dtype=np.int64)
+        nnz = ctypes.c_int64()
+        if lib().orc_fast_h_rows(self.h, _p(q, _i32p), q.shape[0], _p(rows, _i64p), len(rows), ctypes.byref(nnz),
+                                 None, None, None):
+            raise RuntimeError(lib().orc_error().decode())
+        rp = np.zeros(len(rows) + 1, dtype=np.int64)
+        col = np.zeros(nnz.value, dtype=np.int32)
+        val = np.zeros(nnz.value)
+        lib().orc_fast_h_rows(self.h, _p(q, _i32p), q.shape[0], _p(rows, _i64p), len(rows), ctypes.byref(nnz),
+                              _p(rp, _i64p), _p(col, _i32p), _p(val, _dp))
+        return rp, col, val
+
+    def fast_pt2(self, q, C, E, nopt, qA):
+        """(PT2 corrections, pairs) over the external states qA by the
+        hash-based host form"""
+        q = np.ascontiguousarray(q, dtype=np.int32)
+        C = np.ascontiguousarray(C, dtype=np.float64)
+        E = np.ascontiguousarray(E, dtype=np.float64)
+        qA = np.ascontiguousarray(qA, dtype=np.int32)
+        dE = np.zeros(nopt)
+        npairs = ctypes.c_int64()
+        if lib().orc_fast_pt2(self.h, _p(q, _i32p), q.shape[0], _p(C, _dp), C.shape[1], _p(E, _dp), nopt,
+                              _p(qA, _i32p), qA.shape[0], _p(dE, _dp), ctypes.byref(npairs)):
+            raise RuntimeError(lib().orc_error().decode())
+        return dE, npairs.value
+
+    def fast_sample(self, qV, qA, rows_h, rows_p, nopt):
+        """timed sample of the hash-based host form (13 figures, see
+        orc_fast_sample)"""
+        qV = np.ascontiguousarray(qV, dtype=np.int32)
+        qA = np.ascontiguousarray(qA, dtype=np.int32)
+        out = np.zeros(13)
+        if lib().orc_fast_sample(self.h, _p(qV, _i32p), qV.shape[0], _p(qA, _i32p), qA.shape[0], int(rows_h),
+                                 int(rows_p), int(nopt), _p(out, _dp)):
+            raise RuntimeError(lib().orc_error().decode())
+        return out
 
     def build_h_dense(self, q):
         q = np.ascontiguousarray(q, dtype=np.int32)

@@ -968,6 +968,224 @@ int sparse_diag(const Problem& P, const std::vector<int>& basis, std::vector<dou
     return 0;
 }
 
+
+/* ---- Hash-based host form ("fast") -------------------------------------
+ * Benchmark baseline only: the generate-and-lookup algorithm of the device
+ * engine, run on the host cores, so that the CPU figure is not set by the
+ * reference's all-pairs loops alone.  A row's connections are generated from
+ * each element FC's quanta changes (every change that the reference's
+ * ScreenState, Ham.cpp:401-446, accepts for that FC, with the odd-pair
+ * prefilter of Ham.cpp:283) and looked up in an open-addressing index of the
+ * quanta; the FC terms of one pair are added in the reference's list order
+ * (Quartic, Sextic / Cubic, Quintic; Ham.cpp:247-314), so the values equal
+ * the all-pairs form's bit for bit.  Pairs that no FC bridges (the
+ * reference's stored zeros) are not generated. */
+
+struct QIndex {
+    int M = 0;
+    int64_t n = 0;
+    std::vector<uint8_t> q;       // n x M quanta
+    std::vector<uint64_t> coef;   // per-mode multipliers of the linear key sum
+    std::vector<int32_t> slot;    // open addressing, linear probing, -1 empty
+    uint64_t mask = 0;
+    std::vector<uint64_t> bloom;  // two bits per state
+    uint64_t bmask = 0;
+};
+
+inline uint64_t mix64(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+inline uint64_t qlin(const QIndex& X, const uint8_t* q) {
+    uint64_t h = 0;
+    for (int m = 0; m < X.M; m++) h += X.coef[m] * q[m];
+    return h;
+}
+
+int qindex_build(QIndex& X, const int32_t* q, int64_t n, int M) {
+    X.M = M;
+    X.n = n;
+    X.q.resize((size_t)n * M);
+    for (int64_t i = 0; i < n * M; i++) {
+        if (q[i] < 0 || q[i] > 255) return fail("fast form: quanta must lie in 0..255");
+        X.q[i] = (uint8_t)q[i];
+    }
+    uint64_t s = 0x51ED270B27A1C3D5ull;
+    X.coef.resize(M);
+    for (auto& c : X.coef) c = splitmix(s) | 1ull;
+    size_t cap = 16;
+    while (cap < (size_t)(2 * n)) cap <<= 1;
+    X.slot.assign(cap, -1);
+    X.mask = cap - 1;
+    size_t bits = 1024;
+    while (bits < (size_t)(16 * n)) bits <<= 1;
+    X.bloom.assign(bits / 64, 0);
+    X.bmask = bits - 1;
+    for (int64_t i = 0; i < n; i++) {
+        const uint64_t h = mix64(qlin(X, &X.q[(size_t)i * M]));
+        const uint64_t b1 = h & X.bmask, b2 = (h >> 32) & X.bmask;
+        X.bloom[b1 >> 6] |= 1ull << (b1 & 63);
+        X.bloom[b2 >> 6] |= 1ull << (b2 & 63);
+        for (uint64_t p = h & X.mask;; p = (p + 1) & X.mask)
+            if (X.slot[p] < 0) {
+                X.slot[p] = (int32_t)i;
+                break;
+            }
+    }
+    return 0;
+}
+
+// index of the state with quanta t (hash h), or -1
+inline int64_t qindex_find(const QIndex& X, uint64_t h, const uint8_t* t) {
+    const uint64_t b1 = h & X.bmask, b2 = (h >> 32) & X.bmask;
+    if (!((X.bloom[b1 >> 6] >> (b1 & 63)) & 1ull) || !((X.bloom[b2 >> 6] >> (b2 & 63)) & 1ull)) return -1;
+    for (uint64_t p = h & X.mask;; p = (p + 1) & X.mask) {
+        const int32_t s = X.slot[p];
+        if (s < 0) return -1;
+        if (std::memcmp(&X.q[(size_t)s * X.M], t, X.M) == 0) return s;
+    }
+}
+
+/* AnharmPot (Ham.cpp:38-104) on byte quanta, start state qn */
+inline double anharm_pot8(const uint8_t* qn, const uint8_t* qm, const FConst& f) {
+    double v = f.fc;
+    for (size_t i = 0; i < f.sm.size(); i++) v *= coeffsum(f.mp[i], qn[f.sm[i]], qm[f.sm[i]]);
+    return v;
+}
+
+inline double harmonic8(const Problem& P, const uint8_t* q) {
+    double e = 0.;
+    for (int j = 0; j < P.M; j++) {
+        double ej = 0.5;
+        ej += q[j];
+        ej *= P.freq[j];
+        e += ej;
+    }
+    return e;
+}
+
+/* every state of X that one element FC connects to the start quanta qs
+ * (lin = its linear key sum under X's multipliers): hit(index, fc) in
+ * element order; tmp holds a copy of qs and is restored */
+template <class F>
+void walk_connections(const Problem& P, const QIndex& X, const uint8_t* qs, uint64_t lin, uint8_t* tmp, F&& hit) {
+    const int fm = P.fcmax;
+    for (const auto* lst : {&P.quartic, &P.sextic, &P.cubic, &P.quintic})
+        for (const FConst& f : *lst) {
+            const int d = (int)f.sm.size();
+            int dl[6];
+            for (int t = 0; t < d; t++) dl[t] = -f.mp[t];
+            while (true) {
+                int qd = 0;
+                bool ok = true;
+                uint64_t h = lin;
+                for (int t = 0; t < d; t++) {
+                    qd += std::abs(dl[t]);
+                    const int v = qs[f.sm[t]] + dl[t];
+                    ok = ok && v >= 0 && v <= 255;
+                    h += (uint64_t)(int64_t)dl[t] * X.coef[f.sm[t]];
+                }
+                if (qd != 0 && ok && ((qd & 1) == 0 || qd <= fm - 1)) {
+                    const uint64_t hh = mix64(h);
+                    const uint64_t b1 = hh & X.bmask, b2 = (hh >> 32) & X.bmask;
+                    if (((X.bloom[b1 >> 6] >> (b1 & 63)) & 1ull) && ((X.bloom[b2 >> 6] >> (b2 & 63)) & 1ull)) {
+                        for (int t = 0; t < d; t++) tmp[f.sm[t]] = (uint8_t)(qs[f.sm[t]] + dl[t]);
+                        const int64_t j = qindex_find(X, hh, tmp);
+                        for (int t = 0; t < d; t++) tmp[f.sm[t]] = qs[f.sm[t]];
+                        if (j >= 0) hit(j, f);
+                    }
+                }
+                int t = d - 1;
+                while (t >= 0) {
+                    dl[t] += 2;
+                    if (dl[t] <= f.mp[t]) break;
+                    dl[t] = -f.mp[t];
+                    t--;
+                }
+                if (t < 0) break;
+            }
+        }
+}
+
+/* the FCs that couple a state to itself (every mode power even: the ones
+ * ScreenState accepts at zero quanta change), Quartic then Sextic in list
+ * order */
+std::vector<const FConst*> diag_fcs(const Problem& P) {
+    std::vector<const FConst*> d;
+    for (const auto* lst : {&P.quartic, &P.sextic})
+        for (const FConst& f : *lst) {
+            bool even = true;
+            for (int k : f.mp) even = even && (k % 2 == 0);
+            if (even) d.push_back(&f);
+        }
+    return d;
+}
+
+/* row r of H over the states of X: (column, value) ascending, the diagonal
+ * (E_r + V_rr, Ham.cpp:203-226) included; dg = diag_fcs(P) */
+void fast_row(const Problem& P, const std::vector<const FConst*>& dg, const QIndex& X, int64_t r,
+              std::vector<std::pair<int32_t, double>>& out, std::vector<std::pair<int32_t, double>>& raw,
+              std::vector<uint8_t>& tmp) {
+    const int M = X.M;
+    const uint8_t* qr = &X.q[(size_t)r * M];
+    tmp.assign(qr, qr + M);
+    raw.clear();
+    walk_connections(P, X, qr, qlin(X, qr), tmp.data(), [&](int64_t j, const FConst& f) {
+        const uint8_t* qj = &X.q[(size_t)j * M];
+        // the element is evaluated from the lower-indexed state, as in the
+        // reference's upper-triangle loop
+        raw.push_back({(int32_t)j, j > r ? anharm_pot8(qr, qj, f) : anharm_pot8(qj, qr, f)});
+    });
+    double vd = 0;
+    for (const FConst* f : dg) vd += anharm_pot8(qr, qr, *f);
+    raw.push_back({(int32_t)r, harmonic8(P, qr) + vd});
+    // one FC reaches a column at most once, so a stable sort keeps each
+    // pair's terms in element order
+    std::stable_sort(raw.begin(), raw.end(),
+                     [](const std::pair<int32_t, double>& a, const std::pair<int32_t, double>& b) {
+                         return a.first < b.first;
+                     });
+    out.clear();
+    for (size_t k = 0; k < raw.size();) {
+        const int32_t c = raw[k].first;
+        if (c == (int32_t)r) {
+            out.push_back(raw[k]);
+            k++;
+            continue;
+        }
+        double v = 0;
+        for (; k < raw.size() && raw[k].first == c; k++) v += raw[k].second;
+        out.push_back({c, v});
+    }
+}
+
+/* PT2 pairs (a, i, term) of variational row i against the external index A,
+ * terms AnharmPot(a -> i) in element order (PT2.cpp:80-118) */
+struct Pt2Term {
+    uint64_t key;  // a << 32 | i
+    double v;
+};
+
+void fast_pt2_row(const Problem& P, const QIndex& V, const QIndex& A, int64_t i, std::vector<Pt2Term>& out,
+                  std::vector<uint8_t>& tmp) {
+    const int M = V.M;
+    const uint8_t* qi = &V.q[(size_t)i * M];
+    tmp.assign(qi, qi + M);
+    walk_connections(P, A, qi, qlin(A, qi), tmp.data(), [&](int64_t a, const FConst& f) {
+        out.push_back({((uint64_t)a << 32) | (uint64_t)i, anharm_pot8(&A.q[(size_t)a * M], qi, f)});
+    });
+}
+
+/* E_a of an external state: harmonic, then the diagonal FC terms one at a
+ * time (PT2.cpp:119-144); dg = diag_fcs(P) */
+inline double pt2_ea(const Problem& P, const std::vector<const FConst*>& dg, const uint8_t* qa) {
+    double ea = harmonic8(P, qa);
+    for (const FConst* f : dg) ea += anharm_pot8(qa, qa, *f);
+    return ea;
+}
+
 }  // namespace
 
 /* =================== C API =================== */
@@ -1218,6 +1436,264 @@ int orc_sample_rates(const orc_problem* p, const int32_t* qV, int64_t nV, int64_
     out[2] = (double)na * (double)nV;
     out[3] = tp;
     if (chk == 12345.6789) out[0] += 1;  // keeps chk live
+    return 0;
+}
+
+/* Hash-based host form (benchmark baseline).  Rows of H over the states q as
+ * orc_h_rows gives them, without the stored zeros that no FC bridges.
+ * Two-call pattern as orc_h_rows. */
+int orc_fast_h_rows(const orc_problem* p, const int32_t* q, int64_t n, const int64_t* rows, int64_t nr, int64_t* nnz,
+                    int64_t* rowptr, int32_t* col, double* val) {
+    const Problem& P = p->P;
+    int nt = g_threads > 0 ? g_threads : omp_get_max_threads();
+    QIndex X;
+    if (qindex_build(X, q, n, P.M)) return -1;
+    for (int64_t k = 0; k < nr; k++)
+        if (rows[k] < 0 || rows[k] >= n) return fail("orc_fast_h_rows: row out of range");
+    std::vector<std::vector<std::pair<int32_t, double>>> out(nr);
+    const auto dg = diag_fcs(P);
+#pragma omp parallel num_threads(nt)
+    {
+        std::vector<std::pair<int32_t, double>> raw;
+        std::vector<uint8_t> tmp;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t k = 0; k < nr; k++) fast_row(P, dg, X, rows[k], out[k], raw, tmp);
+    }
+    int64_t tot = 0;
+    for (auto& x : out) tot += (int64_t)x.size();
+    *nnz = tot;
+    if (rowptr) {
+        rowptr[0] = 0;
+        for (int64_t k = 0; k < nr; k++) {
+            rowptr[k + 1] = rowptr[k] + (int64_t)out[k].size();
+            for (size_t t = 0; t < out[k].size(); t++) {
+                col[rowptr[k] + t] = out[k][t].first;
+                val[rowptr[k] + t] = out[k][t].second;
+            }
+        }
+    }
+    return 0;
+}
+
+namespace {
+
+/* pairs of the given variational rows against A, grouped by owner thread of
+ * their external state (contiguous ranges of A, as orc_pt2 splits them), each
+ * group sorted by (a, i) with every pair's terms in element order */
+void fast_pt2_pairs(const Problem& P, const QIndex& V, const QIndex& A, const std::vector<int64_t>& rows, int nt,
+                    std::vector<std::vector<Pt2Term>>& own) {
+    std::vector<std::vector<Pt2Term>> gen(nt);
+#pragma omp parallel num_threads(nt)
+    {
+        std::vector<uint8_t> tmp;
+        std::vector<Pt2Term>& g = gen[omp_get_thread_num()];
+#pragma omp for schedule(dynamic, 1)
+        for (size_t k = 0; k < rows.size(); k++) fast_pt2_row(P, V, A, rows[k], g, tmp);
+    }
+    const int64_t na = A.n;
+    own.assign(nt, {});
+    // a thread's row blocks come in ascending order of its generation list;
+    // the stable sort below keeps one pair's terms (all from one row, one
+    // thread) in element order
+    for (int t = 0; t < nt; t++)
+        for (const Pt2Term& x : gen[t]) {
+            const int64_t a = (int64_t)(x.key >> 32);
+            int o = (int)((a * nt) / std::max<int64_t>(1, na));
+            while (o + 1 < nt && a >= na * (o + 1) / nt) o++;
+            while (o > 0 && a < na * o / nt) o--;
+            own[o].push_back(x);
+        }
+#pragma omp parallel for schedule(static, 1) num_threads(nt)
+    for (int t = 0; t < nt; t++)
+        std::stable_sort(own[t].begin(), own[t].end(),
+                         [](const Pt2Term& x, const Pt2Term& y) { return x.key < y.key; });
+}
+
+}  // namespace
+
+/* PT2 corrections (DoPT2, PT2.cpp:16-154) over a given external space qA of
+ * nA states (as orc_hb_select returns it) by the hash-based form.  The
+ * numerators run over ascending i and the corrections over the same thread
+ * ranges of A as orc_pt2, so with the same thread count the results are
+ * equal bit for bit. */
+int orc_fast_pt2(const orc_problem* p, const int32_t* q, int64_t n, const double* C, int ncol, const double* E,
+                 int nopt, const int32_t* qA, int64_t nA, double* dE, int64_t* npairs) {
+    const Problem& P = p->P;
+    int nt = g_threads > 0 ? g_threads : omp_get_max_threads();
+    QIndex V, A;
+    if (qindex_build(V, q, n, P.M) || qindex_build(A, qA, nA, P.M)) return -1;
+    std::vector<int64_t> rows(n);
+    for (int64_t i = 0; i < n; i++) rows[i] = i;
+    std::vector<std::vector<Pt2Term>> own;
+    fast_pt2_pairs(P, V, A, rows, nt, own);
+    std::vector<std::vector<double>> part(nt, std::vector<double>(nopt, 0.0));
+    std::vector<int64_t> pc(nt, 0);
+    const auto dg = diag_fcs(P);
+#pragma omp parallel for schedule(static, 1) num_threads(nt)
+    for (int t = 0; t < nt; t++) {
+        const auto& w = own[t];
+        std::vector<double> sum(nopt);
+        for (size_t k = 0; k < w.size();) {
+            const uint64_t a = w[k].key >> 32;
+            std::fill(sum.begin(), sum.end(), 0.0);
+            while (k < w.size() && (w[k].key >> 32) == a) {
+                const uint64_t key = w[k].key;
+                const int64_t i = (int64_t)(key & 0xffffffffull);
+                double hai = 0;
+                for (; k < w.size() && w[k].key == key; k++) hai += w[k].v;
+                for (int s = 0; s < nopt; s++) sum[s] += hai * C[i * ncol + s];
+                pc[t]++;
+            }
+            const double ea = pt2_ea(P, dg, &A.q[(size_t)a * P.M]);
+            for (int s = 0; s < nopt; s++) part[t][s] += std::pow(sum[s], 2) / (E[s] - ea);
+        }
+    }
+    for (int s = 0; s < nopt; s++) dE[s] = 0.0;
+    int64_t tot = 0;
+    for (int t = 0; t < nt; t++) {
+        for (int s = 0; s < nopt; s++) dE[s] += part[t][s];
+        tot += pc[t];
+    }
+    if (npairs) *npairs = tot;
+    return 0;
+}
+
+/* Timed sample of the hash-based form on a run's states (benchmark
+ * baseline).  H rows and PT2 rows are taken as 16 runs of consecutive rows
+ * spread over the basis.  out: [0] index build s (nV states), [1] H rows
+ * timed, [2] their s, [3] their entries (diagonal included), [4] one
+ * 8-column product over those rows, s (best of 3), [5] external index build
+ * s (nA states), [6] PT2 rows timed, [7] their pair generation s, [8] pairs
+ * found, [9] numerator accumulation over those pairs (nopt columns) and the
+ * corrections of their external states, s, [10] a checksum, [11] external
+ * states those pairs reach, [12] their E_a, s */
+int orc_fast_sample(const orc_problem* p, const int32_t* qV, int64_t nV, const int32_t* qA, int64_t nA,
+                    int64_t rows_h, int64_t rows_p, int nopt, double* out) {
+    using clk = std::chrono::steady_clock;
+    auto secs = [](clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); };
+    const Problem& P = p->P;
+    int nt = g_threads > 0 ? g_threads : omp_get_max_threads();
+    auto runs = [&](int64_t want) {
+        std::vector<int64_t> r;
+        const int nrun = 16;
+        const int64_t len = std::max<int64_t>(1, std::min(want / nrun, nV / nrun));
+        for (int c = 0; c < nrun; c++) {
+            const int64_t s0 = (nV - len) * c / (nrun - 1);
+            for (int64_t i = s0; i < s0 + len && i < nV; i++)
+                if (r.empty() || i > r.back()) r.push_back(i);
+        }
+        return r;
+    };
+    double chk = 0;
+    auto t0 = clk::now();
+    QIndex V;
+    if (qindex_build(V, qV, nV, P.M)) return -1;
+    out[0] = secs(t0);
+    // H rows
+    const std::vector<int64_t> hr = runs(rows_h);
+    const int64_t nh = (int64_t)hr.size();
+    std::vector<std::vector<std::pair<int32_t, double>>> rowv(nh);
+    const auto dg = diag_fcs(P);
+    t0 = clk::now();
+#pragma omp parallel num_threads(nt)
+    {
+        std::vector<std::pair<int32_t, double>> raw;
+        std::vector<uint8_t> tmp;
+#pragma omp for schedule(dynamic, 1)
+        for (int64_t k = 0; k < nh; k++) fast_row(P, dg, V, hr[k], rowv[k], raw, tmp);
+    }
+    out[1] = (double)nh;
+    out[2] = secs(t0);
+    std::vector<int64_t> rp(nh + 1, 0);
+    for (int64_t k = 0; k < nh; k++) rp[k + 1] = rp[k] + (int64_t)rowv[k].size();
+    out[3] = (double)rp[nh];
+    std::vector<int32_t> cl(rp[nh]);
+    std::vector<double> vl(rp[nh]);
+    for (int64_t k = 0; k < nh; k++)
+        for (size_t t = 0; t < rowv[k].size(); t++) {
+            cl[rp[k] + t] = rowv[k][t].first;
+            vl[rp[k] + t] = rowv[k][t].second;
+        }
+    rowv.clear();
+    rowv.shrink_to_fit();
+    // one 8-column product over the sampled rows, X over the whole basis
+    std::vector<double> X((size_t)nV * 8), Y((size_t)nh * 8);
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int64_t i = 0; i < nV * 8; i++) X[i] = 1e-3 * (double)((i * 2654435761ull) % 1000);
+    double best = 1e300;
+    for (int rep = 0; rep < 3; rep++) {
+        t0 = clk::now();
+#pragma omp parallel for schedule(static, 64) num_threads(nt)
+        for (int64_t k = 0; k < nh; k++) {
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int64_t e = rp[k]; e < rp[k + 1]; e++) {
+                const double v = vl[e];
+                const double* x = &X[(size_t)cl[e] * 8];
+                for (int c = 0; c < 8; c++) acc[c] += v * x[c];
+            }
+            for (int c = 0; c < 8; c++) Y[k * 8 + c] = acc[c];
+        }
+        best = std::min(best, secs(t0));
+    }
+    out[4] = best;
+    for (int64_t k = 0; k < nh * 8; k += 97) chk += Y[k];
+    // PT2
+    t0 = clk::now();
+    QIndex A;
+    if (qindex_build(A, qA, nA, P.M)) return -1;
+    out[5] = secs(t0);
+    const std::vector<int64_t> pr = runs(rows_p);
+    std::vector<int32_t> slot_of(nV, -1);
+    for (size_t k = 0; k < pr.size(); k++) slot_of[pr[k]] = (int32_t)k;
+    std::vector<double> Cs(pr.size() * (size_t)nopt);
+    for (size_t k = 0; k < Cs.size(); k++) Cs[k] = 1e-3 * (double)((k * 40503ull) % 997);
+    std::vector<double> Es(nopt, -1e4);
+    out[6] = (double)pr.size();
+    t0 = clk::now();
+    std::vector<std::vector<Pt2Term>> own;
+    fast_pt2_pairs(P, V, A, pr, nt, own);
+    out[7] = secs(t0);
+    // E_a of every external state the sample reaches (a cost per external
+    // state), then the numerators and corrections (a cost per pair)
+    std::vector<std::vector<double>> eas(nt);
+    int64_t seen = 0;
+    t0 = clk::now();
+#pragma omp parallel for schedule(static, 1) num_threads(nt) reduction(+ : seen)
+    for (int t = 0; t < nt; t++) {
+        const auto& w = own[t];
+        for (size_t k = 0; k < w.size(); k++)
+            if (k == 0 || (w[k].key >> 32) != (w[k - 1].key >> 32)) {
+                eas[t].push_back(pt2_ea(P, dg, &A.q[(size_t)(w[k].key >> 32) * P.M]));
+                seen++;
+            }
+    }
+    out[12] = secs(t0);
+    out[11] = (double)seen;
+    int64_t pairs = 0;
+    t0 = clk::now();
+#pragma omp parallel for schedule(static, 1) num_threads(nt) reduction(+ : chk, pairs)
+    for (int t = 0; t < nt; t++) {
+        const auto& w = own[t];
+        std::vector<double> sum(nopt);
+        size_t g = 0;
+        for (size_t k = 0; k < w.size(); g++) {
+            const uint64_t a = w[k].key >> 32;
+            std::fill(sum.begin(), sum.end(), 0.0);
+            while (k < w.size() && (w[k].key >> 32) == a) {
+                const uint64_t key = w[k].key;
+                const int64_t sl = slot_of[(int64_t)(key & 0xffffffffull)];
+                double hai = 0;
+                for (; k < w.size() && w[k].key == key; k++) hai += w[k].v;
+                for (int s = 0; s < nopt; s++) sum[s] += hai * Cs[(size_t)sl * nopt + s];
+                pairs++;
+            }
+            const double ea = eas[t][g];
+            for (int s = 0; s < nopt; s++) chk += sum[s] * sum[s] / (Es[s] - ea);
+        }
+    }
+    out[9] = secs(t0);
+    out[8] = (double)pairs;
+    out[10] = chk;
     return 0;
 }
 

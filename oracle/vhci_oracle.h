@@ -103,6 +103,24 @@ int orc_pt2(const orc_problem* p, const int32_t* q, int64_t n, const double* C,
 int orc_sample_rates(const orc_problem* p, const int32_t* qV, int64_t nV, int64_t rows, const int32_t* qA,
                      int64_t na, int nopt, double* out);
 
+/* Hash-based host form, benchmark baseline only: the device engine's
+ * generate-and-lookup algorithm on the host cores (each element FC's quanta
+ * changes looked up in an open-addressing index of the states).
+ * orc_fast_h_rows: rows of H as orc_h_rows, without the stored zeros no FC
+ * bridges (two-call pattern).  orc_fast_pt2: DoPT2's corrections over a
+ * given external space (as orc_hb_select returns it); with the thread count
+ * of orc_pt2 the results are equal bit for bit.  orc_fast_sample: a timed
+ * sample on a run's states (16 runs of consecutive rows); out[13] = index
+ * build s, H rows, their s, their entries, one 8-column product over them
+ * (s), external index build s, PT2 rows, pair generation s, pairs,
+ * accumulation s, checksum, external states reached, their E_a s. */
+int orc_fast_h_rows(const orc_problem* p, const int32_t* q, int64_t n, const int64_t* rows, int64_t nr, int64_t* nnz,
+                    int64_t* rowptr, int32_t* col, double* val);
+int orc_fast_pt2(const orc_problem* p, const int32_t* q, int64_t n, const double* C, int ncol, const double* E,
+                 int nopt, const int32_t* qA, int64_t nA, double* dE, int64_t* npairs);
+int orc_fast_sample(const orc_problem* p, const int32_t* qV, int64_t nV, const int32_t* qA, int64_t nA,
+                    int64_t rows_h, int64_t rows_p, int nopt, double* out);
+
 /* Whole program (VHCI.cpp:5-185): writes freqs file, optional checkpoint and
  * a JSON summary (basis size per iteration, PT2 space, energies as %.17g,
  * structural nnz per build).  basis_out (optional) receives the final

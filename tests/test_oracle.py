@@ -255,3 +255,71 @@ def test_hb_select_brute_force():
                 seen.add(t)
                 expect.append(t)
     assert [tuple(r) for r in new] == expect
+
+
+def _rows_without_zeros(rp, col, val, rows):
+    out = []
+    for k, r in enumerate(rows):
+        c, v = col[rp[k]:rp[k + 1]], val[rp[k]:rp[k + 1]]
+        keep = (v != 0.0) | (c == r)
+        out.append((c[keep], v[keep]))
+    return out
+
+
+@pytest.mark.parametrize("field", ["clo2", "cubic", "acn"])
+def test_fast_form_h_rows_equal_all_pairs(tmp_path, field):
+    """The hash-based host form (the benchmark's stronger CPU baseline)
+    gives the all-pairs rows bit for bit, stored zeros aside: the 6th-order
+    ClO2 field, the cubic-only field whose odd prefilter drops couplings
+    (fcmax = 3), and the acetonitrile field on a heat-bath grown basis."""
+    rng = np.random.default_rng(11)
+    if field == "clo2":
+        prob = pyoracle.OProblem(os.path.join(GOLDEN, "fields", "CLO2.inp"))
+        q = np.array([s for s in itertools.product(range(7), repeat=3) if sum(s) <= 6], dtype=np.int32)
+        rows = np.arange(len(q))
+    elif field == "cubic":
+        txt = ("c\nHCI_Eps: 0\nNstate: 2\nPT2: 0\nPT2_Eps: 0\nMax_quanta: 3\nModes: 3\n"
+               " 0 1000.0 5\n 1 1500.0 5\n 2 2000.0 5\nForce_constants: 3\n"
+               " 3 0 0 0 50.0\n 3 0 1 2 20.0\n 3 0 0 1 10.0\n")
+        p = tmp_path / "cub.inp"
+        p.write_text(txt)
+        prob = pyoracle.OProblem(str(p))
+        q = np.array([s for s in itertools.product(range(6), repeat=3) if sum(s) <= 5], dtype=np.int32)
+        rows = np.arange(len(q))
+    else:
+        prob = pyoracle.OProblem(os.path.join(GOLDEN, "acn_vhci.inp"))
+        q0 = prob.initial_basis()
+        new = prob.hb_select(q0, rng.standard_normal(q0.shape[0]) * 10, 1.0)
+        q = np.concatenate([q0, new])
+        rows = np.sort(rng.choice(len(q), 200, replace=False))
+    a = _rows_without_zeros(*prob.h_rows(q, rows), rows)
+    b = _rows_without_zeros(*prob.fast_h_rows(q, rows), rows)
+    assert sum(len(x[0]) for x in a) > len(rows)
+    for x, y in zip(a, b):
+        assert np.array_equal(x[0], y[0])
+        assert np.array_equal(x[1], y[1])
+
+
+def test_fast_form_pt2_equals_all_pairs():
+    """The hash-based host form's PT2 corrections over the external space
+    that the heat-bath rule selects equal the all-pairs gather's bit for bit
+    (same thread count); a corrupted coefficient changes them."""
+    pyoracle.set_threads(4)
+    prob = pyoracle.OProblem(os.path.join(GOLDEN, "acn_vhci.inp"))
+    q0 = prob.initial_basis()
+    rng = np.random.default_rng(5)
+    q = np.concatenate([q0, prob.hb_select(q0, rng.standard_normal(q0.shape[0]) * 10, 2.0)])
+    C = rng.standard_normal((len(q), 5))
+    C /= np.linalg.norm(C, axis=0)
+    E = np.array([9000.0, 9100.0, 9200.0, 9300.0, 9400.0])
+    nopt, eps2 = 4, 0.2
+    dE = prob.pt2(q, C, E, nopt, eps2)
+    dE = np.asarray(dE[0] if isinstance(dE, tuple) else dE)
+    qA = prob.hb_select(q, pyoracle.cmax(C, nopt), eps2)
+    fast, pairs = prob.fast_pt2(q, C, E, nopt, qA)
+    assert len(qA) > 1000 and pairs > len(qA)
+    assert np.array_equal(dE, fast)
+    C2 = C.copy()
+    C2[len(q) // 2, 0] += 0.1
+    bad, _ = prob.fast_pt2(q, C2, E, nopt, qA)
+    assert bad[0] != fast[0]
