@@ -175,10 +175,10 @@ __device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots,
     return hash_find_h<NW>(slots, mask, keys, w, key_hash<NW>(w));
 }
 
-// Bloom filter in front of a hash (one bit array, two bits per key, about
-// four bits per key: about 1 MB for a million keys, small enough to stay in
-// a die's L2 while the hash itself does not).  Most probes of the pattern
-// walks find nothing; the filter answers them without touching the hash.
+// Bloom filter in front of a hash (one bit array, two bits set per key,
+// 4 to 32 bits per key and at most 4 MB, small enough to stay in a die's L2
+// while the hash itself does not).  Most probes of the pattern walks find
+// nothing; the filter answers them without touching the hash.
 __device__ __forceinline__ uint64_t bloom_second(uint64_t H) {
     return ((H ^ (H >> 29)) * 0xBF58476D1CE4E5B9ULL) >> 32;
 }
@@ -3034,14 +3034,25 @@ void Engine::get_basis(int32_t* q) const {
         for (int m = 0; m < M; m++) q[i * M + m] = packed[(size_t)i * K + m];
 }
 
+// Bloom filter bits for a hash of cap slots: as many as one die's 4 MB L2
+// holds, between 2 and 16 per slot (VHCI_BLOOM_BITS: bits per slot).  On the
+// 48-mode workload a 4 MB filter gives the fastest pattern walks: fewer
+// false positives send fewer probes to the hash, and a larger filter no
+// longer stays in L2 (H builds 2.70 s at 2 bits per slot, 2.18 s at 8,
+// 2.38 s at 16; PT2 pairs 2.09 s at 2, 1.95 s at 4, 2.40 s at 8).
+static uint64_t bloom_bits(uint64_t cap) {
+    if (const char* e = std::getenv("VHCI_BLOOM_BITS"))
+        return next_pow2((uint64_t)std::max(1, std::min(64, std::atoi(e)))) * cap;
+    return std::max<uint64_t>(2 * cap, std::min<uint64_t>(1ull << 25, 16 * cap));
+}
+
 void Engine::rebuild_hash(int64_t n) {
     uint64_t cap = next_pow2(std::max<uint64_t>(1024, (uint64_t)n * 2));
     slots_.ensure(cap * sizeof(uint64_t));
     hmask_ = cap - 1;
     k_fill<uint64_t><<<grid_for(cap, 256), 256, 0, stream_>>>(slots_.as<uint64_t>(), (int64_t)cap, kEmpty);
-    // filter of 2 bits per slot (at least 4 per key while the hash holds
-    // at most half its slots)
-    const uint64_t fbits = 2 * cap;
+    // at least 4 filter bits per key while the hash holds at most half its slots
+    const uint64_t fbits = bloom_bits(cap);
     bloom_.ensure(fbits / 8);
     bmask_ = fbits - 1;
     k_fill<uint32_t><<<grid_for((int64_t)(fbits / 32), 256), 256, 0, stream_>>>(bloom_.as<uint32_t>(),
@@ -5363,7 +5374,7 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
     const uint64_t cap = next_pow2(std::max<uint64_t>(1024, 2 * (uint64_t)ns2));
     s2slots_.ensure(cap * sizeof(uint64_t));
     k_fill<uint64_t><<<grid_for(cap, 256), 256, 0, stream_>>>(s2slots_.as<uint64_t>(), (int64_t)cap, kEmpty);
-    const uint64_t fbits = 2 * cap;
+    const uint64_t fbits = bloom_bits(cap);
     s2bloom_.ensure(fbits / 8);
     k_fill<uint32_t><<<grid_for((int64_t)(fbits / 32), 256), 256, 0, stream_>>>(s2bloom_.as<uint32_t>(),
                                                                                 (int64_t)(fbits / 32), 0u);
