@@ -179,17 +179,24 @@ __device__ __forceinline__ int64_t hash_find(const uint64_t* __restrict__ slots,
 // 4 to 32 bits per key and at most 4 MB, small enough to stay in a die's L2
 // while the hash itself does not).  Most probes of the pattern walks find
 // nothing; the filter answers them without touching the hash.
+// Blocked: both bits of a key lie in one 64-bit word, so a test is one read
+// (48-mode H builds 2.19 -> 2.08 s and PT2 pairs 1.97 -> 1.78 s against two
+// independent bits; the 4 MB size is still the fastest).
 __device__ __forceinline__ uint64_t bloom_second(uint64_t H) {
     return ((H ^ (H >> 29)) * 0xBF58476D1CE4E5B9ULL) >> 32;
 }
+// the key's word (index into 64-bit words; fm + 1 bits in all, a multiple of 64) and its two bits
+__device__ __forceinline__ uint64_t bloom_word(uint64_t H, uint64_t fm) { return (H & fm) >> 6; }
+__device__ __forceinline__ uint64_t bloom_bits2(uint64_t H) {
+    const uint64_t s = bloom_second(H);
+    return (1ull << (s & 63)) | (1ull << ((s >> 6) & 63));
+}
 __device__ __forceinline__ void bloom_set(uint32_t* f, uint64_t fm, uint64_t H) {
-    const uint64_t b1 = H & fm, b2 = bloom_second(H) & fm;
-    atomicOr(&f[b1 >> 5], 1u << (b1 & 31));
-    atomicOr(&f[b2 >> 5], 1u << (b2 & 31));
+    atomicOr(reinterpret_cast<unsigned long long*>(f) + bloom_word(H, fm), (unsigned long long)bloom_bits2(H));
 }
 __device__ __forceinline__ bool bloom_maybe(const uint32_t* __restrict__ f, uint64_t fm, uint64_t H) {
-    const uint64_t b1 = H & fm, b2 = bloom_second(H) & fm;
-    return ((f[b1 >> 5] >> (b1 & 31)) & 1u) && ((f[b2 >> 5] >> (b2 & 31)) & 1u);
+    const uint64_t b = bloom_bits2(H);
+    return (reinterpret_cast<const uint64_t*>(f)[bloom_word(H, fm)] & b) == b;
 }
 
 // Probe for the key w moved by pattern e (nz modes; the caller has checked
