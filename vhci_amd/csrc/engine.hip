@@ -1820,11 +1820,14 @@ __global__ __launch_bounds__(256) void k_tsm_reduce(const double* __restrict__ p
 }
 
 // Y = beta Y + alpha A S: A n x p (lda), S p x q (lds, col-major) staged in
-// LDS (p*q <= 4096), one row per thread, q <= 16 per launch tile.
+// LDS (p*q <= 4096), one row per thread, q <= 16 per launch tile.  gate
+// (null: none): nothing is done when *gate is 0.
 __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double alpha, const double* __restrict__ A,
                                                 int64_t lda, const double* __restrict__ S, int lds, double beta,
-                                                double* __restrict__ Y, int64_t ldy) {
+                                                double* __restrict__ Y, int64_t ldy,
+                                                const double* __restrict__ gate) {
     __shared__ double Ss[4096];
+    if (gate && *gate == 0.0) return;
     const int c0 = blockIdx.y * 16;
     const int qq = min(16, q - c0);
     for (int idx = threadIdx.x; idx < p * qq; idx += blockDim.x) {
@@ -1994,11 +1997,13 @@ __global__ __launch_bounds__(256) void k_gate_proj(const double* __restrict__ R,
 }
 
 // T[:, 0..kk) = T[:, 0..k) Z in place (T column-major n x k, ld; Z k x kk
-// column-major, kk <= k <= KMAX): each row is independent
+// column-major, kk <= k <= KMAX): each row is independent.  status (null:
+// none): nothing is done when *status is not 0.
 template <int KMAX>
 __global__ __launch_bounds__(256) void k_rowmix(int64_t n, double* __restrict__ T, int64_t ld, int k, int kk,
-                                                const double* __restrict__ Z) {
+                                                const double* __restrict__ Z, const int* __restrict__ status) {
     __shared__ double Zs[KMAX * KMAX];
+    if (status && *status != 0) return;
     for (int i = threadIdx.x; i < k * kk; i += blockDim.x) Zs[i] = Z[i];
     __syncthreads();
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
@@ -2015,6 +2020,149 @@ __global__ __launch_bounds__(256) void k_rowmix(int64_t n, double* __restrict__ 
                 T[r + c * ld] = acc;
             }
         }
+    }
+}
+
+// One Lanczos block's orthonormalisation finished on the device: the common
+// case of Engine::ortho_cm's Lanczos path, so that a cycle's steps need no
+// host round trip.  R (p x k, ld p, k <= 8) = [V^T T; T^T T] with the
+// projection gate at R[p k] (k_gate_proj); when the gate is set, V S has
+// already been taken from T (k_tsm_nn), so the corrected block's Gram matrix
+// is T^T T - S^T S.  SVQB then as on the host: G scaled by D = diag(G)^-1/2,
+// D G D = U L U^T by Jacobi rotations (here in the parallel round-robin
+// order: the k/2 disjoint pairs of a round are applied at once), and
+// Z = D U L^-1/2 with the columns in descending L, written to Z (k x k).
+// *status = 0 when one pass finishes the block; 1 when it does not (a
+// column lost more than half its squared norm to the projection, a
+// direction would be dropped, or the scaled Gram matrix is ill conditioned):
+// then Z is not written, and the host repeats the step on its general path.
+__global__ __launch_bounds__(64) void k_svqb_block(const double* __restrict__ R, int m0, int k, int p,
+                                                   double* __restrict__ Z, int* __restrict__ status) {
+    __shared__ double A[8][8];  // A[c][i]: column c, row i
+    __shared__ double U[8][8];
+    __shared__ double before[8], dsc[8], lam[8];
+    __shared__ double cs[4], sn[4];
+    __shared__ int pp[4], qq[4], order[8];
+    __shared__ int stop;
+    const int t = threadIdx.x;
+    const bool gated = R[(int64_t)p * k] != 0.0;
+    if (t < k * k) {
+        const int c = t / k, i = t - c * k;
+        double g = R[(int64_t)c * p + m0 + i];
+        if (gated) {
+            double d = 0.0;
+            for (int r = 0; r < m0; r++) d += R[(int64_t)c * p + r] * R[(int64_t)i * p + r];
+            g -= d;
+        }
+        A[c][i] = g;
+        U[c][i] = c == i ? 1.0 : 0.0;
+    }
+    if (t < k) before[t] = R[(int64_t)t * p + m0 + t];
+    __syncthreads();
+    if (t == 0) {
+        int bad = 0;
+        for (int i = 0; i < k; i++) bad |= A[i][i] < 0.5 * before[i];
+        stop = bad;
+    }
+    __syncthreads();
+    if (stop) {
+        if (t == 0) *status = 1;
+        return;
+    }
+    if (t < k) dsc[t] = A[t][t] > 0 ? 1.0 / sqrt(A[t][t]) : 0.0;
+    __syncthreads();
+    if (t < k * k) {
+        const int c = t / k, i = t - c * k;
+        A[c][i] *= dsc[i] * dsc[c];
+    }
+    __syncthreads();
+    const int kd = k + (k & 1);  // a dummy index pairs with the odd one out
+    for (int sweep = 0; sweep < 60; sweep++) {
+        if (t == 0) {
+            double off = 0, tot = 0;
+            for (int j = 0; j < k; j++)
+                for (int i = 0; i < k; i++) {
+                    const double v = A[j][i] * A[j][i];
+                    tot += v;
+                    if (i != j) off += v;
+                }
+            stop = off <= 1e-30 * tot;
+        }
+        __syncthreads();
+        if (stop) break;
+        for (int rd = 0; rd < kd - 1; rd++) {
+            if (t < kd / 2) {
+                int a, b;
+                if (t == 0) {
+                    a = kd - 1;
+                    b = rd;
+                } else {
+                    a = (rd + t) % (kd - 1);
+                    b = (rd - t + kd - 1) % (kd - 1);
+                }
+                const int P = min(a, b), Q = max(a, b);
+                double c = 1.0, sv = 0.0;
+                if (Q < k) {
+                    const double apq = A[Q][P];
+                    if (apq != 0.0) {
+                        const double app = A[P][P], aqq = A[Q][Q];
+                        const double tau = (aqq - app) / (2 * apq);
+                        const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+                        c = 1 / sqrt(1 + tt * tt);
+                        sv = tt * c;
+                    }
+                }
+                pp[t] = P;
+                qq[t] = Q < k ? Q : -1;
+                cs[t] = c;
+                sn[t] = sv;
+            }
+            __syncthreads();
+            const int x = t / 8, r = t - x * 8;  // pair x, element r
+            const bool on = x < kd / 2 && r < k && qq[x] >= 0 && sn[x] != 0.0;
+            if (on) {  // columns P, Q
+                const int P = pp[x], Q = qq[x];
+                const double xv = A[P][r], yv = A[Q][r];
+                A[P][r] = cs[x] * xv - sn[x] * yv;
+                A[Q][r] = sn[x] * xv + cs[x] * yv;
+            }
+            __syncthreads();
+            if (on) {  // rows P, Q, and the eigenvector columns
+                const int P = pp[x], Q = qq[x];
+                const double xv = A[r][P], yv = A[r][Q];
+                A[r][P] = cs[x] * xv - sn[x] * yv;
+                A[r][Q] = sn[x] * xv + cs[x] * yv;
+                const double ux = U[P][r], uy = U[Q][r];
+                U[P][r] = cs[x] * ux - sn[x] * uy;
+                U[Q][r] = sn[x] * ux + cs[x] * uy;
+            }
+            __syncthreads();
+        }
+    }
+    // eigenvalues ascending (ties by index)
+    if (t < k) {
+        const double v = A[t][t];
+        int rank = 0;
+        for (int j = 0; j < k; j++) {
+            const double w = A[j][j];
+            rank += (w < v) || (w == v && j < t);
+        }
+        order[rank] = t;
+        lam[rank] = v;
+    }
+    __syncthreads();
+    if (t == 0) {
+        const double lmax = lam[k - 1], lmin = lam[0];
+        stop = !(lmin > 1e-10 * fmax(1.0, lmax)) || !(lmin > 1e-4 * fmax(1.0, lmax));
+        *status = stop ? 1 : 0;
+    }
+    __syncthreads();
+    if (stop) return;
+    if (t < k * k) {
+        const int c = t / k, i = t - c * k;
+        const int src = k - 1 - c;  // columns in descending eigenvalue
+        const double sc = 1.0 / sqrt(lam[src]);
+        Z[(int64_t)c * k + i] = dsc[i] * U[order[src]][i] * sc;
     }
 }
 
@@ -4313,13 +4461,15 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
     HIPCHK(hipGetLastError());
 }
 
-// Y = beta Y + alpha A S (A n x p, S p x q col-major with lds)
+// Y = beta Y + alpha A S (A n x p, S p x q col-major with lds); gate (on the
+// device, null: none): skipped when *gate is 0
 static void tsm_nn(hipStream_t s, int64_t n, int p, int q, double alpha, const double* A, int64_t lda,
-                   const double* S, int lds, double beta, double* Y, int64_t ldy) {
+                   const double* S, int lds, double beta, double* Y, int64_t ldy, const double* gate = nullptr) {
     for (int a0 = 0; a0 < p; a0 += 256) {
         const int pp = std::min(256, p - a0);
         dim3 g(std::min<unsigned>(grid_for(n, 256), 2048), (q + 15) / 16);
-        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy);
+        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy,
+                                   gate);
     }
     HIPCHK(hipGetLastError());
 }
@@ -4557,9 +4707,9 @@ int Engine::ortho_cm(double* V, int64_t ld, int m0, int k, bool lanczos) {
         HIPCHK(hipMemcpyAsync(S_.p, hz, Z.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
         if (small) {
             const unsigned rg = std::min<unsigned>(grid_for(n, 256), 2048);
-            if (k <= 8) k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>());
-            else if (k <= 16) k_rowmix<16><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>());
-            else k_rowmix<32><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>());
+            if (k <= 8) k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>(), nullptr);
+            else if (k <= 16) k_rowmix<16><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>(), nullptr);
+            else k_rowmix<32><<<rg, 256, 0, stream_>>>(n, T, ld, k, kk, S_.as<double>(), nullptr);
             HIPCHK(hipGetLastError());
         } else {
             BLASCHK(rocblas_dgemm(blas_, rocblas_operation_none, rocblas_operation_none, ni, kk, k, &one, T, ld,
@@ -4826,6 +4976,16 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     int got = ortho_cm(V, ld, 0, bs);
     if (got < bs) throw std::runtime_error("Krylov-Schur: degenerate start block");
     bool low = form_ == kMixed;
+    // Lanczos steps orthonormalised on the device (blocks of at most 8
+    // columns; VHCI_HOST_ORTHO=1: every step on the host)
+    const char* ho = std::getenv("VHCI_HOST_ORTHO");
+    const bool dev_ortho = bs <= 8 && !(ho && std::string(ho) == "1");
+    if (dev_ortho) {
+        // sized once, so that no buffer the queued steps use is replaced
+        ostat_.ensure((size_t)(m / bs + 2) * sizeof(int));
+        ozd_.ensure(64 * sizeof(double));
+        S_.ensure(((size_t)(m + 2 * bs) * bs + 1) * sizeof(double));
+    }
     const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
     const int rpb = 8192;
     const int nblk = (int)((n + rpb - 1) / rpb);
@@ -4837,32 +4997,79 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     for (int it = 0; it < maxit; it++) {
         st.iterations = it + 1;
         bool refilled = false;
-        for (int j = k; j < m; j += bs) {
-            // W_j = H V_j, also written to V_(j+bs) as the next block's start
-            apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st, V + (int64_t)(j + bs) * ld);
-            if (j > k) {
-                // Lanczos step: H V_j has large components only along V_j and
-                // V_(j-bs) (the block-tridiagonal part of the projected
-                // matrix; the kept Ritz vectors couple to V_k alone).  Taking
-                // those out first leaves the full pass below a block that is
-                // already nearly orthogonal, so one projection suffices.
-                const int lo = std::max(k, j - bs);
-                const int p = j + bs - lo;
-                S_.ensure((size_t)std::max(p * bs, (j + bs) * bs) * sizeof(double));
+        // Steps after a cycle's first block (Lanczos steps) finish their
+        // orthonormalisation on the device (k_svqb_block), so they are
+        // enqueued without waiting for the host; each leaves a status, read
+        // once the cycle's steps are queued.  The first step whose status
+        // asks for the general path is repeated on the host (ortho_cm), and
+        // the steps after it are enqueued again.
+        const int* dstat = dev_ortho ? ostat_.as<int>() : nullptr;
+        auto lanczos_steps = [&](int j0, bool host_first) -> int {
+            int queued = 0;  // device steps enqueued since the last status check
+            int first_dev = -1;
+            auto check = [&]() -> int {
+                if (queued == 0) return -1;
+                std::vector<int> h(queued);
+                HIPCHK(hipMemcpyAsync(h.data(), dstat, queued * sizeof(int), hipMemcpyDeviceToHost, stream_));
+                HIPCHK(hipStreamSynchronize(stream_));
+                for (int q = 0; q < queued; q++)
+                    if (h[q] != 0) {
+                        st.device_ortho_redone++;
+                        return first_dev + q * bs;
+                    }
+                st.device_ortho_steps += queued;
+                queued = 0;
+                return -1;
+            };
+            for (int j = j0; j < m; j += bs) {
+                // W_j = H V_j, also written to V_(j+bs) as the next block's start
+                apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st, V + (int64_t)(j + bs) * ld);
                 double* T = V + (int64_t)(j + bs) * ld;
-                tsm_tn(stream_, tsm_part_, n, p, bs, V + (int64_t)lo * ld, ld, T, ld, S_.as<double>(), p);
-                tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
+                if (j > k) {
+                    // Lanczos step: H V_j has large components only along V_j and
+                    // V_(j-bs) (the block-tridiagonal part of the projected
+                    // matrix; the kept Ritz vectors couple to V_k alone).  Taking
+                    // those out first leaves the full pass below a block that is
+                    // already nearly orthogonal, so one projection suffices.
+                    const int lo = std::max(k, j - bs);
+                    const int p = j + bs - lo;
+                    S_.ensure((size_t)std::max(p * bs, (j + bs) * bs) * sizeof(double));
+                    tsm_tn(stream_, tsm_part_, n, p, bs, V + (int64_t)lo * ld, ld, T, ld, S_.as<double>(), p);
+                    tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
+                }
+                if (dev_ortho && j > k && !(host_first && j == j0)) {
+                    // one pass over [V, T] gives V^T T and T^T T; the
+                    // projection runs only when the gate is set (as in
+                    // ortho_cm's Lanczos path)
+                    const int m0 = j + bs, p = m0 + bs;
+                    double* R = S_.as<double>();
+                    tsm_tn(stream_, tsm_part_, n, p, bs, V, ld, T, ld, R, p);
+                    k_gate_proj<<<1, 256, 0, stream_>>>(R, m0, bs, p, kOrthoSkip, R + (size_t)p * bs);
+                    tsm_nn(stream_, n, m0, bs, -1.0, V, ld, R, p, 1.0, T, ld, R + (size_t)p * bs);
+                    if (queued == 0) first_dev = j;
+                    k_svqb_block<<<1, 64, 0, stream_>>>(R, m0, bs, p, ozd_.as<double>(), ostat_.as<int>() + queued);
+                    const unsigned rg = std::min<unsigned>(grid_for(n, 256), 2048);
+                    k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, bs, bs, ozd_.as<double>(),
+                                                          ostat_.as<int>() + queued);
+                    HIPCHK(hipGetLastError());
+                    queued++;
+                    continue;
+                }
+                const int bad = check();
+                if (bad >= 0) return bad;
+                int kept = ortho_cm(V, ld, j + bs, bs, j > k);
+                if (kept < bs) refilled = true;
+                for (int tries = 0; kept < bs && tries < 4; tries++) {
+                    // invariant subspace reached for some directions: refill
+                    random_cm(V + (int64_t)(j + bs + kept) * ld, ld, bs - kept, seed + 977 * (it * m + j + tries + 1));
+                    int more = ortho_cm(V, ld, j + bs + kept, bs - kept);
+                    kept += more;
+                }
+                if (kept < bs) throw std::runtime_error("Krylov-Schur: could not extend the basis");
             }
-            int kept = ortho_cm(V, ld, j + bs, bs, j > k);
-            if (kept < bs) refilled = true;
-            for (int tries = 0; kept < bs && tries < 4; tries++) {
-                // invariant subspace reached for some directions: refill
-                random_cm(V + (int64_t)(j + bs + kept) * ld, ld, bs - kept, seed + 977 * (it * m + j + tries + 1));
-                int more = ortho_cm(V, ld, j + bs + kept, bs - kept);
-                kept += more;
-            }
-            if (kept < bs) throw std::runtime_error("Krylov-Schur: could not extend the basis");
-        }
+            return check();
+        };
+        for (int jf = lanczos_steps(k, false); jf >= 0;) jf = lanczos_steps(jf, true);
         // G[:, k:m] = V[:, :m]^T W[:, k:m] (upper triangle of the new columns).
         // One rocBLAS product over all n rows runs at 2.3 TFLOP/s for this
         // shape; split into row chunks of one batched product (MFMA tiles)
@@ -4964,8 +5171,11 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         }
         st.max_resid = maxrel;
         if (std::getenv("VHCI_DEBUG"))
-            std::fprintf(stderr, "[krylov-schur] cycle %d m %d keep %d bs %d maxrel %.3e unconverged %d %s\n", it, m,
-                         keep, bs, maxrel, unconv, low ? "fp32" : "fp64");
+            std::fprintf(stderr,
+                         "[krylov-schur] cycle %d m %d keep %d bs %d maxrel %.3e unconverged %d %s (device steps %d, "
+                         "repeated %d)\n",
+                         it, m, keep, bs, maxrel, unconv, low ? "fp32" : "fp64", st.device_ortho_steps,
+                         st.device_ortho_redone);
         if (unconv == 0 && !low) {
             E.assign(thh.begin(), thh.begin() + nev);
             X_.ensure((size_t)n * nev * sizeof(double));

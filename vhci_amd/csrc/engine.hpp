@@ -110,6 +110,8 @@ struct EigStats {
     int spmm_calls = 0;
     int64_t spmm_vectors = 0;
     int fp64_spmm_vectors = 0;
+    int device_ortho_steps = 0;   // Lanczos steps orthonormalised on the device
+    int device_ortho_redone = 0;  // of those, repeated on the host's general path
     double max_resid = 0;
     double ms = 0;
     bool dense = false;
@@ -329,6 +331,7 @@ class Engine {
     // eigensolver work
     DevBuf V_, W_, HX_, Xw_, T_b_, G_, S_, theta_, work_, part_, X32_, info_;
     DevBuf stage_in_, stage_in32_, stage_out_, gram_, otmp_, Gw_, tsm_part_, rr_part_, rot_s_;
+    DevBuf ostat_, ozd_;  // device-side Lanczos orthonormalisation: step statuses, the block's Z
     // heat-bath work
     DevBuf cmax_, candcnt_, candoff_, nslots_, pool_keys_, pool_rank_, counters_, col_rank_, col_idx_,
         srt_rank_, srt_idx_, newkeys_, scan_tmp_;
