@@ -100,3 +100,20 @@ def test_c1_matrix_free_matches_oracle(ref, fp64_run, monkeypatch):
     assert r["checksum"] == int(ref["checksum"])
     assert np.max(np.abs(r["E_var"] - np.array(ref["E_var"]))) < 1e-6
     assert np.max(np.abs(r["E_var"] - fp64_run["E_var"])) < 1e-6
+
+
+def test_device_orthonormalisation_repeat_path(monkeypatch):
+    """Lanczos steps are orthonormalised on the device and only a step the
+    device declines is repeated on the host.  With every step declined
+    (VHCI_HOST_ORTHO=repeat) the solve must equal the host-only solve
+    (VHCI_HOST_ORTHO=1) bit for bit, and the device form must agree with both
+    within the eigensolver's tolerance and select the same basis."""
+    out = {}
+    for mode in ("1", "repeat", "0"):
+        monkeypatch.setenv("VHCI_HOST_ORTHO", mode)
+        ctx = vhci_amd.Context(vhci_amd.Problem(text=synth.config_input(1, pt2=False)), vhci_amd.FP64)
+        out[mode] = ctx.run()
+        ctx.close()
+    assert out["repeat"]["checksum"] == out["1"]["checksum"] == out["0"]["checksum"]
+    assert np.array_equal(out["repeat"]["E_var"], out["1"]["E_var"])
+    assert np.max(np.abs(out["0"]["E_var"] - out["1"]["E_var"])) < 1e-6

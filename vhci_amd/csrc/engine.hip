@@ -2023,45 +2023,72 @@ __global__ __launch_bounds__(256) void k_rowmix(int64_t n, double* __restrict__ 
     }
 }
 
+// the largest basis offset m0 that k_orth_block stages in LDS (8 x 1024
+// doubles, 64 KB); steps beyond it take the host path
+constexpr int kSvqbRows = 1024;
+
 // One Lanczos block's orthonormalisation finished on the device: the common
 // case of Engine::ortho_cm's Lanczos path, so that a cycle's steps need no
 // host round trip.  R (p x k, ld p, k <= 8) = [V^T T; T^T T] with the
 // projection gate at R[p k] (k_gate_proj); when the gate is set, V S has
 // already been taken from T (k_tsm_nn), so the corrected block's Gram matrix
-// is T^T T - S^T S.  SVQB then as on the host: G scaled by D = diag(G)^-1/2,
-// D G D = U L U^T by Jacobi rotations (here in the parallel round-robin
-// order: the k/2 disjoint pairs of a round are applied at once), and
-// Z = D U L^-1/2 with the columns in descending L, written to Z (k x k).
-// *status = 0 when one pass finishes the block; 1 when it does not (a
-// column lost more than half its squared norm to the projection, a
-// direction would be dropped, or the scaled Gram matrix is ill conditioned):
-// then Z is not written, and the host repeats the step on its general path.
-__global__ __launch_bounds__(64) void k_svqb_block(const double* __restrict__ R, int m0, int k, int p,
-                                                   double* __restrict__ Z, int* __restrict__ status) {
+// is G = T^T T - S^T S.  With D = diag(G)^-1/2 and the Cholesky factor
+// D G D = L L^T, Z = D L^-T makes T Z orthonormal (Cholesky QR), written to
+// Z (k x k, upper triangular).  *status = 0 when one pass finishes the
+// block; 1 when it does not (a column lost more than half its squared norm
+// to the projection, or the Frobenius condition bound |DGD| |(DGD)^-1| of
+// the scaled Gram matrix reaches 1e4, which covers the host's test that its
+// eigenvalues lie within a factor 1e4): then Z is not written, and the host
+// repeats the step on its general path (SVQB).
+__global__ __launch_bounds__(256) void k_orth_block(const double* __restrict__ R, int m0, int k, int p,
+                                                    double* __restrict__ Z, int* __restrict__ status,
+                                                    int decline) {
     __shared__ double A[8][8];  // A[c][i]: column c, row i
-    __shared__ double U[8][8];
-    __shared__ double before[8], dsc[8], lam[8];
-    __shared__ double cs[4], sn[4];
-    __shared__ int pp[4], qq[4], order[8];
+    __shared__ double Li[8][8];  // L^-1 (lower), Li[r][c]: row r, column c
+    __shared__ double before[8], dsc[8];
+    __shared__ double inv2[64];
+    __shared__ double gf2;  // squared Frobenius norm of the scaled Gram matrix
     __shared__ int stop;
+    __shared__ double Rs[8 * kSvqbRows];  // the projection coefficients S (m0 x k), column by column
     const int t = threadIdx.x;
+    if (decline) {  // every step to the host's general path (a test of that path)
+        if (t == 0) *status = 1;
+        return;
+    }
     const bool gated = R[(int64_t)p * k] != 0.0;
-    if (t < k * k) {
-        const int c = t / k, i = t - c * k;
-        double g = R[(int64_t)c * p + m0 + i];
-        if (gated) {
-            double d = 0.0;
-            for (int r = 0; r < m0; r++) d += R[(int64_t)c * p + r] * R[(int64_t)i * p + r];
-            g -= d;
+    if (gated) {
+        // S^T S: S staged in LDS, four lanes per entry (quarters of the rows,
+        // summed in a fixed order)
+        for (int idx = t; idx < k * m0; idx += blockDim.x) {
+            const int c = idx / m0, r = idx - c * m0;
+            Rs[c * m0 + r] = R[(int64_t)c * p + r];
         }
-        A[c][i] = g;
-        U[c][i] = c == i ? 1.0 : 0.0;
+        __syncthreads();
+        const int e = t >> 2, part = t & 3;
+        double d = 0.0;
+        if (e < k * k) {
+            const int c = e / k, i = e - c * k;
+            const int r0 = (m0 * part) / 4, r1 = (m0 * (part + 1)) / 4;
+            for (int r = r0; r < r1; r++) d += Rs[c * m0 + r] * Rs[i * m0 + r];
+        }
+        d += __shfl_xor(d, 1, 4);
+        d += __shfl_xor(d, 2, 4);
+        if (e < k * k && part == 0) {
+            const int c = e / k, i = e - c * k;
+            A[c][i] = R[(int64_t)c * p + m0 + i] - d;
+        }
+    } else if (t < k * k) {
+        const int c = t / k, i = t - c * k;
+        A[c][i] = R[(int64_t)c * p + m0 + i];
     }
     if (t < k) before[t] = R[(int64_t)t * p + m0 + t];
     __syncthreads();
     if (t == 0) {
         int bad = 0;
-        for (int i = 0; i < k; i++) bad |= A[i][i] < 0.5 * before[i];
+        for (int i = 0; i < k; i++) {
+            if (A[i][i] < 0.5 * before[i]) bad = 1;
+            if (!(A[i][i] > 0.0)) bad = 1;
+        }
         stop = bad;
     }
     __syncthreads();
@@ -2069,100 +2096,75 @@ __global__ __launch_bounds__(64) void k_svqb_block(const double* __restrict__ R,
         if (t == 0) *status = 1;
         return;
     }
-    if (t < k) dsc[t] = A[t][t] > 0 ? 1.0 / sqrt(A[t][t]) : 0.0;
+    if (t < k) dsc[t] = 1.0 / sqrt(A[t][t]);
     __syncthreads();
     if (t < k * k) {
         const int c = t / k, i = t - c * k;
         A[c][i] *= dsc[i] * dsc[c];
     }
     __syncthreads();
-    const int kd = k + (k & 1);  // a dummy index pairs with the odd one out
-    for (int sweep = 0; sweep < 60; sweep++) {
-        if (t == 0) {
-            double off = 0, tot = 0;
-            for (int j = 0; j < k; j++)
-                for (int i = 0; i < k; i++) {
-                    const double v = A[j][i] * A[j][i];
-                    tot += v;
-                    if (i != j) off += v;
-                }
-            stop = off <= 1e-30 * tot;
+    if (t == 0) {
+        double f2 = 0.0;
+        for (int c = 0; c < k; c++)
+            for (int i = 0; i < k; i++) f2 += A[c][i] * A[c][i];
+        gf2 = f2;
+        // Cholesky of the scaled Gram matrix (lower L in A's lower triangle,
+        // A[c][i] = L[i][c] for i >= c), then L^-1 by forward substitution
+        int bad = 0;
+        for (int c = 0; c < k && !bad; c++) {
+            double dd = A[c][c];
+            for (int r = 0; r < c; r++) dd -= A[r][c] * A[r][c];
+            if (!(dd > 0.0)) {
+                bad = 1;
+                break;
+            }
+            const double l = sqrt(dd);
+            A[c][c] = l;
+            for (int i = c + 1; i < k; i++) {
+                double v = A[c][i];
+                for (int r = 0; r < c; r++) v -= A[r][i] * A[r][c];
+                A[c][i] = v / l;
+            }
         }
-        __syncthreads();
-        if (stop) break;
-        for (int rd = 0; rd < kd - 1; rd++) {
-            if (t < kd / 2) {
-                int a, b;
-                if (t == 0) {
-                    a = kd - 1;
-                    b = rd;
-                } else {
-                    a = (rd + t) % (kd - 1);
-                    b = (rd - t + kd - 1) % (kd - 1);
-                }
-                const int P = min(a, b), Q = max(a, b);
-                double c = 1.0, sv = 0.0;
-                if (Q < k) {
-                    const double apq = A[Q][P];
-                    if (apq != 0.0) {
-                        const double app = A[P][P], aqq = A[Q][Q];
-                        const double tau = (aqq - app) / (2 * apq);
-                        const double tt = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
-                        c = 1 / sqrt(1 + tt * tt);
-                        sv = tt * c;
+        if (!bad)
+            for (int c = 0; c < k; c++)
+                for (int r = 0; r < k; r++) {
+                    if (r < c) {
+                        Li[r][c] = 0.0;
+                        continue;
                     }
+                    double v = r == c ? 1.0 : 0.0;
+                    for (int q = c; q < r; q++) v -= A[q][r] * Li[q][c];
+                    Li[r][c] = v / A[r][r];
                 }
-                pp[t] = P;
-                qq[t] = Q < k ? Q : -1;
-                cs[t] = c;
-                sn[t] = sv;
-            }
-            __syncthreads();
-            const int x = t / 8, r = t - x * 8;  // pair x, element r
-            const bool on = x < kd / 2 && r < k && qq[x] >= 0 && sn[x] != 0.0;
-            if (on) {  // columns P, Q
-                const int P = pp[x], Q = qq[x];
-                const double xv = A[P][r], yv = A[Q][r];
-                A[P][r] = cs[x] * xv - sn[x] * yv;
-                A[Q][r] = sn[x] * xv + cs[x] * yv;
-            }
-            __syncthreads();
-            if (on) {  // rows P, Q, and the eigenvector columns
-                const int P = pp[x], Q = qq[x];
-                const double xv = A[r][P], yv = A[r][Q];
-                A[r][P] = cs[x] * xv - sn[x] * yv;
-                A[r][Q] = sn[x] * xv + cs[x] * yv;
-                const double ux = U[P][r], uy = U[Q][r];
-                U[P][r] = cs[x] * ux - sn[x] * uy;
-                U[Q][r] = sn[x] * ux + cs[x] * uy;
-            }
-            __syncthreads();
-        }
+        stop = bad;
     }
-    // eigenvalues ascending (ties by index)
-    if (t < k) {
-        const double v = A[t][t];
-        int rank = 0;
-        for (int j = 0; j < k; j++) {
-            const double w = A[j][j];
-            rank += (w < v) || (w == v && j < t);
-        }
-        order[rank] = t;
-        lam[rank] = v;
+    __syncthreads();
+    if (stop) {
+        if (t == 0) *status = 1;
+        return;
+    }
+    // (D G D)^-1 = L^-T L^-1: its squared entries, one per lane
+    if (t < k * k) {
+        const int i = t / k, j = t - i * k;
+        double v = 0.0;
+        for (int r = 0; r < k; r++) v += Li[r][i] * Li[r][j];
+        inv2[t] = v * v;
     }
     __syncthreads();
     if (t == 0) {
-        const double lmax = lam[k - 1], lmin = lam[0];
-        stop = !(lmin > 1e-10 * fmax(1.0, lmax)) || !(lmin > 1e-4 * fmax(1.0, lmax));
+        double i2 = 0.0;
+        for (int e = 0; e < k * k; e++) i2 += inv2[e];
+        // cond_2 <= |DGD|_F |(DGD)^-1|_F
+        stop = !(sqrt(gf2 * i2) < 1e4);
         *status = stop ? 1 : 0;
     }
     __syncthreads();
     if (stop) return;
     if (t < k * k) {
+        // Z = D L^-T: Z[i][c] = dsc[i] * Li[c][i]
         const int c = t / k, i = t - c * k;
-        const int src = k - 1 - c;  // columns in descending eigenvalue
-        const double sc = 1.0 / sqrt(lam[src]);
-        Z[(int64_t)c * k + i] = dsc[i] * U[order[src]][i] * sc;
+        Z[(int64_t)c * k + i] = dsc[i] * Li[c][i];
     }
 }
 
@@ -4980,6 +4982,9 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     // columns; VHCI_HOST_ORTHO=1: every step on the host)
     const char* ho = std::getenv("VHCI_HOST_ORTHO");
     const bool dev_ortho = bs <= 8 && !(ho && std::string(ho) == "1");
+    // VHCI_HOST_ORTHO=repeat: the device declines every step, so each one
+    // is repeated on the host (a test of the repeat path)
+    const bool decline_dev = ho && std::string(ho) == "repeat";
     if (dev_ortho) {
         // sized once, so that no buffer the queued steps use is replaced
         ostat_.ensure((size_t)(m / bs + 2) * sizeof(int));
@@ -4998,15 +5003,17 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         st.iterations = it + 1;
         bool refilled = false;
         // Steps after a cycle's first block (Lanczos steps) finish their
-        // orthonormalisation on the device (k_svqb_block), so they are
+        // orthonormalisation on the device (k_orth_block), so they are
         // enqueued without waiting for the host; each leaves a status, read
         // once the cycle's steps are queued.  The first step whose status
         // asks for the general path is repeated on the host (ortho_cm), and
-        // the steps after it are enqueued again.
+        // the steps after it are enqueued again, checked after 1, 2, 4, ...
+        // steps so that repeated refusals cost no more than the host path.
         const int* dstat = dev_ortho ? ostat_.as<int>() : nullptr;
         auto lanczos_steps = [&](int j0, bool host_first) -> int {
             int queued = 0;  // device steps enqueued since the last status check
             int first_dev = -1;
+            int batch = host_first ? 1 : m;  // steps between checks
             auto check = [&]() -> int {
                 if (queued == 0) return -1;
                 std::vector<int> h(queued);
@@ -5037,7 +5044,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
                     tsm_tn(stream_, tsm_part_, n, p, bs, V + (int64_t)lo * ld, ld, T, ld, S_.as<double>(), p);
                     tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
                 }
-                if (dev_ortho && j > k && !(host_first && j == j0)) {
+                if (dev_ortho && j > k && j + bs <= kSvqbRows && !(host_first && j == j0)) {
                     // one pass over [V, T] gives V^T T and T^T T; the
                     // projection runs only when the gate is set (as in
                     // ortho_cm's Lanczos path)
@@ -5047,12 +5054,18 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
                     k_gate_proj<<<1, 256, 0, stream_>>>(R, m0, bs, p, kOrthoSkip, R + (size_t)p * bs);
                     tsm_nn(stream_, n, m0, bs, -1.0, V, ld, R, p, 1.0, T, ld, R + (size_t)p * bs);
                     if (queued == 0) first_dev = j;
-                    k_svqb_block<<<1, 64, 0, stream_>>>(R, m0, bs, p, ozd_.as<double>(), ostat_.as<int>() + queued);
+                    k_orth_block<<<1, 256, 0, stream_>>>(R, m0, bs, p, ozd_.as<double>(), ostat_.as<int>() + queued,
+                                                          decline_dev ? 1 : 0);
                     const unsigned rg = std::min<unsigned>(grid_for(n, 256), 2048);
                     k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, bs, bs, ozd_.as<double>(),
                                                           ostat_.as<int>() + queued);
                     HIPCHK(hipGetLastError());
                     queued++;
+                    if (queued >= batch) {
+                        const int bad = check();
+                        if (bad >= 0) return bad;
+                        batch *= 2;
+                    }
                     continue;
                 }
                 const int bad = check();
