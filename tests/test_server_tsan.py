@@ -7,7 +7,9 @@ heat-bath loop it runs per request) are compiled with g++
 thread's device-byte counters).  Two client threads submit 40 requests to a
 four-slot server while a third polls its statistics; the program checks
 every result against the same request run alone, and TSan stops it with
-status 66 at the first data race.
+status 66 at the first data race.  A second scenario checks admission: a
+request too large to fit beside the running one waits, and a small request
+queued after it starts first.
 """
 import os
 import shutil
@@ -34,3 +36,5 @@ def test_server_threads_race_free(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=env)
     assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-6000:])
     assert "0 differing from alone" in r.stdout
+    # a large request waiting for memory does not hold back a smaller one queued after it
+    assert "while the first ran: yes" in r.stdout and "passed over 1 time(s); all done: yes" in r.stdout

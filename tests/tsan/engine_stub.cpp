@@ -8,6 +8,7 @@
 // request, so every result is reproducible and can be checked.
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <thread>
 
 #include "engine.hpp"
@@ -18,7 +19,9 @@ thread_local int64_t t_bytes = 0;
 thread_local int64_t t_peak = 0;
 thread_local DevPool* t_pool = nullptr;
 
-void nap(int64_t us) { std::this_thread::sleep_for(std::chrono::microseconds(us)); }
+// requests of STUB_SLOW_MODES modes (environment) take 200 times as long
+thread_local bool t_slow = false;
+void nap(int64_t us) { std::this_thread::sleep_for(std::chrono::microseconds(t_slow ? 200 * us : us)); }
 }  // namespace
 
 int64_t thread_device_bytes() { return t_bytes; }
@@ -39,6 +42,8 @@ void* DevBuf::ensure(size_t) { return nullptr; }
 
 Engine::Engine(const Problem& P, int form, int device) : P_(P), form_(form), device_(device) {
     KW_ = (P_.M + 15) / 16;
+    const char* slow = std::getenv("STUB_SLOW_MODES");
+    t_slow = slow && std::atoi(slow) == P_.M;
     t_bytes += 1 << 20;
     t_peak = std::max(t_peak, t_bytes);
 }

@@ -3331,6 +3331,9 @@ void Engine::build_h_t(HStats& st) {
         int64_t words = 0;
         HIPCHK(hipMemcpyAsync(&words, woff.as<int64_t>() + nb, sizeof(int64_t), hipMemcpyDeviceToHost, stream_));
         HIPCHK(hipStreamSynchronize(stream_));
+        if (std::getenv("VHCI_DEBUG"))
+            std::fprintf(stderr, "[h-build] %lld rows walk %lld steps of 64 patterns (%.0f per row; table %lld)\n",
+                         (long long)nb, (long long)words, (double)words / (double)nb, (long long)a.P);
         size_t free_b = 0, total_b = 0;
         HIPCHK(hipMemGetInfo(&free_b, &total_b));
         if ((double)words * 8.0 > 0.25 * ((double)free_b + (double)pool_.idle_bytes)) return;

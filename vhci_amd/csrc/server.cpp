@@ -53,6 +53,19 @@ double Server::now() const {
 
 bool Server::fits(const Request& r) const { return running_ == 0 || reserved_ + r.predicted <= limit_; }
 
+// Admission: the first queued request that fits beside the running ones, so
+// that a large request waiting for memory does not hold back smaller ones
+// queued after it; but a request passed over kMaxPass times is not passed
+// again (the scan stops there), so it runs once the running ones finish.
+int Server::pick() const {
+    for (size_t i = 0; i < queue_.size(); i++) {
+        const Request& q = *reqs_.at(queue_[i]);
+        if (fits(q)) return (int)i;
+        if (q.passed >= kMaxPass) return -1;
+    }
+    return -1;
+}
+
 int64_t Server::submit(const std::string& text, int form, double tol) {
     auto r = std::make_unique<Request>();
     std::istringstream in(text);
@@ -109,10 +122,12 @@ void Server::stats(double out[5]) {
 void Server::worker(int slot) {
     while (true) {
         std::unique_lock<std::mutex> lk(mu_);
-        cv_work_.wait(lk, [&] { return stop_ || (!queue_.empty() && fits(*reqs_[queue_.front()])); });
+        cv_work_.wait(lk, [&] { return stop_ || pick() >= 0; });
         if (stop_) return;
-        const int64_t id = queue_.front();
-        queue_.pop_front();
+        const int qi = pick();
+        const int64_t id = queue_[qi];
+        for (int i = 0; i < qi; i++) reqs_[queue_[i]]->passed++;
+        queue_.erase(queue_.begin() + qi);
         Request& r = *reqs_[id];
         r.state = State::Running;
         r.slot = slot;

@@ -45,6 +45,7 @@ struct Request {
     uint64_t checksum = 0; // basis checksum of the final variational basis
     double t_submit = 0, t_start = 0, t_end = 0;  // seconds since server start
     int slot = -1;
+    int passed = 0;  // requests queued after it that were admitted before it
 };
 
 // Device bytes predicted for a solve whose basis has n states (the state
@@ -69,10 +70,16 @@ class Server {
     // running, queued, reserved bytes, max concurrent so far, finished
     void stats(double out[5]);
 
+    // a queued request is passed over by later ones that fit at most this
+    // many times; after that the requests behind it wait for it
+    static constexpr int kMaxPass = 16;
+
   private:
     void worker(int slot);
     double now() const;
     bool fits(const Request& r) const;
+    // position in the queue of the request to admit next, or -1 (mu_ held)
+    int pick() const;
 
     int device_;
     int slots_;
