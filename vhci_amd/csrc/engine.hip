@@ -233,6 +233,18 @@ __device__ __forceinline__ uint4 rec_load(const T* base, int64_t i) {
 __device__ __forceinline__ int delta_nz(const uint4& e) { return (int)(e.w >> 28); }
 __device__ __forceinline__ uint32_t delta_group(const uint4& e) { return e.w & 0x03ffffffu; }
 
+// Walk record: the part of a connection pattern that the pattern walks test
+// (modes and quanta changes), 8 bytes instead of the 16 of DeltaEnt, so the
+// walks stream half the bytes; the full record is read only for a hit.
+// Bits 10 s .. 10 s + 5: mode s (< 64, as every key holds at most 64
+// modes); 10 s + 6 .. 10 s + 9: its change, signed (|change| <= 6); bits
+// 60..63: the number of modes.
+__device__ __forceinline__ int delta_nz(uint64_t e) { return (int)(e >> 60); }
+__device__ __forceinline__ int walk_mode(uint64_t e, int s) { return (int)((e >> (10 * s)) & 63u); }
+__device__ __forceinline__ int walk_delta(uint64_t e, int s) {
+    return (int)((int64_t)(e << (54 - 10 * s)) >> 60);  // bits 10 s + 6 .. 10 s + 9, sign-extended
+}
+
 __device__ __forceinline__ double elem_load(const ElemFC* __restrict__ elem, uint32_t i, uint4& b) {
     const uint64_t* q = reinterpret_cast<const uint64_t*>(elem + i);
     const uint64_t u = q[1], v = q[2];
@@ -362,6 +374,7 @@ struct HBuildArgs {
     const uint32_t* bloom;  // filter of the hash's keys (bits bmask + 1)
     uint64_t bmask;
     const DeltaEnt* deltas;
+    const uint64_t* walk;  // the walk records of deltas
     const uint32_t* poff;     // M + 2 block offsets of deltas
     const GroupEnt* groups;
     const uint32_t* fclist;
@@ -483,6 +496,17 @@ __device__ __forceinline__ bool pattern_ok(const uint4 e, int nz, const uint8_t*
     }
     return ok;
 }
+__device__ __forceinline__ bool pattern_ok(uint64_t e, int nz, const uint8_t* __restrict__ sq) {
+    bool ok = true;
+#pragma unroll
+    for (int s = 0; s < kMaxD; s++) {
+        if (s < nz) {
+            const int nqv = (int)sq[walk_mode(e, s)] + walk_delta(e, s);
+            ok = ok && nqv >= 0 && nqv <= 255;
+        }
+    }
+    return ok;
+}
 
 template <int NW>
 __device__ __forceinline__ int64_t probe_shift(const uint64_t* __restrict__ slots, uint64_t mask,
@@ -505,6 +529,35 @@ __device__ __forceinline__ int64_t probe_shift(const uint64_t* __restrict__ slot
 #pragma unroll
     for (int k = 0; k < kMaxD; k++)
         if (k < nz) key_add<NW>(t, (int)rec_u8(e, k), rec_i8(e, 6 + k));
+    while (true) {
+        if ((uint32_t)(s >> 32) == tag && key_eq<NW>(keys, (uint32_t)s, t)) return (int64_t)(uint32_t)s;
+        pos = (pos + 1) & mask;
+        s = slots[pos];
+        if (s == kEmpty) return -1;
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ int64_t probe_shift(const uint64_t* __restrict__ slots, uint64_t mask,
+                                               const uint4* __restrict__ keys, const uint32_t (&w)[NW],
+                                               uint64_t e, int nz, uint64_t H) {
+    uint64_t pos = H & mask;
+    const uint32_t tag = (uint32_t)(H >> 32);
+    // most probes end at an empty slot before any tag matches: the moved key
+    // is built only after the first match
+    uint64_t s;
+    while (true) {
+        s = slots[pos];
+        if (s == kEmpty) return -1;
+        if ((uint32_t)(s >> 32) == tag) break;
+        pos = (pos + 1) & mask;
+    }
+    uint32_t t[NW];
+#pragma unroll
+    for (int i = 0; i < NW; i++) t[i] = w[i];
+#pragma unroll
+    for (int k = 0; k < kMaxD; k++)
+        if (k < nz) key_add<NW>(t, walk_mode(e, k), walk_delta(e, k));
     while (true) {
         if ((uint32_t)(s >> 32) == tag && key_eq<NW>(keys, (uint32_t)s, t)) return (int64_t)(uint32_t)s;
         pos = (pos + 1) & mask;
@@ -584,6 +637,13 @@ __device__ __forceinline__ uint64_t pattern_shift(const uint4 e, int nz, const u
 #pragma unroll
     for (int s = 0; s < kMaxD; s++)
         if (s < nz) h += (uint64_t)(int64_t)rec_i8(e, 6 + s) * cf[rec_u8(e, s)];
+    return h;
+}
+__device__ __forceinline__ uint64_t pattern_shift(uint64_t e, int nz, const uint64_t* __restrict__ cf) {
+    uint64_t h = 0;
+#pragma unroll
+    for (int s = 0; s < kMaxD; s++)
+        if (s < nz) h += (uint64_t)(int64_t)walk_delta(e, s) * cf[walk_mode(e, s)];
     return h;
 }
 
@@ -668,7 +728,7 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     const bool known = FILL && hw;  // fill pass over recorded hits
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
     // each lane's next pattern record is loaded one step ahead
-    uint4 en = !known && pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
+    uint64_t en = !known && pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull;
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
         bool hit = false;
@@ -677,15 +737,15 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
             const uint64_t word = *hw++;
             if (word == 0ull) continue;
             if ((word >> lane) & 1ull) {
-                const uint4 e = rec_load(a.deltas, p);
+                const uint64_t e = a.walk[p];
                 const int nz = delta_nz(e);
                 const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
                 j = probe_shift<NW>(a.slots, a.mask, a.keys, w, e, nz, H);
                 hit = j >= 0;
             }
         } else {
-            const uint4 e = en;
-            if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
+            const uint64_t e = en;
+            if (p + kWave < pb1) en = a.walk[p + kWave];
             if (p < pb1) {
                 const int nz = delta_nz(e);
                 if (pattern_ok(e, nz, sq)) {
@@ -793,11 +853,11 @@ __global__ __launch_bounds__(64) void k_hfree(HFreeArgs f) {
     int blk = -1, pb0 = 0, pb1 = 0;
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
     // each lane's next pattern record is loaded one step ahead
-    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
+    uint64_t en = pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull;
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
-        const uint4 e = en;
-        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
+        const uint64_t e = en;
+        if (p + kWave < pb1) en = a.walk[p + kWave];
         bool hit = false;
         int64_t j = -1;
         if (p < pb1) {
@@ -2592,6 +2652,7 @@ struct Pt2PushArgs {
     const uint4* apool;
     const uint32_t* apos;
     const DeltaEnt* deltas;
+    const uint64_t* walk;    // the walk records of deltas
     const uint32_t* poff;    // M + 2 block offsets of deltas
     uint64_t tlo, thi;       // external states whose hash tag lies in [tlo, thi)
     int64_t rstride;       // count pass over every rstride-th state only (a sample)
@@ -2618,11 +2679,11 @@ __global__ __launch_bounds__(64) void k_pt2_push(Pt2PushArgs a) {
     int blk = -1, pb0 = 0, pb1 = 0;
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
     // each lane's next pattern record is loaded one step ahead
-    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
+    uint64_t en = pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull;
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
-        const uint4 e = en;
-        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
+        const uint64_t e = en;
+        if (p + kWave < pb1) en = a.walk[p + kWave];
         bool hit = false;
         int64_t idx = -1;
         if (p < pb1) {
@@ -2694,11 +2755,11 @@ __global__ __launch_bounds__(64) void k_pt2_push1(Pt2PushArgs a, unsigned long l
     int blk = -1, pb0 = 0, pb1 = 0;
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
     // each lane's next pattern record is loaded one step ahead
-    uint4 en = pb0 + lane < pb1 ? rec_load(a.deltas, pb0 + lane) : uint4{0u, 0u, 0u, 0u};
+    uint64_t en = pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull;
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
-        const uint4 e = en;
-        if (p + kWave < pb1) en = rec_load(a.deltas, p + kWave);
+        const uint64_t e = en;
+        if (p + kWave < pb1) en = a.walk[p + kWave];
         bool hit = false;
         int64_t idx = -1;
         if (p < pb1) {
@@ -3088,12 +3149,33 @@ static void upload(DevBuf& b, const std::vector<T>& v, hipStream_t s) {
     if (!v.empty()) HIPCHK(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
 }
 
+// the pattern walks' 8-byte records of a pattern table (modes < 64,
+// changes within -8..7; layout at walk_mode / walk_delta)
+static std::vector<uint64_t> walk_records(const std::vector<DeltaEnt>& table) {
+    std::vector<uint64_t> walk(table.size());
+    for (size_t i = 0; i < walk.size(); i++) {
+        const DeltaEnt& d = table[i];
+        const int nz = (int)(d.meta >> 28);
+        if (nz > kMaxD) throw std::runtime_error("connection pattern of more than 6 modes");
+        uint64_t r = (uint64_t)nz << 60;
+        for (int t = 0; t < nz; t++) {
+            if (d.mode[t] >= 64 || d.delta[t] < -8 || d.delta[t] > 7)
+                throw std::runtime_error("connection pattern outside the walk record's range");
+            r |= (uint64_t)d.mode[t] << (10 * t);
+            r |= (uint64_t)((unsigned)d.delta[t] & 15u) << (10 * t + 6);
+        }
+        walk[i] = r;
+    }
+    return walk;
+}
+
 void Engine::upload_tables() {
     std::vector<double> L;
     build_ladder(255, L);
     upload(d_ladder_, L, stream_);
     upload(d_elem_, T_.elem, stream_);
     upload(d_deltas_, T_.deltas, stream_);
+    upload(d_walk_, walk_records(T_.deltas), stream_);
     upload(d_doff_, T_.delta_off, stream_);
     upload(d_groups_, T_.groups, stream_);
     upload(d_fclist_, T_.fclist, stream_);
@@ -3272,6 +3354,8 @@ void Engine::build_h_t(HStats& st) {
     a.bloom = bloom_.as<uint32_t>();
     a.bmask = bmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
+    a.walk = d_walk_.as<uint64_t>();
+    if (!a.walk) throw std::runtime_error("pattern walk records missing");
     a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
     a.fclist = d_fclist_.as<uint32_t>();
@@ -3811,6 +3895,7 @@ int64_t Engine::structural_nnz() {
     HIPCHK(hipSetDevice(device_));
     if (!d_ball_.p) {
         upload(d_ball_, T_.ball, stream_);
+        upload(d_ball_walk_, walk_records(T_.ball), stream_);
         upload(d_boff_, T_.ball_off, stream_);
     }
     HBuildArgs a{};
@@ -3823,7 +3908,9 @@ int64_t Engine::structural_nnz() {
     a.bloom = bloom_.as<uint32_t>();
     a.bmask = bmask_;
     a.deltas = d_ball_.as<DeltaEnt>();
+    a.walk = d_ball_walk_.as<uint64_t>();
     a.poff = d_boff_.as<uint32_t>();
+    if (!a.walk) throw std::runtime_error("structural_nnz: walk records missing");
     rowcnt_.ensure((n_ + 1) * sizeof(int64_t));
     a.rowcnt = rowcnt_.as<int64_t>();
     switch (KW_) {
@@ -3913,6 +4000,8 @@ void Engine::spmm_free_t(const double* X, int ldx, double* Y, int ldy, int b) {
     a.bloom = bloom_.as<uint32_t>();
     a.bmask = bmask_;
     a.deltas = d_deltas_.as<DeltaEnt>();
+    a.walk = d_walk_.as<uint64_t>();
+    if (!a.walk) throw std::runtime_error("pattern walk records missing");
     a.poff = d_doff_.as<uint32_t>();
     a.groups = d_groups_.as<GroupEnt>();
     a.fclist = d_fclist_.as<uint32_t>();
@@ -5621,6 +5710,8 @@ bool Engine::pt2_push_t(const double* Ed, int nopt, int nwaves, int64_t max_pair
     a.apool = newkeys_.as<uint4>();
     a.apos = nullptr;
     a.deltas = d_deltas_.as<DeltaEnt>();
+    a.walk = d_walk_.as<uint64_t>();
+    if (!a.walk) throw std::runtime_error("pattern walk records missing");
     a.poff = d_doff_.as<uint32_t>();
     a.tlo = tlo;
     a.thi = thi;

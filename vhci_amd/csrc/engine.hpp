@@ -290,6 +290,8 @@ class Engine {
     // tables
     DevBuf d_ladder_, d_elem_, d_deltas_, d_groups_, d_fclist_, d_diag_, d_freq_;
     DevBuf d_hbfc_, d_hbvaroff_, d_hbvars_, d_ball_;
+    DevBuf d_walk_;           // the pattern walks' 8-byte records of d_deltas_
+    DevBuf d_ball_walk_;      // and of d_ball_
     DevBuf d_doff_, d_boff_;  // block offsets of the pattern table and the ball
     int P_deltas_ = 0;
     // Hamiltonian
