@@ -117,3 +117,19 @@ def test_device_orthonormalisation_repeat_path(monkeypatch):
     assert out["repeat"]["checksum"] == out["1"]["checksum"] == out["0"]["checksum"]
     assert np.array_equal(out["repeat"]["E_var"], out["1"]["E_var"])
     assert np.max(np.abs(out["0"]["E_var"] - out["1"]["E_var"])) < 1e-6
+
+
+@pytest.mark.parametrize("form", [vhci_amd.FP64, vhci_amd.MIXED])
+def test_graph_captured_cycles_equal_eager(monkeypatch, form):
+    """VHCI_GRAPH=1 captures a cycle's device Lanczos steps once in a HIP
+    graph and replays it in the cycles of the same shape (the mixed form's
+    FP32 and FP64 cycles each get their own); the solve must equal the eager
+    one bit for bit."""
+    out = {}
+    for mode in ("0", "1"):
+        monkeypatch.setenv("VHCI_GRAPH", mode)
+        ctx = vhci_amd.Context(vhci_amd.Problem(text=synth.config_input(1, pt2=False)), form)
+        out[mode] = ctx.run()
+        ctx.close()
+    assert out["1"]["checksum"] == out["0"]["checksum"]
+    assert np.array_equal(out["1"]["E_var"], out["0"]["E_var"])

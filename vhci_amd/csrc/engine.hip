@@ -5077,6 +5077,26 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     // VHCI_HOST_ORTHO=repeat: the device declines every step, so each one
     // is repeated on the host (a test of the repeat path)
     const bool decline_dev = ho && std::string(ho) == "repeat";
+    // VHCI_GRAPH=1: a cycle's device steps are captured once in a HIP graph
+    // and replayed in the cycles of the same shape (not while the bench
+    // times the products, whose events the graph would replay)
+    const char* ge = std::getenv("VHCI_GRAPH");
+    const bool graph_ok = dev_ortho && ge && std::string(ge) == "1" && prof_.b == 0 && m + bs <= kSvqbRows &&
+                          !sym_ && !matfree_;  // full CSR products only
+    struct CycleGraph {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        std::vector<const void*> key;
+        int64_t vectors = 0, fp64 = 0;
+        int calls = 0;
+        void reset() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            exec = nullptr;
+            graph = nullptr;
+        }
+        ~CycleGraph() { reset(); }
+    } cyc;
     if (dev_ortho) {
         // sized once, so that no buffer the queued steps use is replaced
         ostat_.ensure((size_t)(m / bs + 2) * sizeof(int));
@@ -5102,6 +5122,68 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         // the steps after it are enqueued again, checked after 1, 2, 4, ...
         // steps so that repeated refusals cost no more than the host path.
         const int* dstat = dev_ortho ? ostat_.as<int>() : nullptr;
+        // Lanczos step: H V_j has large components only along V_j and
+        // V_(j-bs) (the block-tridiagonal part of the projected matrix; the
+        // kept Ritz vectors couple to V_k alone).  Taking those out first
+        // leaves the full pass a block that is already nearly orthogonal, so
+        // one projection suffices.
+        auto local_projection = [&](int j) {
+            double* T = V + (int64_t)(j + bs) * ld;
+            const int lo = std::max(k, j - bs);
+            const int p = j + bs - lo;
+            S_.ensure((size_t)std::max(p * bs, (j + bs) * bs) * sizeof(double));
+            tsm_tn(stream_, tsm_part_, n, p, bs, V + (int64_t)lo * ld, ld, T, ld, S_.as<double>(), p);
+            tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
+        };
+        // one device step (status in slot): the product, the local
+        // projection, then one pass over [V, T] for V^T T and T^T T, the
+        // projection only when the gate is set (as in ortho_cm's Lanczos
+        // path), and the block's Cholesky QR
+        auto enqueue_step = [&](int j, int slot) {
+            apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st, V + (int64_t)(j + bs) * ld);
+            local_projection(j);
+            double* T = V + (int64_t)(j + bs) * ld;
+            const int m0 = j + bs, p = m0 + bs;
+            double* R = S_.as<double>();
+            tsm_tn(stream_, tsm_part_, n, p, bs, V, ld, T, ld, R, p);
+            k_gate_proj<<<1, 256, 0, stream_>>>(R, m0, bs, p, kOrthoSkip, R + (size_t)p * bs);
+            tsm_nn(stream_, n, m0, bs, -1.0, V, ld, R, p, 1.0, T, ld, R + (size_t)p * bs);
+            k_orth_block<<<1, 256, 0, stream_>>>(R, m0, bs, p, ozd_.as<double>(), ostat_.as<int>() + slot,
+                                                  decline_dev ? 1 : 0);
+            const unsigned rg = std::min<unsigned>(grid_for(n, 256), 2048);
+            k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, bs, bs, ozd_.as<double>(), ostat_.as<int>() + slot);
+            HIPCHK(hipGetLastError());
+        };
+        // the device steps j0, j0 + bs, ... (nsteps) from a captured graph,
+        // captured again when the cycle's shape or a buffer it uses changed
+        auto launch_graph = [&](int j0, int nsteps) {
+            const std::vector<const void*> key = {V, W, S_.p, tsm_part_.p, ozd_.p, ostat_.p, stage_in_.p,
+                                                  stage_out_.p, stage_in32_.p, X32_.p,
+                                                  reinterpret_cast<const void*>((intptr_t)j0),
+                                                  reinterpret_cast<const void*>((intptr_t)nsteps),
+                                                  reinterpret_cast<const void*>((intptr_t)(low ? 1 : 0))};
+            if (!cyc.exec || key != cyc.key) {
+                cyc.reset();
+                const int64_t spmv0 = st.spmm_vectors, calls0 = st.spmm_calls, fp640 = st.fp64_spmm_vectors;
+                HIPCHK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
+                for (int q = 0; q < nsteps; q++) enqueue_step(j0 + q * bs, q);
+                HIPCHK(hipStreamEndCapture(stream_, &cyc.graph));
+                HIPCHK(hipGraphInstantiate(&cyc.exec, cyc.graph, nullptr, nullptr, 0));
+                cyc.key = key;
+                cyc.vectors = st.spmm_vectors - spmv0;
+                cyc.calls = (int)(st.spmm_calls - calls0);
+                cyc.fp64 = st.fp64_spmm_vectors - fp640;
+                st.spmm_vectors = spmv0;
+                st.spmm_calls = (int)calls0;
+                st.fp64_spmm_vectors = (int)fp640;
+                st.graphs_captured++;
+            }
+            HIPCHK(hipGraphLaunch(cyc.exec, stream_));
+            st.spmm_vectors += cyc.vectors;
+            st.spmm_calls += cyc.calls;
+            st.fp64_spmm_vectors += (int)cyc.fp64;
+            st.graph_launches++;
+        };
         auto lanczos_steps = [&](int j0, bool host_first) -> int {
             int queued = 0;  // device steps enqueued since the last status check
             int first_dev = -1;
@@ -5121,37 +5203,18 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
                 return -1;
             };
             for (int j = j0; j < m; j += bs) {
-                // W_j = H V_j, also written to V_(j+bs) as the next block's start
-                apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st, V + (int64_t)(j + bs) * ld);
-                double* T = V + (int64_t)(j + bs) * ld;
-                if (j > k) {
-                    // Lanczos step: H V_j has large components only along V_j and
-                    // V_(j-bs) (the block-tridiagonal part of the projected
-                    // matrix; the kept Ritz vectors couple to V_k alone).  Taking
-                    // those out first leaves the full pass below a block that is
-                    // already nearly orthogonal, so one projection suffices.
-                    const int lo = std::max(k, j - bs);
-                    const int p = j + bs - lo;
-                    S_.ensure((size_t)std::max(p * bs, (j + bs) * bs) * sizeof(double));
-                    tsm_tn(stream_, tsm_part_, n, p, bs, V + (int64_t)lo * ld, ld, T, ld, S_.as<double>(), p);
-                    tsm_nn(stream_, n, p, bs, -1.0, V + (int64_t)lo * ld, ld, S_.as<double>(), p, 1.0, T, ld);
+                const bool dev = dev_ortho && j > k && j + bs <= kSvqbRows && !(host_first && j == j0);
+                if (dev && graph_ok && j == k + bs && j0 == k) {
+                    // every device step of the cycle from one captured graph
+                    const int nsteps = (m - j) / bs;
+                    launch_graph(j, nsteps);
+                    first_dev = j;
+                    queued = nsteps;
+                    break;
                 }
-                if (dev_ortho && j > k && j + bs <= kSvqbRows && !(host_first && j == j0)) {
-                    // one pass over [V, T] gives V^T T and T^T T; the
-                    // projection runs only when the gate is set (as in
-                    // ortho_cm's Lanczos path)
-                    const int m0 = j + bs, p = m0 + bs;
-                    double* R = S_.as<double>();
-                    tsm_tn(stream_, tsm_part_, n, p, bs, V, ld, T, ld, R, p);
-                    k_gate_proj<<<1, 256, 0, stream_>>>(R, m0, bs, p, kOrthoSkip, R + (size_t)p * bs);
-                    tsm_nn(stream_, n, m0, bs, -1.0, V, ld, R, p, 1.0, T, ld, R + (size_t)p * bs);
+                if (dev) {
                     if (queued == 0) first_dev = j;
-                    k_orth_block<<<1, 256, 0, stream_>>>(R, m0, bs, p, ozd_.as<double>(), ostat_.as<int>() + queued,
-                                                          decline_dev ? 1 : 0);
-                    const unsigned rg = std::min<unsigned>(grid_for(n, 256), 2048);
-                    k_rowmix<8><<<rg, 256, 0, stream_>>>(n, T, ld, bs, bs, ozd_.as<double>(),
-                                                          ostat_.as<int>() + queued);
-                    HIPCHK(hipGetLastError());
+                    enqueue_step(j, queued);
                     queued++;
                     if (queued >= batch) {
                         const int bad = check();
@@ -5162,6 +5225,9 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
                 }
                 const int bad = check();
                 if (bad >= 0) return bad;
+                // W_j = H V_j, also written to V_(j+bs) as the next block's start
+                apply_cm(V + (int64_t)j * ld, W + (int64_t)j * ld, ld, bs, low, st, V + (int64_t)(j + bs) * ld);
+                if (j > k) local_projection(j);
                 int kept = ortho_cm(V, ld, j + bs, bs, j > k);
                 if (kept < bs) refilled = true;
                 for (int tries = 0; kept < bs && tries < 4; tries++) {
@@ -5278,9 +5344,9 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         if (std::getenv("VHCI_DEBUG"))
             std::fprintf(stderr,
                          "[krylov-schur] cycle %d m %d keep %d bs %d maxrel %.3e unconverged %d %s (device steps %d, "
-                         "repeated %d)\n",
+                         "repeated %d, graphs %d captured %d launched)\n",
                          it, m, keep, bs, maxrel, unconv, low ? "fp32" : "fp64", st.device_ortho_steps,
-                         st.device_ortho_redone);
+                         st.device_ortho_redone, st.graphs_captured, st.graph_launches);
         if (unconv == 0 && !low) {
             E.assign(thh.begin(), thh.begin() + nev);
             X_.ensure((size_t)n * nev * sizeof(double));

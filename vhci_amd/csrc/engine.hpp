@@ -112,6 +112,8 @@ struct EigStats {
     int fp64_spmm_vectors = 0;
     int device_ortho_steps = 0;   // Lanczos steps orthonormalised on the device
     int device_ortho_redone = 0;  // of those, repeated on the host's general path
+    int graphs_captured = 0;      // cycles' device steps captured as a HIP graph
+    int graph_launches = 0;       // and launched from it
     double max_resid = 0;
     double ms = 0;
     bool dense = false;
