@@ -3307,6 +3307,30 @@ void Engine::rebuild_hash(int64_t n) {
 }
 
 // ---------------------------------------------------------------- H build
+// perm = the n keys' indices sorted by their quanta read as one integer,
+// the highest mode most significant (stable LSD radix sorts over the packed
+// keys, 8 modes at a time)
+void Engine::quanta_order(const uint32_t* keys, int64_t n, int32_t* perm) {
+    DevBuf kin, kout, pout;
+    kin.ensure(n * sizeof(uint64_t));
+    kout.ensure(n * sizeof(uint64_t));
+    pout.ensure(n * sizeof(int32_t));
+    k_iota32<<<grid_for(n, 256), 256, 0, stream_>>>(perm, n);
+    const int nchunk = (P_.M + 7) / 8;
+    for (int c = 0; c < nchunk; c++) {
+        k_key_chunk<<<grid_for(n, 256), 256, 0, stream_>>>(keys, 4 * KW_, perm, n, c, kin.as<uint64_t>());
+        const unsigned endbit = 8u * (unsigned)std::min(8, P_.M - 8 * c);
+        size_t tmp = 0;
+        HIPCHK(rocprim::radix_sort_pairs(nullptr, tmp, kin.as<uint64_t>(), kout.as<uint64_t>(), perm,
+                                         pout.as<int32_t>(), (size_t)n, 0, endbit, stream_));
+        scan_tmp_.ensure(tmp);
+        HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, tmp, kin.as<uint64_t>(), kout.as<uint64_t>(), perm,
+                                         pout.as<int32_t>(), (size_t)n, 0, endbit, stream_));
+        HIPCHK(hipMemcpyAsync(perm, pout.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
+    }
+    HIPCHK(hipGetLastError());
+}
+
 void Engine::make_solver_order() {
     const int64_t n = n_;
     const char* e = std::getenv("VHCI_REORDER");
@@ -3316,25 +3340,7 @@ void Engine::make_solver_order() {
     if (!perm_on_) return;
     perm_.ensure(n * sizeof(int32_t));
     inv_.ensure(n * sizeof(int32_t));
-    DevBuf kin, kout, pout;
-    kin.ensure(n * sizeof(uint64_t));
-    kout.ensure(n * sizeof(uint64_t));
-    pout.ensure(n * sizeof(int32_t));
-    k_iota32<<<grid_for(n, 256), 256, 0, stream_>>>(perm_.as<int32_t>(), n);
-    // least significant chunk first; each radix sort is stable
-    const int nchunk = (P_.M + 7) / 8;
-    for (int c = 0; c < nchunk; c++) {
-        k_key_chunk<<<grid_for(n, 256), 256, 0, stream_>>>(keys_.as<uint32_t>(), 4 * KW_, perm_.as<int32_t>(), n, c,
-                                                           kin.as<uint64_t>());
-        const unsigned endbit = 8u * (unsigned)std::min(8, P_.M - 8 * c);
-        size_t tmp = 0;
-        HIPCHK(rocprim::radix_sort_pairs(nullptr, tmp, kin.as<uint64_t>(), kout.as<uint64_t>(), perm_.as<int32_t>(),
-                                         pout.as<int32_t>(), (size_t)n, 0, endbit, stream_));
-        scan_tmp_.ensure(tmp);
-        HIPCHK(rocprim::radix_sort_pairs(scan_tmp_.p, tmp, kin.as<uint64_t>(), kout.as<uint64_t>(),
-                                         perm_.as<int32_t>(), pout.as<int32_t>(), (size_t)n, 0, endbit, stream_));
-        HIPCHK(hipMemcpyAsync(perm_.p, pout.p, n * sizeof(int32_t), hipMemcpyDeviceToDevice, stream_));
-    }
+    quanta_order(keys_.as<uint32_t>(), n, perm_.as<int32_t>());
     k_invert32<<<grid_for(n, 256), 256, 0, stream_>>>(perm_.as<int32_t>(), n, inv_.as<int32_t>());
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream_));

@@ -244,6 +244,7 @@ class Engine {
     mutable DevPool pool_;
     void upload_tables();
     void make_solver_order();
+    void quanta_order(const uint32_t* keys, int64_t n, int32_t* perm);
     void rebuild_hash(int64_t cap_hint);
     void spmm(const double* X, int ldx, double* Y, int ldy, int b, bool low, const float* X32 = nullptr,
               int ldx32 = 0);
