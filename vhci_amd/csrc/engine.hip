@@ -647,6 +647,37 @@ __device__ __forceinline__ uint64_t pattern_shift(uint64_t e, int nz, const uint
     return h;
 }
 
+// One walk step's candidate, prepared a step ahead so that its Bloom word is
+// loading while the step before is tested and probed: ok when the pattern
+// keeps every quantum in 0..255 (and, with a tag range, its hash tag lies in
+// it), H its moved key's hash, bw the Bloom word to test.
+struct WalkCand {
+    uint64_t e, H, bw;
+    bool ok;
+};
+__device__ __forceinline__ WalkCand walk_cand(uint64_t e, bool in, const uint8_t* __restrict__ sq, uint64_t h0,
+                                              const uint64_t* __restrict__ cf, const uint32_t* __restrict__ bloom,
+                                              uint64_t bmask, uint64_t tlo = 0, uint64_t thi = ~0ull) {
+    WalkCand c{e, 0ull, 0ull, false};
+    if (in) {
+        const int nz = delta_nz(e);
+        if (pattern_ok(e, nz, sq)) {
+            c.H = fmix64(h0 + pattern_shift(e, nz, cf));
+            const uint64_t tag = c.H >> 32;
+            if (tag >= tlo && tag < thi) {
+                c.ok = true;
+                c.bw = reinterpret_cast<const uint64_t*>(bloom)[bloom_word(c.H, bmask)];
+            }
+        }
+    }
+    return c;
+}
+__device__ __forceinline__ bool walk_maybe(const WalkCand& c) {
+    const uint64_t b = bloom_bits2(c.H);
+    return c.ok && (c.bw & b) == b;
+}
+
+
 // the same for a heat-bath variant (d modes at bytes 1..6, deltas at 7..12)
 __device__ __forceinline__ uint64_t variant_shift(const uint4 v, int d, const uint64_t* __restrict__ cf) {
     uint64_t h = 0;
@@ -728,7 +759,13 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
     const bool known = FILL && hw;  // fill pass over recorded hits
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
     // each lane's next pattern record is loaded one step ahead
-    uint64_t en = !known && pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull;
+    // the current step's candidate and the next step's record
+    WalkCand cur{};
+    uint64_t en = 0ull;
+    if (!known) {
+        cur = walk_cand(pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull, pb0 + lane < pb1, sq, h0, cf, a.bloom, a.bmask);
+        en = pb0 + kWave + lane < pb1 ? a.walk[pb0 + kWave + lane] : 0ull;
+    }
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
         bool hit = false;
@@ -744,18 +781,15 @@ __global__ __launch_bounds__(64) void k_hbuild(HBuildArgs a) {
                 hit = j >= 0;
             }
         } else {
-            const uint64_t e = en;
-            if (p + kWave < pb1) en = a.walk[p + kWave];
-            if (p < pb1) {
-                const int nz = delta_nz(e);
-                if (pattern_ok(e, nz, sq)) {
-                    const uint64_t H = fmix64(h0 + pattern_shift(e, nz, cf));
-                    if (bloom_maybe(a.bloom, a.bmask, H)) {
-                        j = probe_shift<NW>(a.slots, a.mask, a.keys, w, e, nz, H);
-                        hit = j >= 0;
-                    }
-                }
+            // the next step's candidate first, so its Bloom word loads while
+            // this one is probed
+            const WalkCand nxt = walk_cand(en, p + kWave < pb1, sq, h0, cf, a.bloom, a.bmask);
+            en = p + 2 * kWave < pb1 ? a.walk[p + 2 * kWave] : 0ull;
+            if (walk_maybe(cur)) {
+                j = probe_shift<NW>(a.slots, a.mask, a.keys, w, cur.e, delta_nz(cur.e), cur.H);
+                hit = j >= 0;
             }
+            cur = nxt;
         }
         const uint64_t bal = __ballot(hit);
         if (!FILL && hw) {
@@ -2755,24 +2789,22 @@ __global__ __launch_bounds__(64) void k_pt2_push1(Pt2PushArgs a, unsigned long l
     int blk = -1, pb0 = 0, pb1 = 0;
     while (next_block(a.poff, a.M, sq, blk, pb0, pb1)) {
     // each lane's next pattern record is loaded one step ahead
-    uint64_t en = pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull;
+    // the current step's candidate (its Bloom word loading a step ahead)
+    // and the next step's record
+    WalkCand cur = walk_cand(pb0 + lane < pb1 ? a.walk[pb0 + lane] : 0ull, pb0 + lane < pb1, sq, h0, cf, a.abloom,
+                             a.abmask, a.tlo, a.thi);
+    uint64_t en = pb0 + kWave + lane < pb1 ? a.walk[pb0 + kWave + lane] : 0ull;
     for (int p0 = pb0; p0 < pb1; p0 += kWave) {
         const int p = p0 + lane;
-        const uint64_t e = en;
-        if (p + kWave < pb1) en = a.walk[p + kWave];
+        const WalkCand nxt = walk_cand(en, p + kWave < pb1, sq, h0, cf, a.abloom, a.abmask, a.tlo, a.thi);
+        en = p + 2 * kWave < pb1 ? a.walk[p + 2 * kWave] : 0ull;
         bool hit = false;
         int64_t idx = -1;
-        if (p < pb1) {
-            const int nz = delta_nz(e);
-            if (pattern_ok(e, nz, sq)) {
-                const uint64_t h = fmix64(h0 + pattern_shift(e, nz, cf));
-                const uint64_t tag = h >> 32;
-                if (tag >= a.tlo && tag < a.thi && bloom_maybe(a.abloom, a.abmask, h)) {
-                    idx = probe_shift<NW>(a.aslots, a.amask, a.apool, w, e, nz, h);
-                    hit = idx >= 0;
-                }
-            }
+        if (walk_maybe(cur)) {
+            idx = probe_shift<NW>(a.aslots, a.amask, a.apool, w, cur.e, delta_nz(cur.e), cur.H);
+            hit = idx >= 0;
         }
+        cur = nxt;
         const uint64_t bal = __ballot(hit);
         const int nh = __popcll(bal);
         if (nh == 0) continue;
