@@ -171,22 +171,33 @@ std::string parse_input(std::istream& in, bool restart, Problem& P) {
         }
         if (std::abs(t.fc) > 1e-8) P.hb.push_back(t);
     }
+    // double terms: a quartic with a cubed mode adds 3 f to the pair of its
+    // two modes, one with two single modes 2 f to the pair of those.  Each
+    // quartic adds to its one pair, in list order, so every pair's sum has
+    // the order of a scan of the list per pair.
+    std::vector<double> pair_fc((size_t)M * M, 0.0);
+    for (int id : P.quartic) {
+        const ForceConst& c = P.fcs[id];
+        if (c.sm.size() == 2 && has(c.mp, 3)) {
+            const int a = std::min(c.sm[0], c.sm[1]), b = std::max(c.sm[0], c.sm[1]);
+            pair_fc[(size_t)a * M + b] += 3 * c.fc;
+        } else if (c.sm.size() == 3) {
+            int ones[3], k = 0;
+            for (int t = 0; t < 3; t++)
+                if (c.mp[t] == 1) ones[k++] = c.sm[t];
+            if (k == 2) {
+                const int a = std::min(ones[0], ones[1]), b = std::max(ones[0], ones[1]);
+                pair_fc[(size_t)a * M + b] += 2 * c.fc;
+            }
+        }
+    }
     for (int m = 0; m + 1 < M; m++)
         for (int n = m + 1; n < M; n++) {
             ForceConst t;
             t.fcpow = {m, n};
             t.sm = {m, n};
             t.mp = {1, 1};
-            for (int id : P.quartic) {
-                const ForceConst& c = P.fcs[id];
-                bool hm = has(c.sm, m), hn = has(c.sm, n);
-                if (c.sm.size() == 2 && has(c.mp, 3) && hm && hn) t.fc += 3 * c.fc;
-                if (c.sm.size() == 3 && hm && hn) {
-                    int im = (int)(std::find(c.sm.begin(), c.sm.end(), m) - c.sm.begin());
-                    int in_ = (int)(std::find(c.sm.begin(), c.sm.end(), n) - c.sm.begin());
-                    if (c.mp[im] == 1 && c.mp[in_] == 1) t.fc += 2 * c.fc;
-                }
-            }
+            t.fc = pair_fc[(size_t)m * M + n];
             if (std::abs(t.fc) > 1e-8) P.hb.push_back(t);
         }
     // HeatBath_Sort_FC (HB.cpp:17-22), stable so ties keep construction order
