@@ -2119,7 +2119,7 @@ __global__ __launch_bounds__(256) void k_rowmix(int64_t n, double* __restrict__ 
 
 // the largest basis offset m0 that k_orth_block stages in LDS (8 x 1024
 // doubles, 64 KB); steps beyond it take the host path
-constexpr int kSvqbRows = 1024;
+constexpr int kOrthRows = 1024;
 
 // One Lanczos block's orthonormalisation finished on the device: the common
 // case of Engine::ortho_cm's Lanczos path, so that a cycle's steps need no
@@ -2133,7 +2133,8 @@ constexpr int kSvqbRows = 1024;
 // to the projection, or the Frobenius condition bound |DGD| |(DGD)^-1| of
 // the scaled Gram matrix reaches 1e4, which covers the host's test that its
 // eigenvalues lie within a factor 1e4): then Z is not written, and the host
-// repeats the step on its general path (SVQB).
+// repeats the step on its general path (SVQB).  decline != 0 declines every
+// block (a test of that path).
 __global__ __launch_bounds__(256) void k_orth_block(const double* __restrict__ R, int m0, int k, int p,
                                                     double* __restrict__ Z, int* __restrict__ status,
                                                     int decline) {
@@ -2143,7 +2144,7 @@ __global__ __launch_bounds__(256) void k_orth_block(const double* __restrict__ R
     __shared__ double inv2[64];
     __shared__ double gf2;  // squared Frobenius norm of the scaled Gram matrix
     __shared__ int stop;
-    __shared__ double Rs[8 * kSvqbRows];  // the projection coefficients S (m0 x k), column by column
+    __shared__ double Rs[8 * kOrthRows];  // the projection coefficients S (m0 x k), column by column
     const int t = threadIdx.x;
     if (decline) {  // every step to the host's general path (a test of that path)
         if (t == 0) *status = 1;
@@ -5119,7 +5120,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
     // and replayed in the cycles of the same shape (not while the bench
     // times the products, whose events the graph would replay)
     const char* ge = std::getenv("VHCI_GRAPH");
-    const bool graph_ok = dev_ortho && ge && std::string(ge) == "1" && prof_.b == 0 && m + bs <= kSvqbRows &&
+    const bool graph_ok = dev_ortho && ge && std::string(ge) == "1" && prof_.b == 0 && m + bs <= kOrthRows &&
                           !sym_ && !matfree_;  // full CSR products only
     struct CycleGraph {
         hipGraph_t graph = nullptr;
@@ -5241,7 +5242,7 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
                 return -1;
             };
             for (int j = j0; j < m; j += bs) {
-                const bool dev = dev_ortho && j > k && j + bs <= kSvqbRows && !(host_first && j == j0);
+                const bool dev = dev_ortho && j > k && j + bs <= kOrthRows && !(host_first && j == j0);
                 if (dev && graph_ok && j == k + bs && j0 == k) {
                     // every device step of the cycle from one captured graph
                     const int nsteps = (m - j) / bs;
