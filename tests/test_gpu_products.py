@@ -154,3 +154,30 @@ def test_products_half_storage(grown_text, form, monkeypatch):
     y2 = ctx.apply_h(X)
     assert np.array_equal(y1, y2)
     ctx.close()
+
+
+def test_layout_knobs_leave_results_unchanged(monkeypatch):
+    """The performance choices change where work runs, not what it computes:
+    XCD row runs in the products (VHCI_XCD_RUN_ROWS=0 against the default),
+    the count pass's hit words (VHCI_HIT_WORDS=0) and the Bloom filter size
+    (VHCI_BLOOM_BITS=2) must give the same H, products, basis and energies
+    bit for bit on the 15-mode workload."""
+    text = synth.config_input(1, pt2=False)
+    runs = {}
+    for name, env in (("default", {}), ("plain", {"VHCI_XCD_RUN_ROWS": "0", "VHCI_HIT_WORDS": "0",
+                                                   "VHCI_BLOOM_BITS": "2"})):
+        for k in ("VHCI_XCD_RUN_ROWS", "VHCI_HIT_WORDS", "VHCI_BLOOM_BITS"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        ctx = vhci_amd.Context(vhci_amd.Problem(text=text), vhci_amd.FP64)
+        r = ctx.run()
+        rp, col, val, diag = ctx.get_h()
+        X = np.random.default_rng(3).standard_normal((ctx.n, 8))
+        runs[name] = (r, rp, col, val, diag, ctx.apply_h(X), ctx.apply_h(X[:, 0]))
+        ctx.close()
+    a, b = runs["default"], runs["plain"]
+    assert a[0]["checksum"] == b[0]["checksum"]
+    assert np.array_equal(a[0]["E_var"], b[0]["E_var"])
+    for x, y in zip(a[1:], b[1:]):
+        assert np.array_equal(x, y)
