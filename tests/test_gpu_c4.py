@@ -94,3 +94,38 @@ def test_c4_mixed_and_chunked_pt2(c4, monkeypatch):
           flush=True)
     assert chunks > 1 and rc["ns2"] == r["ns2"] and rc["checksum"] == r["checksum"]
     assert np.max(np.abs(rc["E"] - r["E"])) < 1e-9 * np.max(np.abs(r["E"]))
+
+
+def test_c4_half_storage(c4, monkeypatch):
+    """Half storage (upper triangle, transposed products through the
+    fixed-point accumulator) selects the same basis and gives the same
+    energies within 1e-6 cm^-1."""
+    ctx, r, _, text = c4
+    monkeypatch.setenv("VHCI_H_STORAGE", "half")
+    c = vhci_amd.Context(vhci_amd.Problem(text=text), vhci_amd.FP64)
+    rs = c.run()
+    c.close()
+    err = float(np.max(np.abs(rs["E"] - r["E"])))
+    print("c4 half storage: timings %s, max |dE| %.3e cm^-1" % (rs["timings"], err), flush=True)
+    assert rs["checksum"] == r["checksum"] and rs["ns2"] == r["ns2"]
+    assert err < 1e-6
+
+
+def test_c4_matrix_free(monkeypatch):
+    """Matrix-free products (every product regenerates the quintic and
+    sextic terms from the hash) against the stored form, at a higher
+    threshold (eps1 10) that keeps the matrix-free solve short: the same
+    basis and energies within 1e-6 cm^-1."""
+    text = synth.config_input(4, eps1=10.0, nstate=20, eps2=0.5)
+    out = {}
+    for storage in ("full", "free"):
+        monkeypatch.setenv("VHCI_H_STORAGE", storage)
+        c = vhci_amd.Context(vhci_amd.Problem(text=text), vhci_amd.FP64)
+        out[storage] = c.run()
+        out[storage]["sizes"] = c.iteration_sizes()
+        c.close()
+    err = float(np.max(np.abs(out["free"]["E"] - out["full"]["E"])))
+    print("c4 matrix-free (eps1 10): sizes %s, timings %s, max |dE| %.3e cm^-1"
+          % (out["free"]["sizes"], out["free"]["timings"], err), flush=True)
+    assert out["free"]["checksum"] == out["full"]["checksum"] and out["free"]["ns2"] == out["full"]["ns2"]
+    assert err < 1e-6
