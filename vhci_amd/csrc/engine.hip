@@ -1916,27 +1916,42 @@ __global__ __launch_bounds__(256) void k_tsm_reduce(const double* __restrict__ p
 // Y = beta Y + alpha A S: A n x p (lda), S p x q (lds, col-major) staged in
 // LDS (p*q <= 4096), one row per thread, q <= 16 per launch tile.  gate
 // (null: none): nothing is done when *gate is 0.
+template <int QW, int U>
 __global__ __launch_bounds__(256) void k_tsm_nn(int64_t n, int p, int q, double alpha, const double* __restrict__ A,
                                                 int64_t lda, const double* __restrict__ S, int lds, double beta,
                                                 double* __restrict__ Y, int64_t ldy,
                                                 const double* __restrict__ gate) {
-    __shared__ double Ss[4096];
+    // QW output columns per workgroup (p <= 256 rows of S in LDS); U columns
+    // of A loaded before they are used, so that each wave keeps U loads in
+    // flight.  Every output sums a = 0, 1, ... in order, whatever U.
+    __shared__ double Ss[256 * QW];
     if (gate && *gate == 0.0) return;
-    const int c0 = blockIdx.y * 16;
-    const int qq = min(16, q - c0);
-    for (int idx = threadIdx.x; idx < p * qq; idx += blockDim.x) {
-        const int a = idx / qq, c = idx - a * qq;
-        Ss[a * 16 + c] = S[a + (int64_t)(c0 + c) * lds];
+    const int c0 = blockIdx.y * QW;
+    const int qq = min(QW, q - c0);
+    for (int idx = threadIdx.x; idx < p * QW; idx += blockDim.x) {
+        const int a = idx / QW, c = idx - a * QW;
+        Ss[idx] = c < qq ? S[a + (int64_t)(c0 + c) * lds] : 0.0;
     }
     __syncthreads();
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        double acc[16];
+        double acc[QW];
 #pragma unroll
-        for (int c = 0; c < 16; c++) acc[c] = 0.0;
-        for (int a = 0; a < p; a++) {
-            const double x = A[r + (int64_t)a * lda];
+        for (int c = 0; c < QW; c++) acc[c] = 0.0;
+        const double* Ar = A + r;
+        int a = 0;
+        for (; a + U <= p; a += U) {
+            double x[U];
 #pragma unroll
-            for (int c = 0; c < 16; c++) acc[c] = fma(x, Ss[a * 16 + c], acc[c]);
+            for (int u = 0; u < U; u++) x[u] = Ar[(int64_t)(a + u) * lda];
+#pragma unroll
+            for (int u = 0; u < U; u++)
+#pragma unroll
+                for (int c = 0; c < QW; c++) acc[c] = fma(x[u], Ss[(a + u) * QW + c], acc[c]);
+        }
+        for (; a < p; a++) {
+            const double x = Ar[(int64_t)a * lda];
+#pragma unroll
+            for (int c = 0; c < QW; c++) acc[c] = fma(x, Ss[a * QW + c], acc[c]);
         }
         for (int c = 0; c < qq; c++) {
             double* y = Y + r + (int64_t)(c0 + c) * ldy;
@@ -4572,12 +4587,51 @@ EigStats Engine::dense_all(std::vector<double>& E) {
     return st;
 }
 
+// workgroups of 256 threads of `kernel` that the current device holds at
+// once (all CUs), kept per device and kernel
+static int resident_groups(const void* kernel) {
+    static std::mutex mu;
+    static std::map<std::pair<int, const void*>, int> memo;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = memo.find({dev, kernel});
+    if (it != memo.end()) return it->second;
+    int per_cu = 0, cus = 0;
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int r = std::max(1, per_cu) * std::max(1, cus);
+    memo[{dev, kernel}] = r;
+    return r;
+}
+
 // C (p x q, col-major, ldc) = A^T B for column-major A (n x p), B (n x q)
 static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const double* A, int64_t lda,
                    const double* B, int64_t ldb, double* C, int ldc) {
-    // about 384 chunks of rows (>= 256 rows each): enough blocks for the CUs,
-    // few partial sums to combine
-    const int64_t rpc = std::max<int64_t>(256, ((n + 383) / 384 + 31) / 32 * 32);
+    // Row chunks (>= 256 rows each) times output tiles make the grid.  The
+    // chunk count is set so that the grid is one wave of the workgroups the
+    // device holds at once: every workgroup does the same work, so a second,
+    // partly filled wave would cost as long as the first (384 chunks of the
+    // 128-column tiles of a 296-column basis are 1,152 workgroups against
+    // 1,024 resident).  At most 512 chunks, which k_tsm_reduce sums per
+    // output.  VHCI_TSM_WAVES=k: k waves; 0: 384 chunks whatever the tiles.
+    const void* kern;
+    int tiles;
+    if (q <= 8) {
+        kern = reinterpret_cast<const void*>(&k_tsm_tn<128, 8>);
+        tiles = (p + 127) / 128;
+    } else if (q > 16 && p > 16) {
+        kern = reinterpret_cast<const void*>(&k_tsm_tn_wide);
+        tiles = ((p + 63) / 64) * ((q + 63) / 64);
+    } else {
+        kern = reinterpret_cast<const void*>(&k_tsm_tn<64, 16>);
+        tiles = ((p + 63) / 64) * ((q + 15) / 16);
+    }
+    const char* we = std::getenv("VHCI_TSM_WAVES");
+    const int waves = we ? std::max(0, std::atoi(we)) : 1;
+    const int64_t want =
+        waves == 0 ? 384 : std::clamp<int64_t>((int64_t)waves * resident_groups(kern) / tiles, 1, 512);
+    const int64_t rpc = std::max<int64_t>(256, ((n + want - 1) / want + 31) / 32 * 32);
     const int nchunk = (int)((n + rpc - 1) / rpc);
     part.ensure((size_t)nchunk * p * q * sizeof(double));
     if (q <= 8) {
@@ -4598,11 +4652,19 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
 // device, null: none): skipped when *gate is 0
 static void tsm_nn(hipStream_t s, int64_t n, int p, int q, double alpha, const double* A, int64_t lda,
                    const double* S, int lds, double beta, double* Y, int64_t ldy, const double* gate = nullptr) {
+    // the eigensolver's blocks (q = 8) take the 8-column form: half the
+    // FMAs and LDS of the 16-column one, and 8 loads in flight per wave
+    // (15-mode eigensolver 799 -> 780 ms, the same results bit for bit)
     for (int a0 = 0; a0 < p; a0 += 256) {
         const int pp = std::min(256, p - a0);
-        dim3 g(std::min<unsigned>(grid_for(n, 256), 2048), (q + 15) / 16);
-        k_tsm_nn<<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, a0 == 0 ? beta : 1.0, Y, ldy,
-                                   gate);
+        const double b = a0 == 0 ? beta : 1.0;
+        if (q <= 8) {
+            dim3 g(std::min<unsigned>(grid_for(n, 256), 2048), 1);
+            k_tsm_nn<8, 8><<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, b, Y, ldy, gate);
+        } else {
+            dim3 g(std::min<unsigned>(grid_for(n, 256), 2048), (q + 15) / 16);
+            k_tsm_nn<16, 4><<<g, 256, 0, s>>>(n, pp, q, alpha, A + a0 * lda, lda, S + a0, lds, b, Y, ldy, gate);
+        }
     }
     HIPCHK(hipGetLastError());
 }
