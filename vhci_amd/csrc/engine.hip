@@ -2213,6 +2213,72 @@ __global__ __launch_bounds__(256) void k_orth_block(const double* __restrict__ R
         A[c][i] *= dsc[i] * dsc[c];
     }
     __syncthreads();
+    if (k == 8) {
+        // The eigensolver's block size: the same steps as below, in the same
+        // order, by one lane on registers (fully unrolled), since the serial
+        // chain through LDS cost most of the kernel's time.
+        if (t == 0) {
+            double a[8][8], li[8][8];
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+#pragma unroll
+                for (int i = 0; i < 8; i++) a[c][i] = A[c][i];
+            double f2 = 0.0;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+#pragma unroll
+                for (int i = 0; i < 8; i++) f2 += a[c][i] * a[c][i];
+            int bad = 0;
+#pragma unroll
+            for (int c = 0; c < 8; c++) {
+                double dd = a[c][c];
+#pragma unroll
+                for (int r = 0; r < c; r++) dd -= a[r][c] * a[r][c];
+                if (!(dd > 0.0)) bad = 1;  // nothing computed after this is used
+                const double l = sqrt(dd);
+                a[c][c] = l;
+#pragma unroll
+                for (int i = c + 1; i < 8; i++) {
+                    double v = a[c][i];
+#pragma unroll
+                    for (int r = 0; r < c; r++) v -= a[r][i] * a[r][c];
+                    a[c][i] = v / l;
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+#pragma unroll
+                for (int r = 0; r < 8; r++) {
+                    if (r < c) {
+                        li[r][c] = 0.0;
+                        continue;
+                    }
+                    double v = r == c ? 1.0 : 0.0;
+#pragma unroll
+                    for (int q = c; q < r; q++) v -= a[q][r] * li[q][c];
+                    li[r][c] = v / a[r][r];
+                }
+            double i2 = 0.0;
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) {
+                    double v = 0.0;
+#pragma unroll
+                    for (int r = 0; r < 8; r++) v += li[r][i] * li[r][j];
+                    i2 += v * v;
+                }
+            // cond_2 <= |DGD|_F |(DGD)^-1|_F
+            const int st = bad || !(sqrt(f2 * i2) < 1e4);
+            *status = st;
+            if (!st)
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+#pragma unroll
+                    for (int i = 0; i < 8; i++) Z[c * 8 + i] = dsc[i] * li[c][i];
+        }
+        return;
+    }
     if (t == 0) {
         double f2 = 0.0;
         for (int c = 0; c < k; c++)
