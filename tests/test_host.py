@@ -215,3 +215,16 @@ def test_published_run_inputs():
         assert p.eps1 == float(eps1) and p.neig == int(nstate) and p.perturb == (pt2 == "1")
         assert p.eps2 == (float(eps2) if pt2 == "1" else 0.0)
         assert p.ntot == 2 and p.n0 == math.comb(p.M + 2, 2)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: a request for more exits with status 2 and
+    says so, before it touches a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120, cwd=ROOT)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr and "--gpus 2" in r.stderr
+    assert r.stdout == ""
