@@ -6,6 +6,13 @@
  * check the GPU library; the product never links, loads or calls it.  Only
  * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it.
  *
+ * Parity unpinned (the reference's own files cannot fix these, so the tests
+ * compare them order-insensitively or within a tolerance): the row order of
+ * the states heat bath adds (the reference iterates a hashed unordered_set;
+ * here the first-discovery order of its loop), the bits of eigenvectors and
+ * with them of Cmax (Spectra's), and the order-dependent bits of the PT2 sum
+ * (the reference sums with omp atomic).
+ *
  * Quanta are passed as int32 rows (N x M, row-major).  Eigenvector blocks are
  * row-major N x ncol.  All functions return 0 on success and a negative code
  * on error; orc_error() returns the message of the last error on this thread.
