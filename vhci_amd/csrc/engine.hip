@@ -4671,9 +4671,8 @@ static int resident_groups(const void* kernel) {
     return r;
 }
 
-// C (p x q, col-major, ldc) = A^T B for column-major A (n x p), B (n x q)
-static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const double* A, int64_t lda,
-                   const double* B, int64_t ldb, double* C, int ldc) {
+// rows per chunk of tsm_tn's partial products for an n x p by n x q product
+static int64_t tsm_rows_per_chunk(int64_t n, int p, int q) {
     // Row chunks (>= 256 rows each) times output tiles make the grid.  The
     // chunk count is set so that the grid is one wave of the workgroups the
     // device holds at once: every workgroup does the same work, so a second,
@@ -4697,7 +4696,19 @@ static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const d
     const int waves = we ? std::max(0, std::atoi(we)) : 1;
     const int64_t want =
         waves == 0 ? 384 : std::clamp<int64_t>((int64_t)waves * resident_groups(kern) / tiles, 1, 512);
-    const int64_t rpc = std::max<int64_t>(256, ((n + want - 1) / want + 31) / 32 * 32);
+    return std::max<int64_t>(256, ((n + want - 1) / want + 31) / 32 * 32);
+}
+
+// bytes of tsm_tn's partial products for an n x p by n x q product
+static size_t tsm_part_bytes(int64_t n, int p, int q) {
+    const int64_t rpc = tsm_rows_per_chunk(n, p, q);
+    return (size_t)((n + rpc - 1) / rpc) * p * q * sizeof(double);
+}
+
+// C (p x q, col-major, ldc) = A^T B for column-major A (n x p), B (n x q)
+static void tsm_tn(hipStream_t s, DevBuf& part, int64_t n, int p, int q, const double* A, int64_t lda,
+                   const double* B, int64_t ldb, double* C, int ldc) {
+    const int64_t rpc = tsm_rows_per_chunk(n, p, q);
     const int nchunk = (int)((n + rpc - 1) / rpc);
     part.ensure((size_t)nchunk * p * q * sizeof(double));
     if (q <= 8) {
@@ -5269,6 +5280,10 @@ void Engine::krylov_schur(int nev, double tol, int maxit, std::vector<double>& E
         ostat_.ensure((size_t)(m / bs + 2) * sizeof(int));
         ozd_.ensure(64 * sizeof(double));
         S_.ensure(((size_t)(m + 2 * bs) * bs + 1) * sizeof(double));
+        // the Gram's partial products for every width a step projects on
+        size_t part_max = 0;
+        for (int p = 1; p <= m + 2 * bs; p++) part_max = std::max(part_max, tsm_part_bytes(n, p, bs));
+        tsm_part_.ensure(part_max);
     }
     const double eps23 = std::pow(std::numeric_limits<double>::epsilon(), 2.0 / 3.0);
     const int rpb = 8192;
